@@ -1,0 +1,131 @@
+"""Pins init_walkers, the stretch move and its accounting in the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+W_TEST = 44 * 256  # amd_gcn_test.clj:162
+
+
+def _uniform(seed=123):
+    s = O.Stretch(O.UNIFORM_LOGPDF, 1, W_TEST, [-1, 2], seed, params_len=2)
+    s.init(seed)
+    s.init_position(seed, [-1, 2])
+    return s
+
+
+def test_uniform_stretch_sample_goldens(goldens):
+    s = _uniform()
+    for row in goldens["G3_uniform_stretch_sample"]["rows"]:
+        assert s.sample()[:4, 0].tolist() == row
+
+
+def test_manual_bare_half_steps_and_accu(goldens):
+    # amd_gcn_test.clj:210-254: two bare pairs (steps 0 and 1), then one accu
+    # pair with seeds 123/124, constants 1111/2222, step 0
+    rows = goldens["G3_uniform_stretch_sample"]["rows"]
+    s = _uniform()
+    s.move_bare()
+    assert s.xs[:4].tolist() == rows[0]
+    s.move_bare()
+    assert s.xs[:4].tolist() == rows[1]
+    g = goldens["G5_stretch_accu"]
+    groups = s.K // 256
+    means_acc = np.zeros(groups, np.float32)
+    s.move_seed = g["seed_odd"] - 2  # init-move! adds 2 (amd_gcn.clj:330)
+    s.init_move(means_acc, g["a"])
+    s.move()
+    assert s.xs[:4].tolist() == g["xs_first4"]
+    assert means_acc[:10].tolist() == np.float32(g["means_acc_first10"]).tolist()
+    assert s.accept[:10].tolist() == g["accept_first10"]
+    # sum_means_vertical over the 22 groups; the reference then sums on the
+    # device with a tree of its own choice, hence a 2-ulp tolerance
+    assert float(O.tree_sum_f32(means_acc)) == pytest.approx(g["sum_acc"], abs=2 * 4.9e-4)
+
+
+def test_gaussian_stretch_sample_goldens(goldens):
+    g = goldens["G6_gaussian_stretch_sample"]
+    s = O.Stretch(O.GAUSSIAN_MCMC_LOGPDF, 1, g["walkers"], g["params"], g["seed"], params_len=2)
+    s.init(g["seed"])
+    s.init_position(g["seed"], g["limits"])
+    for row in g["rows"]:
+        assert s.sample()[:4, 0].tolist() == row
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_planted_contraction_mistakes_fail_goldens(goldens, variant):
+    """Planting an un-fused z (bit 0) or Y (bit 1) in the oracle must break
+    the uniform or Gaussian goldens: the contraction choice is pinned."""
+    try:
+        O.set_variant(variant)
+        s = _uniform()
+        got = [s.sample()[:4, 0].tolist() for _ in range(4)]
+        g = goldens["G6_gaussian_stretch_sample"]
+        t = O.Stretch(O.GAUSSIAN_MCMC_LOGPDF, 1, g["walkers"], g["params"], g["seed"], params_len=2)
+        t.init_position(g["seed"], g["limits"])
+        got2 = [t.sample()[:4, 0].tolist() for _ in range(3)]
+    finally:
+        O.set_variant(0)
+    assert got != goldens["G3_uniform_stretch_sample"]["rows"] or got2 != g["rows"]
+
+
+def test_walker_count_rule():
+    # amd_gcn.clj:516, 566-567
+    with pytest.raises(ValueError, match="multiple of 512"):
+        O.Stretch(O.UNIFORM_LOGPDF, 1, 768, [-1, 2], 1, params_len=2)
+    with pytest.raises(ValueError):
+        O.Stretch(O.UNIFORM_LOGPDF, 1, 256, [-1, 2], 1, params_len=2)
+
+
+def test_init_walkers_layout_and_limits():
+    dim, W = 3, 512
+    lim = np.array([[-1, 1], [10, 20], [-5, -4]], np.float32)
+    xs = O.init_walkers(7, lim, dim, W).reshape(W, dim)
+    for d in range(dim):
+        assert (xs[:, d] > lim[d, 0]).all() and (xs[:, d] < lim[d, 1]).all()
+    # coordinate i of the flat array takes limits row i % dim (stretch.cl:143-146)
+    r = O.philox4x32_10([0, 0xF00DCAFE, 0xDEADBEEF, 0xBEEFF00D], [7, 0xDECAFAAA])
+    u = np.float32(O.u01(int(r[1])))
+    expect = np.float32(np.float64(u) * 20.0 + np.float64(np.float32((np.float32(1) - u) * np.float32(10))))
+    assert xs[0, 1] == expect
+
+
+def test_partner_modes_agree_for_dim1():
+    a = O.Stretch(O.GAUSSIAN_MCMC_LOGPDF, 1, 1024, [0, 1], 5, params_len=2, partner=O.PARTNER_FLAT)
+    b = O.Stretch(O.GAUSSIAN_MCMC_LOGPDF, 1, 1024, [0, 1], 5, params_len=2, partner=O.PARTNER_WALKER)
+    for s in (a, b):
+        s.init_position(5, [-3, 3])
+        s.burn_in(20, 2.0)
+    assert np.array_equal(a.xs, b.xs)
+
+
+def test_gaussian_burn_in_statistics(goldens):
+    # amd_gcn_test.clj:274-291: trajectory-dependent values, checked loosely;
+    # the exact path depends on the device's native_exp in the accept test
+    g = goldens["G7_gaussian_burn_in"]
+    s = O.Stretch(O.GAUSSIAN_MCMC_LOGPDF, 1, W_TEST, g["params"], 123, params_len=2)
+    s.init(g["init_seed"])
+    s.init_position(g["position_seed"], g["limits"])
+    s.burn_in(g["burn_in"], g["a"])
+    x = s.sample()[:, 0].astype(np.float64)
+    assert x.mean() == pytest.approx(g["mean"], abs=0.03)
+    assert x.std() == pytest.approx(g["sd"], abs=0.03)
+
+
+def test_mvn_closed_form_posterior_walker_mode():
+    """4-D correlated Gaussian: ensemble moments converge to mu and diag(Sigma)
+    (the model's closed form) with walker-aligned partners."""
+    rng = np.random.default_rng(1001)
+    mu = rng.standard_normal(4).astype(np.float32)
+    A = rng.standard_normal((4, 4))
+    Sigma = A @ A.T + np.eye(4)
+    P = np.linalg.inv(Sigma).astype(np.float32)
+    params = np.concatenate([mu, P.reshape(-1)]).astype(np.float32)
+    s = O.Stretch(O.MVN_LOGPDF, 4, 1024, params, 123, params_len=20,
+                  partner=O.PARTNER_WALKER, threads=4)
+    s.init_position(123, np.tile(np.array([[-5, 5]], np.float32), (4, 1)))
+    s.burn_in(400, 2.0)
+    draws = np.concatenate([s.sample() for _ in range(200)]).astype(np.float64)
+    sd = np.sqrt(np.diag(Sigma))
+    assert np.all(np.abs(draws.mean(0) - mu) < 0.25 * sd)
+    assert np.all(np.abs(draws.std(0) / sd - 1) < 0.15)
