@@ -1,0 +1,1249 @@
+// C-ABI of the HIP engine (include/bayadera_hip.h): handle types, hipRTC
+// compilation of model sources together with the engine kernels, and the
+// host loops of the stretch sampler (GCNStretch, amd_gcn.clj:271-567).
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bayadera_hip.h"
+#include "bh_kernels.hpp"
+
+// Sources handed to hipRTC, embedded at build time as raw string literals.
+#include "bh_rtc_sources.inc"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define BH_HIP(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(BH_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+#define BH_TRY(expr)              \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != BH_OK) return rc_; \
+    } while (0)
+
+#define BH_ARG(cond, ...)                                  \
+    do {                                                   \
+        if (!(cond)) return fail(BH_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Device buffer owned for the duration of a call.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        if (n <= bytes && p) return BH_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        BH_HIP(hipMalloc(&p, n ? n : 1));
+        bytes = n;
+        return BH_OK;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// An input array as a device pointer: used in place when it already is one,
+// else copied up on the stream.
+struct DevIn {
+    const void* dev = nullptr;
+    DevBuf own;
+    int init(const void* src, size_t bytes, hipStream_t s) {
+        if (is_device_ptr(src)) {
+            dev = src;
+            return BH_OK;
+        }
+        BH_TRY(own.alloc(bytes));
+        BH_HIP(hipMemcpyAsync(own.p, src, bytes, hipMemcpyHostToDevice, s));
+        dev = own.p;
+        return BH_OK;
+    }
+    template <class T>
+    const T* as() const {
+        return static_cast<const T*>(dev);
+    }
+};
+
+// An output array: written in place when it is a device pointer, else
+// staged and copied down by finish().
+struct DevOut {
+    void* dst = nullptr;
+    void* dev = nullptr;
+    size_t bytes = 0;
+    DevBuf own;
+    int init(void* d, size_t n) {
+        dst = d;
+        bytes = n;
+        if (is_device_ptr(d)) {
+            dev = d;
+            return BH_OK;
+        }
+        BH_TRY(own.alloc(n));
+        dev = own.p;
+        return BH_OK;
+    }
+    int finish(hipStream_t s) {
+        if (dev != dst) BH_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, s));
+        return BH_OK;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(dev);
+    }
+};
+
+// ------------------------------------------------------------------------
+// hipRTC: model source + engine kernels -> code object (cached per process)
+// ------------------------------------------------------------------------
+std::mutex g_rtc_mu;
+std::unordered_map<std::string, std::vector<char>> g_rtc_cache;
+
+int rtc_compile(const std::string& arch, const std::string& model_source,
+                const std::vector<std::string>& defines, std::vector<char>& code, char* log,
+                size_t log_size) {
+    std::string src;
+    src += "#include \"bh_rtc_prelude.h\"\n";
+    src += "#pragma clang force_cuda_host_device begin\n";
+    src += model_source;
+    src += "\n#pragma clang force_cuda_host_device end\n";
+    src += kRtcEngineSource;
+    std::string key = arch + "\x1f" + src;
+    for (const auto& d : defines) key += "\x1f" + d;
+    {
+        std::lock_guard<std::mutex> lk(g_rtc_mu);
+        auto it = g_rtc_cache.find(key);
+        if (it != g_rtc_cache.end()) {
+            code = it->second;
+            if (log && log_size) log[0] = 0;
+            return BH_OK;
+        }
+    }
+    const char* hdr_src[] = {kRtcPhiloxSource, kRtcPreludeSource};
+    const char* hdr_name[] = {"bh_philox.hpp", "bh_rtc_prelude.h"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "bh_model.hip", 2, hdr_src, hdr_name) !=
+        HIPRTC_SUCCESS)
+        return fail(BH_ERR_RTC, "hiprtcCreateProgram failed");
+    std::vector<std::string> opts = {"--offload-arch=" + arch,
+                                     "-O3",
+                                     "-std=c++17",
+                                     "-ffp-contract=on",
+                                     "-fhip-fp32-correctly-rounded-divide-sqrt",
+                                     "-Xclang",
+                                     "-target-feature",
+                                     "-Xclang",
+                                     "-packed-fp32-ops"};
+    for (const auto& d : defines) opts.push_back(d);
+    std::vector<const char*> copts;
+    for (const auto& o : opts) copts.push_back(o.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string plog(ls + 1, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &plog[0]);
+    if (log && log_size) {
+        snprintf(log, log_size, "%s", plog.c_str());
+    }
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(BH_ERR_RTC, "model source failed to compile: %.1500s", plog.c_str());
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    std::lock_guard<std::mutex> lk(g_rtc_mu);
+    g_rtc_cache[key] = code;
+    return BH_OK;
+}
+
+struct Module {
+    hipModule_t mod = nullptr;
+    ~Module() {
+        if (mod) (void)hipModuleUnload(mod);
+    }
+    int load(const std::vector<char>& code) {
+        BH_HIP(hipModuleLoadData(&mod, code.data()));
+        return BH_OK;
+    }
+    int fn(const char* name, hipFunction_t* f) {
+        BH_HIP(hipModuleGetFunction(f, mod, name));
+        return BH_OK;
+    }
+};
+
+int launch(hipFunction_t f, unsigned grid, unsigned block, unsigned lds, hipStream_t s,
+           void** args) {
+    BH_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, lds, s, args, nullptr));
+    return BH_OK;
+}
+
+bool valid_name(const char* n) {
+    if (!n || !*n) return false;
+    for (const char* p = n; *p; ++p)
+        if (!(isalnum((unsigned char)*p) || *p == '_')) return false;
+    return true;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// Handle types
+// --------------------------------------------------------------------------
+struct bh_ctx {
+    int device = 0;
+    uint32_t wgs = 256;
+    int cu = 0;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    DevBuf scratch;
+};
+
+struct bh_dist_engine {
+    bh_ctx* ctx = nullptr;
+    uint32_t params_len = 0;
+    Module mod;
+    hipFunction_t logpdf = nullptr, pdf = nullptr;
+};
+
+struct bh_lik_engine {
+    bh_ctx* ctx = nullptr;
+    Module mod;
+    hipFunction_t loglik = nullptr, lik = nullptr, evidence = nullptr;
+};
+
+struct bh_stretch_factory {
+    bh_ctx* ctx = nullptr;
+    uint32_t dim = 0, params_len = 0;
+    Module mod;
+    hipFunction_t logfn = nullptr, bare = nullptr, accu = nullptr, resident = nullptr;
+};
+
+struct bh_stretch {
+    bh_stretch_factory* f = nullptr;
+    bh_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t dim = 0, W = 0, K = 0, wgs = 0, groups = 0;
+    uint32_t params_count = 0, data_len = 0, params_len = 0;
+    DevBuf xs, logfn, params, accept, means_acc, tmp;
+    uint64_t means_cap = 0;
+    uint32_t bare_seed = 0, move_seed = 0, bare_counter = 0, move_counter = 0;
+    uint64_t iter = 0;
+    float bare_a = 2.0f, beta = 1.0f, accu_a = 2.0f;
+    int partner = BH_PARTNER_FLAT, exec = BH_EXEC_AUTO;
+    bool move_ready = false;
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+int ctx_scratch(bh_ctx* c, size_t bytes, void** p) {
+    BH_TRY(c->scratch.alloc(bytes));
+    *p = c->scratch.p;
+    return BH_OK;
+}
+
+int sync(hipStream_t s) {
+    BH_HIP(hipStreamSynchronize(s));
+    return BH_OK;
+}
+
+// Event pair around a call's kernels on the sampler's stream.
+struct Timed {
+    bh_stretch* s;
+    explicit Timed(bh_stretch* st) : s(st) {
+        if (s->timing) (void)hipEventRecord(s->ev0, s->stream);
+    }
+    int stop() {
+        if (!s->timing) return BH_OK;
+        BH_HIP(hipEventRecord(s->ev1, s->stream));
+        BH_HIP(hipEventSynchronize(s->ev1));
+        float ms = 0.0f;
+        BH_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms;
+        return BH_OK;
+    }
+};
+
+// block size of the resident kernel: a multiple of WGS dividing K, <= 1024
+unsigned resident_block(const bh_stretch* s) {
+    unsigned nb = s->K <= 1024 ? s->K : 1024;
+    while (nb > s->wgs && (s->K % nb) != 0) nb -= s->wgs;
+    return nb;
+}
+
+size_t resident_lds(const bh_stretch* s, int mode) {
+    const unsigned nb = resident_block(s);
+    // positions, logfn values, and the accounting staging area (the
+    // kernel's BH_ACCU_CHUNK = min(DIM, 8) floats per thread)
+    const size_t chunk = s->dim < 8 ? s->dim : 8;
+    size_t floats = (size_t)s->W * s->dim + s->W + (mode == 1 ? chunk * nb : 0);
+    return floats * 4 + (nb / 64) * 4;
+}
+
+constexpr size_t kResidentLdsMax = 64 * 1024;
+
+bool use_resident(const bh_stretch* s, int mode) {
+    if (s->exec == BH_EXEC_STEPWISE) return false;
+    const bool fits = resident_lds(s, mode) <= kResidentLdsMax;
+    if (s->exec == BH_EXEC_RESIDENT) return fits;
+    // a single work-group pays off while the ensemble is small
+    return fits && s->W <= 4096;
+}
+
+int run_resident(bh_stretch* s, int mode, uint32_t seed_odd, uint32_t seed_even, uint32_t c_odd,
+                 uint32_t c_even, float a, float beta, const float* d_betas, uint32_t step0,
+                 uint32_t nsteps, float* sample_out, unsigned long long sample_floats) {
+    unsigned data_len = s->data_len, params_len = s->params_len, K = s->K;
+    const float* params = s->params.as<float>();
+    float* xs = s->xs.as<float>();
+    float* lf = s->logfn.as<float>();
+    int partner = s->partner;
+    unsigned* acc = mode == 1 ? s->accept.as<unsigned>() : nullptr;
+    float* means = mode == 1 ? s->means_acc.as<float>() : nullptr;
+    void* args[] = {&seed_odd, &seed_even, &c_odd,   &c_even,      &data_len, &params_len,
+                    &params,   &xs,        &lf,      &K,           &a,        &beta,
+                    &d_betas,  &step0,     &nsteps,  &partner,     &mode,     &acc,
+                    &means,    &sample_out, &sample_floats};
+    return launch(s->f->resident, 1, resident_block(s), (unsigned)resident_lds(s, mode), s->stream,
+                  args);
+}
+
+// One bare half-step pair (move-bare!, amd_gcn.clj:347-353): odd half with
+// seed and constant 3333, even half with seed+1 and 4444 (:547-550).
+int bare_pair(bh_stretch* s, float beta) {
+    unsigned data_len = s->data_len, params_len = s->params_len, K = s->K;
+    const float* params = s->params.as<float>();
+    float* x0 = s->xs.as<float>();
+    float* x1 = x0 + (size_t)s->K * s->dim;
+    float* l0 = s->logfn.as<float>();
+    float* l1 = l0 + s->K;
+    float a = s->bare_a;
+    unsigned step = s->bare_counter;
+    int partner = s->partner;
+    unsigned seed0 = s->bare_seed, seed1 = s->bare_seed + 1u, c0 = 3333, c1 = 4444;
+    const float* S1 = x1;
+    const float* S0 = x0;
+    void* a0[] = {&seed0, &c0, &data_len, &params_len, &params, &S1, &x0, &l0, &a, &beta, &step, &K, &partner};
+    void* a1[] = {&seed1, &c1, &data_len, &params_len, &params, &S0, &x1, &l1, &a, &beta, &step, &K, &partner};
+    const unsigned grid = s->K / s->wgs;
+    BH_TRY(launch(s->f->bare, grid, s->wgs, 0, s->stream, a0));
+    BH_TRY(launch(s->f->bare, grid, s->wgs, 0, s->stream, a1));
+    return BH_OK;
+}
+
+// One accounting pair (move!, amd_gcn.clj:340-346): constants 1111/2222.
+int accu_pair(bh_stretch* s) {
+    unsigned data_len = s->data_len, params_len = s->params_len, K = s->K;
+    const float* params = s->params.as<float>();
+    float* x0 = s->xs.as<float>();
+    float* x1 = x0 + (size_t)s->K * s->dim;
+    float* l0 = s->logfn.as<float>();
+    float* l1 = l0 + s->K;
+    unsigned* acc = s->accept.as<unsigned>();
+    float* means = s->means_acc.as<float>();
+    float a = s->accu_a;
+    unsigned step = s->move_counter;
+    int partner = s->partner;
+    unsigned seed0 = s->move_seed, seed1 = s->move_seed + 1u, c0 = 1111, c1 = 2222;
+    const float* S1 = x1;
+    const float* S0 = x0;
+    void* a0[] = {&seed0, &c0, &data_len, &params_len, &params, &S1, &x0, &l0, &acc, &means, &a, &step, &K, &partner};
+    void* a1[] = {&seed1, &c1, &data_len, &params_len, &params, &S0, &x1, &l1, &acc, &means, &a, &step, &K, &partner};
+    const unsigned grid = s->K / s->wgs;
+    BH_TRY(launch(s->f->accu, grid, s->wgs, 0, s->stream, a0));
+    BH_TRY(launch(s->f->accu, grid, s->wgs, 0, s->stream, a1));
+    return BH_OK;
+}
+
+int bare_steps(bh_stretch* s, uint64_t n, const float* h_betas) {
+    if (n == 0) return BH_OK;
+    if (use_resident(s, 0)) {
+        DevBuf d_betas;
+        const float* db = nullptr;
+        if (h_betas) {
+            BH_TRY(d_betas.alloc(n * sizeof(float)));
+            BH_HIP(hipMemcpyAsync(d_betas.p, h_betas, n * sizeof(float), hipMemcpyHostToDevice,
+                                  s->stream));
+            db = d_betas.as<float>();
+        }
+        // the kernel's step counter is 32-bit, as the reference's (int-array)
+        uint64_t done = 0;
+        while (done < n) {
+            const uint32_t chunk = (uint32_t)std::min<uint64_t>(n - done, 1u << 20);
+            BH_TRY(run_resident(s, 0, s->bare_seed, s->bare_seed + 1u, 3333, 4444, s->bare_a,
+                                s->beta, db ? db + done : nullptr, s->bare_counter, chunk, nullptr,
+                                0));
+            s->bare_counter += chunk;
+            done += chunk;
+        }
+        if (h_betas) s->beta = h_betas[n - 1];
+        BH_TRY(sync(s->stream));
+        return BH_OK;
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        if (h_betas) s->beta = h_betas[i];
+        BH_TRY(bare_pair(s, s->beta));
+        s->bare_counter += 1;
+    }
+    return sync(s->stream);
+}
+
+int init_move(bh_stretch* s, float a, uint64_t steps) {
+    s->move_seed += 2u;
+    s->move_counter = 0;
+    s->accu_a = a;
+    BH_TRY(s->accept.alloc((size_t)s->groups * sizeof(unsigned)));
+    BH_HIP(hipMemsetAsync(s->accept.p, 0, (size_t)s->groups * sizeof(unsigned), s->stream));
+    const size_t mbytes = (size_t)s->dim * s->groups * (steps ? steps : 1) * sizeof(float);
+    BH_TRY(s->means_acc.alloc(mbytes));
+    BH_HIP(hipMemsetAsync(s->means_acc.p, 0, mbytes, s->stream));
+    s->means_cap = steps ? steps : 1;
+    s->move_ready = true;
+    return BH_OK;
+}
+
+int accu_steps(bh_stretch* s, uint64_t n) {
+    if (!s->move_ready) return fail(BH_ERR_STATE, "move! before init-move!");
+    if (s->move_counter + n > s->means_cap)
+        return fail(BH_ERR_STATE, "move! beyond the %llu steps reserved by init-move!",
+                    (unsigned long long)s->means_cap);
+    if (n == 0) return BH_OK;
+    if (use_resident(s, 1)) {
+        BH_TRY(run_resident(s, 1, s->move_seed, s->move_seed + 1u, 1111, 2222, s->accu_a, 1.0f,
+                            nullptr, s->move_counter, (uint32_t)n, nullptr, 0));
+        s->move_counter += (uint32_t)n;
+        return sync(s->stream);
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        BH_TRY(accu_pair(s));
+        s->move_counter += 1;
+    }
+    return sync(s->stream);
+}
+
+uint32_t acor_lag(uint32_t n, uint32_t wgs) {
+    // amd_gcn.clj:224-228
+    uint32_t lag = n / 5;
+    lag = std::min(lag, wgs);
+    lag = std::min(lag, 64u);
+    return std::max(lag, 10u);
+}
+
+// acor on a device series (dim x n), which it overwrites.
+int acor_device(bh_ctx* c, hipStream_t st, float* d_series, uint32_t dim, uint32_t n, float* tau,
+                float* mean, float* sigma, uint32_t* lag_out) {
+    const uint32_t lag = acor_lag(n, c->wgs);
+    if (lag_out) *lag_out = lag;
+    if ((uint64_t)lag * 5 > n)
+        return fail(BH_ERR_RANGE,
+                    "The autocorrelation time is too long relative to the variance. Number of "
+                    "steps (%u) must not be less than %u.",
+                    n, lag * 5);
+    const uint32_t wgsm = std::min(std::min(dim, 16u), c->wgs);
+    const uint32_t local_n_mean = c->wgs / wgsm;
+    const uint32_t half = (n + local_n_mean - 1) / local_n_mean + (n + c->wgs - 1) / c->wgs + 8;
+    const uint32_t per_dim = 2 * half;
+    DevBuf scratch, out;
+    BH_TRY(scratch.alloc((size_t)per_dim * dim * sizeof(float)));
+    BH_TRY(out.alloc((size_t)3 * dim * sizeof(float)));
+    float* o = out.as<float>();
+    BH_HIP(bh::launch_acor(st, d_series, dim, n, lag, c->wgs, scratch.as<float>(), per_dim, o,
+                           o + dim, o + 2 * dim));
+    std::vector<float> h(3 * dim);
+    BH_HIP(hipMemcpyAsync(h.data(), o, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    BH_TRY(sync(st));
+    for (uint32_t d = 0; d < dim; ++d) {
+        if (tau) tau[d] = h[d];
+        if (mean) mean[d] = h[dim + d];
+        if (sigma) sigma[d] = h[2 * dim + d];
+    }
+    return BH_OK;
+}
+
+int histogram_device(bh_ctx* c, hipStream_t st, const float* d_a, uint32_t m, uint64_t n,
+                     uint32_t ld, float* d_limits, unsigned* d_counts, bool compute_limits) {
+    void* scratch = nullptr;
+    BH_TRY(ctx_scratch(c, bh::dataset_scratch_bytes(m, n), &scratch));
+    if (compute_limits) BH_HIP(bh::launch_min_max(st, d_a, m, n, ld, d_limits, scratch));
+    BH_HIP(bh::launch_histogram(st, d_a, m, n, ld, d_limits, c->wgs, d_counts));
+    return BH_OK;
+}
+
+int check_matrix(uint32_t m, uint64_t n, uint32_t ld) {
+    BH_ARG(m > 0 && n > 0, "empty matrix (%u x %llu)", m, (unsigned long long)n);
+    BH_ARG(ld >= m, "leading dimension %u < rows %u", ld, m);
+    return BH_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C-ABI
+// ==========================================================================
+extern "C" {
+
+const char* bh_last_error(void) { return g_err.c_str(); }
+
+int bh_version(void) { return 100; }
+
+int bh_rtc_compile_check(const char* source, const char* logfn_name, const char* logpdf_name,
+                         const char* loglik_name, uint32_t dim, uint32_t wgs, const char* arch,
+                         char* log, size_t log_size) {
+    BH_ARG(source && arch, "NULL argument");
+    BH_ARG(wgs >= 64 && wgs <= 1024 && (wgs & (wgs - 1)) == 0, "bad WGS %u", wgs);
+    std::vector<std::string> defs = {"-DWGS=" + std::to_string(wgs)};
+    if (logfn_name) {
+        BH_ARG(valid_name(logfn_name), "invalid logfn function name");
+        BH_ARG(dim >= 1 && dim <= 256, "dimension %u outside [1, 256]", dim);
+        defs.push_back("-DDIM=" + std::to_string(dim));
+        defs.push_back(std::string("-DLOGFN=") + logfn_name);
+    }
+    if (logpdf_name) {
+        BH_ARG(valid_name(logpdf_name), "invalid logpdf function name");
+        defs.push_back(std::string("-DLOGPDF=") + logpdf_name);
+    }
+    if (loglik_name) {
+        BH_ARG(valid_name(loglik_name), "invalid loglik function name");
+        defs.push_back(std::string("-DLOGLIK=") + loglik_name);
+    }
+    std::vector<char> code;
+    return rtc_compile(arch, source, defs, code, log, log_size);
+}
+
+int bh_ctx_create(int device, uint32_t wgs, bh_ctx** out) {
+    BH_ARG(out, "out is NULL");
+    BH_ARG(wgs >= 64 && wgs <= 1024 && (wgs & (wgs - 1)) == 0,
+           "WGS must be a power of two in [64, 1024], got %u", wgs);
+    BH_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BH_HIP(hipGetDeviceProperties(&prop, device));
+    auto c = std::make_unique<bh_ctx>();
+    c->device = device;
+    c->wgs = wgs;
+    c->cu = prop.multiProcessorCount;
+    std::string arch = prop.gcnArchName;
+    c->arch = arch.substr(0, arch.find(':'));
+    BH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    return BH_OK;
+}
+
+int bh_ctx_destroy(bh_ctx* ctx) {
+    if (!ctx) return BH_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BH_OK;
+}
+
+int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs) {
+    BH_ARG(ctx, "ctx is NULL");
+    if (compute_units) *compute_units = ctx->cu;
+    if (wgs) *wgs = ctx->wgs;
+    return BH_OK;
+}
+
+int bh_uniform_sample(bh_ctx* ctx, uint32_t seed, float lower, float upper, float* out,
+                      uint64_t n) {
+    BH_ARG(ctx && out, "NULL argument");
+    BH_ARG(n % 4 == 0 && n / 4 <= 0xffffffffull,
+           "the direct sampler needs a multiple of 4 samples (amd_gcn.clj:55-62), got %llu",
+           (unsigned long long)n);
+    BH_HIP(hipSetDevice(ctx->device));
+    DevOut o;
+    BH_TRY(o.init(out, n * sizeof(float)));
+    BH_HIP(bh::launch_uniform_sample(ctx->stream, seed, lower, upper, (unsigned)(n / 4),
+                                     o.as<float>()));
+    BH_TRY(o.finish(ctx->stream));
+    return sync(ctx->stream);
+}
+
+// ---------------------------------------------------------------- density
+int bh_dist_engine_create(bh_ctx* ctx, const char* source, const char* logpdf_name,
+                          uint32_t params_len, bh_dist_engine** out, char* log,
+                          size_t log_size) {
+    BH_ARG(ctx && source && out, "NULL argument");
+    BH_ARG(valid_name(logpdf_name), "invalid logpdf function name");
+    BH_HIP(hipSetDevice(ctx->device));
+    std::vector<char> code;
+    BH_TRY(rtc_compile(ctx->arch, source,
+                       {"-DWGS=" + std::to_string(ctx->wgs), std::string("-DLOGPDF=") + logpdf_name},
+                       code, log, log_size));
+    auto e = std::make_unique<bh_dist_engine>();
+    e->ctx = ctx;
+    e->params_len = params_len;
+    BH_TRY(e->mod.load(code));
+    BH_TRY(e->mod.fn("bh_logpdf", &e->logpdf));
+    BH_TRY(e->mod.fn("bh_pdf", &e->pdf));
+    *out = e.release();
+    return BH_OK;
+}
+
+int bh_dist_engine_destroy(bh_dist_engine* e) {
+    delete e;
+    return BH_OK;
+}
+
+static int density_impl(bh_dist_engine* e, hipFunction_t fn, const float* params,
+                        uint32_t params_count, const float* x, uint32_t dim, uint64_t n,
+                        float* res) {
+    BH_ARG(e && params && x && res, "NULL argument");
+    BH_ARG(dim > 0 && n > 0, "empty input");
+    bh_ctx* c = e->ctx;
+    BH_HIP(hipSetDevice(c->device));
+    // data_len = max(0, params_count - params_len) (amd_gcn.clj:75-77)
+    unsigned data_len = params_count > e->params_len ? params_count - e->params_len : 0u;
+    unsigned params_len = e->params_len;
+    DevIn dp, dx;
+    DevOut dr;
+    BH_TRY(dp.init(params, (size_t)params_count * sizeof(float), c->stream));
+    BH_TRY(dx.init(x, (size_t)dim * n * sizeof(float), c->stream));
+    BH_TRY(dr.init(res, n * sizeof(float)));
+    const float* pp = dp.as<float>();
+    const float* xp = dx.as<float>();
+    float* rp = dr.as<float>();
+    unsigned long long nn = n;
+    void* args[] = {&data_len, &params_len, &pp, &dim, &xp, &rp, &nn};
+    BH_TRY(launch(fn, (unsigned)((n + c->wgs - 1) / c->wgs), c->wgs, 0, c->stream, args));
+    BH_TRY(dr.finish(c->stream));
+    return sync(c->stream);
+}
+
+int bh_log_density(bh_dist_engine* e, const float* params, uint32_t params_count, const float* x,
+                   uint32_t dim, uint64_t n, float* res) {
+    return density_impl(e, e ? e->logpdf : nullptr, params, params_count, x, dim, n, res);
+}
+
+int bh_density(bh_dist_engine* e, const float* params, uint32_t params_count, const float* x,
+               uint32_t dim, uint64_t n, float* res) {
+    return density_impl(e, e ? e->pdf : nullptr, params, params_count, x, dim, n, res);
+}
+
+// ------------------------------------------------------------- likelihood
+int bh_lik_engine_create(bh_ctx* ctx, const char* source, const char* loglik_name,
+                         bh_lik_engine** out, char* log, size_t log_size) {
+    BH_ARG(ctx && source && out, "NULL argument");
+    BH_ARG(valid_name(loglik_name), "invalid loglik function name");
+    BH_HIP(hipSetDevice(ctx->device));
+    std::vector<char> code;
+    BH_TRY(rtc_compile(ctx->arch, source,
+                       {"-DWGS=" + std::to_string(ctx->wgs), std::string("-DLOGLIK=") + loglik_name},
+                       code, log, log_size));
+    auto e = std::make_unique<bh_lik_engine>();
+    e->ctx = ctx;
+    BH_TRY(e->mod.load(code));
+    BH_TRY(e->mod.fn("bh_loglik", &e->loglik));
+    BH_TRY(e->mod.fn("bh_lik", &e->lik));
+    BH_TRY(e->mod.fn("bh_evidence_partial", &e->evidence));
+    *out = e.release();
+    return BH_OK;
+}
+
+int bh_lik_engine_destroy(bh_lik_engine* e) {
+    delete e;
+    return BH_OK;
+}
+
+static int lik_impl(bh_lik_engine* e, hipFunction_t fn, const float* data, uint32_t data_len,
+                    const float* x, uint32_t dim, uint64_t n, float* res, double* evidence) {
+    BH_ARG(e && x && (res || evidence), "NULL argument");
+    BH_ARG(data || data_len == 0, "NULL data");
+    BH_ARG(dim > 0 && n > 0, "empty input");
+    bh_ctx* c = e->ctx;
+    BH_HIP(hipSetDevice(c->device));
+    DevIn dd, dx;
+    float zero = 0.0f;
+    BH_TRY(dd.init(data ? (const void*)data : (const void*)&zero,
+                   (size_t)std::max(data_len, 1u) * sizeof(float), c->stream));
+    BH_TRY(dx.init(x, (size_t)dim * n * sizeof(float), c->stream));
+    const float* dp = dd.as<float>();
+    const float* xp = dx.as<float>();
+    unsigned long long nn = n;
+    const unsigned grid = (unsigned)((n + c->wgs - 1) / c->wgs);
+    if (evidence) {
+        DevBuf partial, total;
+        BH_TRY(partial.alloc((size_t)grid * sizeof(double)));
+        BH_TRY(total.alloc(sizeof(double)));
+        double* pp = partial.as<double>();
+        void* args[] = {&data_len, &dp, &dim, &xp, &nn, &pp};
+        BH_TRY(launch(fn, grid, c->wgs, 0, c->stream, args));
+        BH_HIP(bh::launch_sum_f64(c->stream, pp, grid, total.as<double>()));
+        double h = 0.0;
+        BH_HIP(hipMemcpyAsync(&h, total.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        BH_TRY(sync(c->stream));
+        *evidence = h / (double)n;
+        return BH_OK;
+    }
+    DevOut dr;
+    BH_TRY(dr.init(res, n * sizeof(float)));
+    float* rp = dr.as<float>();
+    void* args[] = {&data_len, &dp, &dim, &xp, &rp, &nn};
+    BH_TRY(launch(fn, grid, c->wgs, 0, c->stream, args));
+    BH_TRY(dr.finish(c->stream));
+    return sync(c->stream);
+}
+
+int bh_loglik(bh_lik_engine* e, const float* data, uint32_t data_len, const float* x,
+              uint32_t dim, uint64_t n, float* res) {
+    return lik_impl(e, e ? e->loglik : nullptr, data, data_len, x, dim, n, res, nullptr);
+}
+
+int bh_lik(bh_lik_engine* e, const float* data, uint32_t data_len, const float* x, uint32_t dim,
+           uint64_t n, float* res) {
+    return lik_impl(e, e ? e->lik : nullptr, data, data_len, x, dim, n, res, nullptr);
+}
+
+int bh_evidence(bh_lik_engine* e, const float* data, uint32_t data_len, const float* x,
+                uint32_t dim, uint64_t n, double* out) {
+    BH_ARG(out, "NULL argument");
+    return lik_impl(e, e ? e->evidence : nullptr, data, data_len, x, dim, n, nullptr, out);
+}
+
+// ---------------------------------------------------------------- dataset
+static int moments_impl(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                        float* out, bool variance) {
+    BH_ARG(ctx && data && out, "NULL argument");
+    BH_TRY(check_matrix(m, n, ld));
+    BH_HIP(hipSetDevice(ctx->device));
+    DevIn dx;
+    BH_TRY(dx.init(data, ((size_t)(n - 1) * ld + m) * sizeof(float), ctx->stream));
+    DevBuf mv;
+    BH_TRY(mv.alloc((size_t)2 * m * sizeof(float)));
+    void* scratch = nullptr;
+    BH_TRY(ctx_scratch(ctx, bh::dataset_scratch_bytes(m, n), &scratch));
+    float* mean = mv.as<float>();
+    float* var = mean + m;
+    BH_HIP(bh::launch_moments(ctx->stream, dx.as<float>(), m, n, ld, mean,
+                              variance ? var : nullptr, scratch));
+    DevOut o;
+    BH_TRY(o.init(out, (size_t)m * sizeof(float)));
+    BH_HIP(hipMemcpyAsync(o.dev, variance ? var : mean, (size_t)m * sizeof(float),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    BH_TRY(o.finish(ctx->stream));
+    return sync(ctx->stream);
+}
+
+int bh_dataset_mean(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                    float* out) {
+    return moments_impl(ctx, data, m, n, ld, out, false);
+}
+
+int bh_dataset_variance(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                        float* out) {
+    return moments_impl(ctx, data, m, n, ld, out, true);
+}
+
+int bh_dataset_histogram(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                         float* limits, float* pdf, float* bin_ranks) {
+    BH_ARG(ctx && data && limits && pdf && bin_ranks, "NULL argument");
+    BH_TRY(check_matrix(m, n, ld));
+    BH_HIP(hipSetDevice(ctx->device));
+    const uint32_t bins = ctx->wgs;
+    DevIn dx;
+    BH_TRY(dx.init(data, ((size_t)(n - 1) * ld + m) * sizeof(float), ctx->stream));
+    DevBuf counts;
+    BH_TRY(counts.alloc((size_t)m * bins * sizeof(unsigned)));
+    BH_HIP(hipMemsetAsync(counts.p, 0, (size_t)m * bins * sizeof(unsigned), ctx->stream));
+    DevOut dl, dp, dr;
+    BH_TRY(dl.init(limits, (size_t)2 * m * sizeof(float)));
+    BH_TRY(dp.init(pdf, (size_t)bins * m * sizeof(float)));
+    BH_TRY(dr.init(bin_ranks, (size_t)bins * m * sizeof(float)));
+    BH_TRY(histogram_device(ctx, ctx->stream, dx.as<float>(), m, n, ld, dl.as<float>(),
+                            counts.as<unsigned>(), true));
+    // alpha = WGS / n as the reference's float (amd_gcn.clj:206)
+    const float alpha = (float)((double)bins / (double)n);
+    BH_HIP(bh::launch_histogram_finalize(ctx->stream, counts.as<unsigned>(), dl.as<float>(), alpha,
+                                         m, bins, dp.as<float>(), dr.as<float>()));
+    BH_TRY(dl.finish(ctx->stream));
+    BH_TRY(dp.finish(ctx->stream));
+    BH_TRY(dr.finish(ctx->stream));
+    return sync(ctx->stream);
+}
+
+int bh_acor(bh_ctx* ctx, const float* series, uint32_t dim, uint32_t n, float* tau, float* mean,
+            float* sigma, uint32_t* lag) {
+    BH_ARG(ctx && series, "NULL argument");
+    BH_ARG(dim > 0 && n > 0, "empty series");
+    BH_HIP(hipSetDevice(ctx->device));
+    DevBuf copy;
+    BH_TRY(copy.alloc((size_t)dim * n * sizeof(float)));
+    BH_HIP(hipMemcpyAsync(copy.p, series, (size_t)dim * n * sizeof(float), hipMemcpyDefault,
+                          ctx->stream));
+    return acor_device(ctx, ctx->stream, copy.as<float>(), dim, n, tau, mean, sigma, lag);
+}
+
+// ---------------------------------------------------------------- stretch
+int bh_stretch_factory_create(bh_ctx* ctx, const char* source, const char* logfn_name,
+                              uint32_t dim, uint32_t params_len, bh_stretch_factory** out,
+                              char* log, size_t log_size) {
+    BH_ARG(ctx && source && out, "NULL argument");
+    BH_ARG(valid_name(logfn_name), "invalid logfn function name");
+    BH_ARG(dim >= 1 && dim <= 256, "dimension %u outside [1, 256]", dim);
+    BH_HIP(hipSetDevice(ctx->device));
+    std::vector<char> code;
+    BH_TRY(rtc_compile(ctx->arch, source,
+                       {"-DWGS=" + std::to_string(ctx->wgs), "-DDIM=" + std::to_string(dim),
+                        std::string("-DLOGFN=") + logfn_name},
+                       code, log, log_size));
+    auto f = std::make_unique<bh_stretch_factory>();
+    f->ctx = ctx;
+    f->dim = dim;
+    f->params_len = params_len;
+    BH_TRY(f->mod.load(code));
+    BH_TRY(f->mod.fn("bh_logfn", &f->logfn));
+    BH_TRY(f->mod.fn("bh_stretch_move_bare", &f->bare));
+    BH_TRY(f->mod.fn("bh_stretch_move_accu", &f->accu));
+    BH_TRY(f->mod.fn("bh_stretch_resident", &f->resident));
+    *out = f.release();
+    return BH_OK;
+}
+
+int bh_stretch_factory_destroy(bh_stretch_factory* f) {
+    delete f;
+    return BH_OK;
+}
+
+int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
+                      const float* params, uint32_t params_count, bh_stretch** out) {
+    BH_ARG(f && out, "NULL argument");
+    BH_ARG(params || params_count == 0, "NULL params");
+    bh_ctx* c = f->ctx;
+    if (!(walkers >= 2 * c->wgs && walkers % (2 * c->wgs) == 0))
+        return fail(BH_ERR_RANGE, "Number of walkers (%u) must be a multiple of %u.", walkers,
+                    2 * c->wgs);
+    BH_ARG((uint64_t)walkers * f->dim < (1ull << 31), "ensemble too large");
+    BH_HIP(hipSetDevice(c->device));
+    auto s = std::make_unique<bh_stretch>();
+    s->f = f;
+    s->ctx = c;
+    s->dim = f->dim;
+    s->W = walkers;
+    s->K = walkers / 2;
+    s->wgs = c->wgs;
+    s->groups = s->K / c->wgs;
+    s->params_count = params_count;
+    s->params_len = f->params_len;
+    s->data_len = params_count > f->params_len ? params_count - f->params_len : 0u;
+    BH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    BH_TRY(s->xs.alloc((size_t)walkers * f->dim * sizeof(float)));
+    BH_TRY(s->logfn.alloc((size_t)walkers * sizeof(float)));
+    BH_TRY(s->params.alloc((size_t)std::max(params_count, 1u) * sizeof(float)));
+    BH_HIP(hipMemsetAsync(s->xs.p, 0, (size_t)walkers * f->dim * sizeof(float), s->stream));
+    BH_HIP(hipMemsetAsync(s->logfn.p, 0, (size_t)walkers * sizeof(float), s->stream));
+    if (params_count)
+        BH_HIP(hipMemcpyAsync(s->params.p, params, (size_t)params_count * sizeof(float),
+                              hipMemcpyDefault, s->stream));
+    BH_TRY(sync(s->stream));
+    BH_TRY(bh_stretch_init(s.get(), seed));
+    *out = s.release();
+    return BH_OK;
+}
+
+int bh_stretch_destroy(bh_stretch* s) {
+    if (!s) return BH_OK;
+    if (s->stream) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamDestroy(s->stream);
+    }
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    delete s;
+    return BH_OK;
+}
+
+int bh_stretch_set_timing(bh_stretch* s, int on) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    if (on && !s->ev0) {
+        BH_HIP(hipEventCreate(&s->ev0));
+        BH_HIP(hipEventCreate(&s->ev1));
+    }
+    s->timing = on != 0;
+    return BH_OK;
+}
+
+int bh_stretch_last_gpu_ms(const bh_stretch* s, double* ms) {
+    BH_ARG(s && ms, "NULL argument");
+    *ms = s->last_ms;
+    return BH_OK;
+}
+
+int bh_stretch_set_partner(bh_stretch* s, int mode) {
+    BH_ARG(s, "NULL sampler");
+    BH_ARG(mode == BH_PARTNER_FLAT || mode == BH_PARTNER_WALKER, "unknown partner mode %d", mode);
+    s->partner = mode;
+    return BH_OK;
+}
+
+int bh_stretch_set_exec(bh_stretch* s, int mode) {
+    BH_ARG(s, "NULL sampler");
+    BH_ARG(mode == BH_EXEC_AUTO || mode == BH_EXEC_STEPWISE || mode == BH_EXEC_RESIDENT,
+           "unknown execution mode %d", mode);
+    if (mode == BH_EXEC_RESIDENT && resident_lds(s, 1) > kResidentLdsMax)
+        return fail(BH_ERR_RANGE, "ensemble (%u walkers x %u) does not fit in one work-group's LDS",
+                    s->W, s->dim);
+    s->exec = mode;
+    return BH_OK;
+}
+
+int bh_stretch_info(const bh_stretch* s, uint32_t* walkers, uint64_t* iteration_counter) {
+    BH_ARG(s, "NULL sampler");
+    if (walkers) *walkers = s->W;
+    if (iteration_counter) *iteration_counter = s->iter;
+    return BH_OK;
+}
+
+int bh_stretch_init(bh_stretch* s, uint32_t seed) {
+    BH_ARG(s, "NULL sampler");
+    s->bare_seed = seed;
+    s->bare_a = 2.0f;
+    s->beta = 1.0f;
+    s->bare_counter = 0;
+    s->move_seed = seed;
+    return BH_OK;
+}
+
+static int logfn_all(bh_stretch* s) {
+    unsigned data_len = s->data_len, params_len = s->params_len, n = s->W;
+    const float* params = s->params.as<float>();
+    const float* xs = s->xs.as<float>();
+    float* lf = s->logfn.as<float>();
+    void* args[] = {&data_len, &params_len, &params, &xs, &lf, &n};
+    return launch(s->f->logfn, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, args);
+}
+
+int bh_stretch_init_position(bh_stretch* s, uint32_t seed, const float* limits) {
+    BH_ARG(s && limits, "NULL argument");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    DevIn dl;
+    BH_TRY(dl.init(limits, (size_t)2 * s->dim * sizeof(float), s->stream));
+    BH_HIP(bh::launch_init_walkers(s->stream, seed, dl.as<float>(), s->dim, s->W, s->xs.as<float>()));
+    BH_TRY(logfn_all(s));
+    s->iter = 0;
+    return sync(s->stream);
+}
+
+int bh_stretch_init_position_from(bh_stretch* s, const bh_stretch* p) {
+    BH_ARG(s && p, "NULL argument");
+    BH_ARG(p->dim == s->dim && p->W == s->W, "position sampler has another shape");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    BH_HIP(hipMemcpyAsync(s->xs.p, p->xs.p, (size_t)s->W * s->dim * sizeof(float),
+                          hipMemcpyDeviceToDevice, s->stream));
+    BH_TRY(logfn_all(s));
+    s->iter = 0;
+    return sync(s->stream);
+}
+
+int bh_stretch_move_bare(bh_stretch* s) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    Timed t(s);
+    BH_TRY(bare_pair(s, s->beta));
+    BH_TRY(t.stop());
+    s->bare_counter += 1;
+    return sync(s->stream);
+}
+
+int bh_stretch_set_temperature(bh_stretch* s, double t) {
+    BH_ARG(s, "NULL sampler");
+    s->beta = (float)(1.0 / t);
+    return BH_OK;
+}
+
+int bh_stretch_burn_in(bh_stretch* s, uint64_t n, float a) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    s->bare_a = a;
+    s->beta = 1.0f;
+    Timed t(s);
+    BH_TRY(bare_steps(s, n, nullptr));
+    BH_TRY(t.stop());
+    s->iter += n;
+    return BH_OK;
+}
+
+int bh_stretch_anneal(bh_stretch* s, const double* temperatures, uint64_t n, float a) {
+    BH_ARG(s && (temperatures || n == 0), "NULL argument");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    s->bare_a = a;
+    std::vector<float> betas(n);
+    for (uint64_t i = 0; i < n; ++i) betas[i] = (float)(1.0 / temperatures[i]);
+    Timed t(s);
+    BH_TRY(bare_steps(s, n, betas.data()));
+    BH_TRY(t.stop());
+    s->iter += n;
+    return BH_OK;
+}
+
+int bh_stretch_init_move(bh_stretch* s, float a, uint64_t max_steps) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    BH_TRY(init_move(s, a, max_steps));
+    return sync(s->stream);
+}
+
+int bh_stretch_move(bh_stretch* s) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    if (!s->move_ready) return fail(BH_ERR_STATE, "move! before init-move!");
+    if (s->move_counter + 1ull > s->means_cap)
+        return fail(BH_ERR_STATE, "move! beyond the %llu steps reserved by init-move!",
+                    (unsigned long long)s->means_cap);
+    Timed t(s);
+    BH_TRY(accu_pair(s));
+    BH_TRY(t.stop());
+    s->move_counter += 1;
+    return sync(s->stream);
+}
+
+int bh_stretch_read_accounting(bh_stretch* s, uint32_t* accept, float* means_acc,
+                               uint64_t steps) {
+    BH_ARG(s, "NULL sampler");
+    if (!s->move_ready) return fail(BH_ERR_STATE, "no init-move! yet");
+    BH_ARG(steps <= s->means_cap, "more steps than reserved");
+    if (accept)
+        BH_HIP(hipMemcpyAsync(accept, s->accept.p, (size_t)s->groups * sizeof(unsigned),
+                              hipMemcpyDefault, s->stream));
+    if (means_acc)
+        BH_HIP(hipMemcpyAsync(means_acc, s->means_acc.p,
+                              (size_t)s->dim * s->groups * steps * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+    return sync(s->stream);
+}
+
+static int accept_total(bh_stretch* s, unsigned long long* total) {
+    DevBuf t;
+    BH_TRY(t.alloc(sizeof(unsigned long long)));
+    BH_HIP(bh::launch_sum_u32(s->stream, s->accept.as<unsigned>(), s->groups,
+                              t.as<unsigned long long>()));
+    BH_HIP(hipMemcpyAsync(total, t.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          s->stream));
+    return sync(s->stream);
+}
+
+int bh_stretch_run(bh_stretch* s, uint64_t n, float a, double* acceptance_rate, float* means,
+                   float* tau, float* mean, float* sigma, uint32_t* lag) {
+    BH_ARG(s, "NULL sampler");
+    BH_ARG(n > 0 && n < (1ull << 31), "steps %llu out of range", (unsigned long long)n);
+    BH_HIP(hipSetDevice(s->ctx->device));
+    BH_TRY(init_move(s, a, n));
+    {
+        Timed t(s);
+        BH_TRY(accu_steps(s, n));
+        BH_TRY(t.stop());
+    }
+    s->iter += n;
+    // per-step ensemble means: sum_means_vertical + sum_reduction_horizontal
+    // over the groups of each (step, dim) row, then scal! 0.5/(WGS*groups)
+    // (amd_gcn.clj:437-439)
+    const uint32_t rows = (uint32_t)(n * s->dim);
+    const uint32_t local_n = std::min(s->groups, s->wgs);
+    DevBuf dmeans, work;
+    BH_TRY(dmeans.alloc((size_t)rows * sizeof(float)));
+    BH_TRY(work.alloc((size_t)rows * ((s->groups + local_n - 1) / local_n + 1) * sizeof(float)));
+    const float scale = (float)(0.5 / ((double)s->wgs * s->groups));
+    BH_HIP(bh::launch_row_tree_sum(s->stream, s->means_acc.as<float>(), rows, s->groups, local_n,
+                                   s->groups, work.as<float>(), scale, dmeans.as<float>(), 1));
+    unsigned long long total = 0;
+    BH_TRY(accept_total(s, &total));
+    if (acceptance_rate) *acceptance_rate = (double)total / ((double)s->W * (double)n);
+    if (means)
+        BH_HIP(hipMemcpyAsync(means, dmeans.p, (size_t)rows * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+    BH_TRY(sync(s->stream));
+    return acor_device(s->ctx, s->stream, dmeans.as<float>(), s->dim, (uint32_t)n, tau, mean,
+                       sigma, lag);
+}
+
+int bh_stretch_acc_rate(bh_stretch* s, float a, double* out) {
+    BH_ARG(s && out, "NULL argument");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    BH_TRY(init_move(s, a, 1));
+    BH_TRY(accu_steps(s, 1));
+    s->iter += 1;
+    unsigned long long total = 0;
+    BH_TRY(accept_total(s, &total));
+    *out = (double)total / (double)s->W;
+    return BH_OK;
+}
+
+int bh_stretch_sample(bh_stretch* s, float* out, uint64_t ncols) {
+    BH_ARG(s && out, "NULL argument");
+    BH_ARG(ncols > 0, "no columns requested");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    s->beta = 1.0f;  // set-temperature! 1.0 (amd_gcn.clj:366)
+    const uint64_t avail = (uint64_t)s->W * s->dim;
+    const uint64_t req = ncols * s->dim;
+    const uint64_t steps = (req + avail - 1) / avail;
+    DevOut o;
+    BH_TRY(o.init(out, req * sizeof(float)));
+    Timed t(s);
+    if (use_resident(s, 0) && steps < (1ull << 31)) {
+        BH_TRY(run_resident(s, 0, s->bare_seed, s->bare_seed + 1u, 3333, 4444, s->bare_a, 1.0f,
+                            nullptr, s->bare_counter, (uint32_t)steps, o.as<float>(), req));
+        s->bare_counter += (uint32_t)steps;
+    } else {
+        uint64_t ofst = 0;
+        for (uint64_t i = 0; i < steps; ++i) {
+            BH_TRY(bare_pair(s, 1.0f));
+            s->bare_counter += 1;
+            const uint64_t chunk = std::min(avail, req - ofst);
+            BH_HIP(hipMemcpyAsync(o.as<float>() + ofst, s->xs.p, chunk * sizeof(float),
+                                  hipMemcpyDeviceToDevice, s->stream));
+            ofst += chunk;
+        }
+    }
+    BH_TRY(t.stop());
+    s->iter += steps;
+    BH_TRY(o.finish(s->stream));
+    return sync(s->stream);
+}
+
+int bh_stretch_histogram(bh_stretch* s, uint32_t cycles, float* limits, float* pdf,
+                         float* bin_ranks) {
+    BH_ARG(s && limits && pdf && bin_ranks, "NULL argument");
+    BH_ARG(cycles >= 1, "cycles must be >= 1");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    bh_ctx* c = s->ctx;
+    const uint32_t bins = s->wgs, m = s->dim;
+    DevBuf counts;
+    BH_TRY(counts.alloc((size_t)m * bins * sizeof(unsigned)));
+    BH_HIP(hipMemsetAsync(counts.p, 0, (size_t)m * bins * sizeof(unsigned), s->stream));
+    DevOut dl, dp, dr;
+    BH_TRY(dl.init(limits, (size_t)2 * m * sizeof(float)));
+    BH_TRY(dp.init(pdf, (size_t)bins * m * sizeof(float)));
+    BH_TRY(dr.init(bin_ranks, (size_t)bins * m * sizeof(float)));
+    // limits from the first cycle only (amd_gcn.clj:469-476)
+    BH_TRY(histogram_device(c, s->stream, s->xs.as<float>(), m, s->W, m, dl.as<float>(),
+                            counts.as<unsigned>(), true));
+    s->beta = 1.0f;
+    for (uint32_t i = 1; i < cycles; ++i) {
+        BH_TRY(bare_pair(s, 1.0f));
+        s->bare_counter += 1;
+        BH_TRY(histogram_device(c, s->stream, s->xs.as<float>(), m, s->W, m, dl.as<float>(),
+                                counts.as<unsigned>(), false));
+    }
+    s->iter += cycles - 1;
+    const float alpha = (float)((double)bins / ((double)cycles * s->W));
+    BH_HIP(bh::launch_histogram_finalize(s->stream, counts.as<unsigned>(), dl.as<float>(), alpha,
+                                         m, bins, dp.as<float>(), dr.as<float>()));
+    BH_TRY(dl.finish(s->stream));
+    BH_TRY(dp.finish(s->stream));
+    BH_TRY(dr.finish(s->stream));
+    return sync(s->stream);
+}
+
+static int stretch_moments(bh_stretch* s, float* out, bool variance) {
+    BH_ARG(s && out, "NULL argument");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    DevBuf mv;
+    BH_TRY(mv.alloc((size_t)2 * s->dim * sizeof(float)));
+    void* scratch = nullptr;
+    BH_TRY(ctx_scratch(s->ctx, bh::dataset_scratch_bytes(s->dim, s->W), &scratch));
+    float* mean = mv.as<float>();
+    float* var = mean + s->dim;
+    BH_HIP(bh::launch_moments(s->stream, s->xs.as<float>(), s->dim, s->W, s->dim, mean,
+                              variance ? var : nullptr, scratch));
+    BH_HIP(hipMemcpyAsync(out, variance ? var : mean, (size_t)s->dim * sizeof(float),
+                          hipMemcpyDefault, s->stream));
+    return sync(s->stream);
+}
+
+int bh_stretch_mean(bh_stretch* s, float* out) { return stretch_moments(s, out, false); }
+
+int bh_stretch_variance(bh_stretch* s, float* out) { return stretch_moments(s, out, true); }
+
+int bh_stretch_state_get(const bh_stretch* s, float* xs, float* logfn, uint32_t counters[6]) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    if (xs)
+        BH_HIP(hipMemcpyAsync(xs, s->xs.p, (size_t)s->W * s->dim * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+    if (logfn)
+        BH_HIP(hipMemcpyAsync(logfn, s->logfn.p, (size_t)s->W * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+    if (counters) {
+        counters[0] = s->bare_seed;
+        counters[1] = s->move_seed;
+        counters[2] = s->bare_counter;
+        counters[3] = s->move_counter;
+        counters[4] = (uint32_t)s->iter;
+        counters[5] = (uint32_t)(s->iter >> 32);
+    }
+    return sync(s->stream);
+}
+
+int bh_stretch_state_set(bh_stretch* s, const float* xs, const float* logfn,
+                         const uint32_t counters[6]) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    if (xs)
+        BH_HIP(hipMemcpyAsync(s->xs.p, xs, (size_t)s->W * s->dim * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+    if (logfn)
+        BH_HIP(hipMemcpyAsync(s->logfn.p, logfn, (size_t)s->W * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+    else if (xs)
+        BH_TRY(logfn_all(s));
+    if (counters) {
+        s->bare_seed = counters[0];
+        s->move_seed = counters[1];
+        s->bare_counter = counters[2];
+        s->move_counter = counters[3];
+        s->iter = (uint64_t)counters[4] | ((uint64_t)counters[5] << 32);
+    }
+    return sync(s->stream);
+}
+
+}  // extern "C"

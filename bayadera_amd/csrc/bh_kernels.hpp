@@ -1,0 +1,32 @@
+// Launchers of the ahead-of-time kernels (bh_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace bh {
+
+constexpr unsigned kMaxDatasetBlocks = 2048;  // 8 blocks per CU over 256 CUs
+
+hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits, unsigned dim,
+                               unsigned walkers, float* xs);
+hipError_t launch_uniform_sample(hipStream_t s, unsigned seed, float lo, float hi, unsigned quads,
+                                 float* x);
+size_t dataset_scratch_bytes(unsigned m, unsigned long long n);
+hipError_t launch_min_max(hipStream_t s, const float* a, unsigned m, unsigned long long n,
+                          unsigned ld, float* limits, void* scratch);
+hipError_t launch_histogram(hipStream_t s, const float* a, unsigned m, unsigned long long n,
+                            unsigned ld, const float* limits, unsigned bins, unsigned* counts);
+hipError_t launch_histogram_finalize(hipStream_t s, const unsigned* counts, const float* limits,
+                                     float alpha, unsigned m, unsigned bins, float* pdf,
+                                     float* ranks);
+hipError_t launch_moments(hipStream_t s, const float* a, unsigned m, unsigned long long n,
+                          unsigned ld, float* mean, float* var, void* scratch);
+hipError_t launch_row_tree_sum(hipStream_t s, const float* a, unsigned rows, unsigned count,
+                               unsigned local_n, unsigned long long row_stride, float* work,
+                               float scale, float* out, unsigned long long out_stride);
+hipError_t launch_sum_u32(hipStream_t s, const unsigned* v, unsigned n, unsigned long long* out);
+hipError_t launch_sum_f64(hipStream_t s, const double* v, unsigned n, double* out);
+hipError_t launch_acor(hipStream_t s, float* series, unsigned dim, unsigned n, unsigned lag,
+                       unsigned wgs, float* scratch, unsigned scratch_per_dim, float* tau,
+                       float* mean, float* sigma);
+
+}  // namespace bh

@@ -1,0 +1,342 @@
+// Model-dependent kernels of the HIP engine, compiled at run time by hipRTC
+// together with the user's model source (the reference compiles its OpenCL
+// engine the same way: amd_gcn.clj:573-631, with -DLOGFN/-DLOGPDF/-DLOGLIK,
+// -DDIM and -DWGS).  The host defines:
+//   WGS     reference work-group size: accounting groups and block size
+//   DIM     model dimension (stretch kernels only)
+//   LOGFN   mcmc log-density  REAL f(uint data_len, uint params_len, const REAL* params, uint dim, const REAL* x)
+//   LOGPDF  log-density       (same signature)
+//   LOGLIK  log-likelihood    REAL f(uint data_len, const REAL* data, uint dim, const REAL* x)
+//
+// Walker state is AoS, x[walker*DIM + d], the two half-ensembles contiguous,
+// exactly the layout the reference hands across its boundary (DIM x W
+// column-major).  The stretch move reads the partner walker's DIM floats as
+// one contiguous run, so a random partner costs one cache line rather than
+// DIM of them.
+
+#include "bh_philox.hpp"
+
+#define BH_PARTNER_FLAT 0
+#define BH_PARTNER_WALKER 1
+
+// Sum v over each aligned slice of WGS threads with the reference's
+// work-group tree (amd-gcn-mcmc-stretch.cl:5-29 for power-of-two WGS: level
+// i adds item l+i into item l, i = WGS/2 ... 1).  The levels i >= 64 cross
+// wavefronts and run in LDS; levels 32 ... 1 run inside the slice's first
+// wave with cross-lane moves.  Float addition is not associative, so the
+// tree shape is kept exactly: the result is bit-identical to the reference's.
+// Values are staged in buf[d*blockDim + t]; the total of dim d for the slice
+// is returned in out[d] by the slice's lane 0.
+template <int ND>
+__device__ __forceinline__ void bh_slice_tree_sums(const float (&v)[ND], float* buf,
+                                                   float (&out)[ND]) {
+    const unsigned t = threadIdx.x;
+    const unsigned nb = blockDim.x;
+    const unsigned l = t & (WGS - 1);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) buf[d * nb + t] = v[d];
+    __syncthreads();
+#pragma unroll
+    for (unsigned i = WGS / 2; i >= 64; i >>= 1) {
+        if (l < i) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) buf[d * nb + t] = buf[d * nb + t] + buf[d * nb + t + i];
+        }
+        __syncthreads();
+    }
+    if (l < 64) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            float x = buf[d * nb + t];
+#pragma unroll
+            for (int i = 32; i >= 1; i >>= 1) x += __shfl_down(x, i, 64);
+            out[d] = x;
+        }
+    }
+    __syncthreads();  // buf is reused by the caller
+}
+
+// Number of set flags over each slice of WGS threads (order-independent).
+__device__ __forceinline__ unsigned bh_slice_count(bool flag, unsigned* wave_counts) {
+    const unsigned t = threadIdx.x;
+    const unsigned long long m = __ballot(flag);
+    if ((t & 63u) == 0) wave_counts[t >> 6] = (unsigned)__popcll(m);
+    __syncthreads();
+    unsigned total = 0;
+    if ((t & (WGS - 1)) == 0) {
+        const unsigned w0 = t >> 6;
+        for (unsigned w = 0; w < WGS / 64; ++w) total += wave_counts[w0 + w];
+    }
+    __syncthreads();
+    return total;
+}
+
+#ifdef LOGFN
+
+// One walker's stretch move (amd-gcn-mcmc-stretch.cl:31-81).
+//   counter {k, step, odd_or_even, 0xbeeff00d}, key {seed, 0xdecafbad}
+//   z  = fma((a-2+1/a)*u1, u1, (2(1-1/a))*u1) + 1/a                     (:53-54)
+//   Y  = fma(z, Xk - Xj, Xj)                                           (:64)
+//   q  = isfinite(ly) ? z^(DIM-1) exp(beta (ly - lX)) : 0; accept iff u2 <= q (:68-71)
+// Partner: FLAT reads the complementary half at element offset
+// j0 = (uint)((u0*K)*DIM) (:57), wrapping modulo K*DIM where the reference
+// reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1).
+// xk holds the walker's coordinates on entry and its new ones on exit.
+__device__ __forceinline__ bool bh_stretch_walker(unsigned k, unsigned K, unsigned seed,
+                                                  unsigned odd_or_even, unsigned step, float a,
+                                                  float beta, int partner, const float* S,
+                                                  float (&xk)[DIM], float& lx,
+                                                  unsigned data_len, unsigned params_len,
+                                                  const float* params) {
+    const bh_f32x4 u = bh_u01x4(
+        bh_philox4x32_10(bh_u32x4{k, step, odd_or_even, BH_CTR3}, seed, BH_KEY1_STRETCH));
+    const float inv_a = 1.0f / a;
+    const float ca = a - 2.0f + inv_a;
+    const float cb = 2.0f * (1.0f - inv_a);
+    const float z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
+    float y[DIM];
+    if (partner == BH_PARTNER_FLAT) {
+        const unsigned total = K * DIM;
+        unsigned e = (unsigned)((u.x * (float)K) * (float)DIM);
+        if (e >= total) e -= total;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const float xj = S[e];
+            y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+            e = (e + 1 == total) ? 0u : e + 1;
+        }
+    } else {
+        unsigned j = (unsigned)(u.x * (float)K);
+        j = j < K ? j : K - 1;
+        const float* sj = S + (size_t)j * DIM;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const float xj = sj[i];
+            y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+        }
+    }
+    const float ly = LOGFN(data_len, params_len, params, DIM, y);
+    float zp = 1.0f;
+#pragma unroll
+    for (int i = 0; i < DIM - 1; ++i) zp *= z;
+    const float q = __builtin_isfinite(ly) ? zp * expf(beta * (ly - lx)) : 0.0f;
+    const bool accepted = u.z <= q;
+    if (accepted) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) xk[i] = y[i];
+        lx = ly;
+    }
+    return accepted;
+}
+
+// logfn kernel (amd-gcn-mcmc-stretch.cl:161-167): res[w] = LOGFN(x + DIM*w).
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_logfn(unsigned data_len, unsigned params_len, const float* __restrict__ params,
+         const float* __restrict__ x, float* __restrict__ res, unsigned n) {
+    const unsigned w = blockIdx.x * WGS + threadIdx.x;
+    if (w < n) res[w] = LOGFN(data_len, params_len, params, DIM, x + (size_t)w * DIM);
+}
+
+// Accounting of stretch_move_accu (amd-gcn-mcmc-stretch.cl:98-114) for the
+// slice of WGS threads holding group g: accept[g] += accepted count;
+// means[groups*DIM*step + i*groups + g] += tree sum of coordinate i.
+// Dimensions are reduced BH_ACCU_CHUNK at a time so the LDS staging area
+// stays at BH_ACCU_CHUNK floats per thread whatever DIM is.
+#define BH_ACCU_CHUNK (DIM < 8 ? DIM : 8)
+
+__device__ __forceinline__ void bh_accu(const float (&xk)[DIM], bool accepted, unsigned g,
+                                        unsigned groups, unsigned mstep, unsigned* accept,
+                                        float* means, float* red, unsigned* wave_counts) {
+    const unsigned cnt = bh_slice_count(accepted, wave_counts);
+    const bool leader = (threadIdx.x & (WGS - 1)) == 0;
+    if (leader) accept[g] += cnt;
+    float* m = means + (size_t)groups * DIM * mstep + g;
+#pragma unroll
+    for (int d0 = 0; d0 < DIM; d0 += BH_ACCU_CHUNK) {
+        float v[BH_ACCU_CHUNK], sums[BH_ACCU_CHUNK];
+#pragma unroll
+        for (int c = 0; c < BH_ACCU_CHUNK; ++c) v[c] = (d0 + c < DIM) ? xk[d0 + c] : 0.0f;
+        bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums);
+        if (leader) {
+#pragma unroll
+            for (int c = 0; c < BH_ACCU_CHUNK; ++c)
+                if (d0 + c < DIM) m[(size_t)(d0 + c) * groups] += sums[c];
+        }
+    }
+}
+
+// One half-step over K walkers, one thread per walker, WGS walkers per block
+// (stretch_move_bare / stretch_move_accu, amd-gcn-mcmc-stretch.cl:83-134).
+template <bool ACCU>
+__device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_even,
+                                                unsigned data_len, unsigned params_len,
+                                                const float* __restrict__ params,
+                                                const float* __restrict__ S, float* __restrict__ X,
+                                                float* __restrict__ logfn_X, float a, float beta,
+                                                unsigned step, unsigned K, int partner,
+                                                unsigned* __restrict__ accept,
+                                                float* __restrict__ means) {
+    const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    float xk[DIM];
+    float* xp = X + (size_t)k * DIM;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) xk[i] = xp[i];
+    float lx = logfn_X[k];
+    const bool accepted = bh_stretch_walker(k, K, seed, odd_or_even, step, a, beta, partner, S,
+                                            xk, lx, data_len, params_len, params);
+    if (accepted) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) xp[i] = xk[i];
+        logfn_X[k] = lx;
+    }
+    if (ACCU) {
+        __shared__ float red[BH_ACCU_CHUNK * WGS];
+        __shared__ unsigned wave_counts[WGS / 64];
+        bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, red, wave_counts);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_stretch_move_bare(unsigned seed, unsigned odd_or_even, unsigned data_len, unsigned params_len,
+                     const float* __restrict__ params, const float* __restrict__ S,
+                     float* __restrict__ X, float* __restrict__ logfn_X, float a, float beta,
+                     unsigned step, unsigned K, int partner) {
+    bh_stretch_half<false>(seed, odd_or_even, data_len, params_len, params, S, X, logfn_X, a,
+                           beta, step, K, partner, nullptr, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_stretch_move_accu(unsigned seed, unsigned odd_or_even, unsigned data_len,
+                     unsigned params_len, const float* __restrict__ params,
+                     const float* __restrict__ S, float* __restrict__ X,
+                     float* __restrict__ logfn_X, unsigned* __restrict__ accept,
+                     float* __restrict__ means, float a, unsigned step, unsigned K,
+                     int partner) {
+    bh_stretch_half<true>(seed, odd_or_even, data_len, params_len, params, S, X, logfn_X, a,
+                          1.0f, step, K, partner, accept, means);
+}
+
+// Resident multi-step kernel: one work-group holds the whole ensemble
+// (positions and logfn values) in LDS and runs nsteps full steps -- both
+// half-steps each -- without returning to the host.  Every walker's
+// arithmetic and every accounting tree is the same as in the per-half-step
+// kernels, so the results are bit-identical; what goes away is two kernel
+// launches per step, which bound small ensembles (W = 1024: 16 KB of state).
+//   mode 0: bare steps (burn-in!, anneal! with betas[s], sample! with out)
+//   mode 1: accounting steps (move!): accept/means as stretch_move_accu
+// blockDim.x is a multiple of WGS dividing K.
+extern "C" __global__ void __launch_bounds__(1024)
+bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsigned c_even,
+                    unsigned data_len, unsigned params_len, const float* __restrict__ params,
+                    float* __restrict__ xs, float* __restrict__ logfn, unsigned K, float a,
+                    float beta, const float* __restrict__ betas, unsigned step0,
+                    unsigned nsteps, int partner, int mode, unsigned* __restrict__ accept,
+                    float* __restrict__ means, float* __restrict__ sample_out,
+                    unsigned long long sample_floats) {
+    extern __shared__ float lds[];
+    const unsigned W = 2 * K;
+    const unsigned nb = blockDim.x;
+    const unsigned t = threadIdx.x;
+    float* sx = lds;                       // W*DIM
+    float* sl = sx + (size_t)W * DIM;      // W
+    float* red = sl + W;                   // BH_ACCU_CHUNK*nb (mode 1)
+    unsigned* wave_counts = (unsigned*)(red + (mode == 1 ? BH_ACCU_CHUNK * nb : 0));
+    for (unsigned i = t; i < W * DIM; i += nb) sx[i] = xs[i];
+    for (unsigned i = t; i < W; i += nb) sl[i] = logfn[i];
+    __syncthreads();
+    const unsigned groups = K / WGS;
+    unsigned long long written = 0;
+    for (unsigned s = 0; s < nsteps; ++s) {
+        const unsigned step = step0 + s;
+        const float b = betas ? betas[s] : beta;
+        for (int h = 0; h < 2; ++h) {
+            float* X = sx + (h ? (size_t)K * DIM : 0);
+            const float* S = sx + (h ? 0 : (size_t)K * DIM);
+            float* L = sl + (h ? K : 0);
+            const unsigned seed = h ? seed_even : seed_odd;
+            const unsigned oe = h ? c_even : c_odd;
+            for (unsigned k0 = 0; k0 < K; k0 += nb) {
+                const unsigned k = k0 + t;
+                float xk[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
+                float lx = L[k];
+                const bool acc = bh_stretch_walker(k, K, seed, oe, step, a, mode == 1 ? 1.0f : b,
+                                                   partner, S, xk, lx, data_len, params_len,
+                                                   params);
+                if (acc) {
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) X[(size_t)k * DIM + i] = xk[i];
+                    L[k] = lx;
+                }
+                if (mode == 1)
+                    bh_accu(xk, acc, k / WGS, groups, step, accept, means, red, wave_counts);
+            }
+            __syncthreads();
+        }
+        if (sample_out && written < sample_floats) {
+            const unsigned long long left = sample_floats - written;
+            const unsigned chunk = left < (unsigned long long)W * DIM ? (unsigned)left : W * DIM;
+            for (unsigned i = t; i < chunk; i += nb) sample_out[written + i] = sx[i];
+            written += chunk;
+        }
+    }
+    for (unsigned i = t; i < W * DIM; i += nb) xs[i] = sx[i];
+    for (unsigned i = t; i < W; i += nb) logfn[i] = sl[i];
+}
+
+#endif  // LOGFN
+
+#ifdef LOGPDF
+// logpdf / pdf kernels (amd-gcn-distribution.cl:1-14).
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_logpdf(unsigned data_len, unsigned params_len, const float* __restrict__ params, unsigned dim,
+          const float* __restrict__ x, float* __restrict__ res, unsigned long long n) {
+    const unsigned long long j = (unsigned long long)blockIdx.x * WGS + threadIdx.x;
+    if (j < n) res[j] = LOGPDF(data_len, params_len, params, dim, x + j * dim);
+}
+
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_pdf(unsigned data_len, unsigned params_len, const float* __restrict__ params, unsigned dim,
+       const float* __restrict__ x, float* __restrict__ res, unsigned long long n) {
+    const unsigned long long j = (unsigned long long)blockIdx.x * WGS + threadIdx.x;
+    if (j < n) res[j] = native_exp(LOGPDF(data_len, params_len, params, dim, x + j * dim));
+}
+#endif  // LOGPDF
+
+#ifdef LOGLIK
+// loglik / lik / evidence kernels (amd-gcn-likelihood.cl:1-29); the
+// evidence accumulates in double (ACCUMULATOR=double, amd_gcn.clj:580).
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_loglik(unsigned data_len, const float* __restrict__ data, unsigned dim,
+          const float* __restrict__ x, float* __restrict__ res, unsigned long long n) {
+    const unsigned long long j = (unsigned long long)blockIdx.x * WGS + threadIdx.x;
+    if (j < n) res[j] = LOGLIK(data_len, data, dim, x + j * dim);
+}
+
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_lik(unsigned data_len, const float* __restrict__ data, unsigned dim,
+       const float* __restrict__ x, float* __restrict__ res, unsigned long long n) {
+    const unsigned long long j = (unsigned long long)blockIdx.x * WGS + threadIdx.x;
+    if (j < n) res[j] = native_exp(LOGLIK(data_len, data, dim, x + j * dim));
+}
+
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_evidence_partial(unsigned data_len, const float* __restrict__ data, unsigned dim,
+                    const float* __restrict__ x, unsigned long long n,
+                    double* __restrict__ partial) {
+    const unsigned long long j = (unsigned long long)blockIdx.x * WGS + threadIdx.x;
+    double v = 0.0;
+    if (j < n) v = (double)native_exp(LOGLIK(data_len, data, dim, x + j * dim));
+#pragma unroll
+    for (int i = 32; i >= 1; i >>= 1) v += __shfl_down(v, i, 64);
+    __shared__ double wsum[WGS / 64];
+    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < WGS / 64; ++w) s += wsum[w];
+        partial[blockIdx.x] = s;
+    }
+}
+#endif  // LOGLIK

@@ -1,0 +1,468 @@
+"""The HIP engine behind the reference's protocols.
+
+One class per engine type of ``src/clojure/uncomplicate/bayadera/internal/
+device/amd_gcn.clj``, with the protocol methods of ``internal/protocols.clj``
+under their Python names (``burn-in!`` -> ``burn_in`` and so on).  Every
+method is a call through the C-ABI of ``libbayadera_hip.so``.
+
+Arrays: a DIM x n column-major matrix of the reference is a numpy array of
+shape (n, DIM) in C order (the same memory), or a CUDA torch tensor of that
+shape, which the engine reads in place.  Results come back as numpy arrays
+(the reference's ``native!``), or are written into a CUDA tensor passed as
+``out``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+from .models import DistributionModel, LikelihoodModel
+
+WGS_DEFAULT = 256
+
+
+def _source(model) -> bytes:
+    return "\n".join(model.source).encode()
+
+
+class _Arr:
+    """A float32 (or other dtype) array as a pointer the C-ABI can take."""
+
+    def __init__(self, x, dtype=np.float32):
+        if hasattr(x, "data_ptr") and hasattr(x, "is_cuda"):
+            import torch
+            want = {np.float32: torch.float32, np.float64: torch.float64,
+                    np.uint32: torch.int32}[dtype]
+            if x.dtype != want or not x.is_contiguous():
+                raise ValueError(f"device tensors must be contiguous {want}")
+            self.keep = x
+            self.ptr = x.data_ptr()
+            self.size = x.numel()
+        else:
+            a = np.ascontiguousarray(x, dtype=dtype)
+            self.keep = a
+            self.ptr = a.ctypes.data
+            self.size = a.size
+
+
+def _out(out, shape, dtype=np.float32):
+    if out is not None:
+        return out, _Arr(out, dtype)
+    a = np.zeros(shape, dtype)
+    return a, _Arr(a, dtype)
+
+
+@dataclass
+class Histogram:
+    """protocols.clj:13-18.  limits (DIM, 2); pdf and bin_ranks (DIM, WGS)
+    (the reference's 2 x DIM and WGS x DIM column-major matrices)."""
+    limits: np.ndarray
+    pdf: np.ndarray
+    bin_ranks: np.ndarray
+
+
+@dataclass
+class Autocorrelation:
+    """protocols.clj:111-116."""
+    tau: np.ndarray
+    mean: np.ndarray
+    sigma: np.ndarray
+    steps: int
+    lag: int
+
+
+class _Handle:
+    _destroy = ""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def release(self):
+        if self._h and self._h.value:
+            getattr(_lib.load(), self._destroy)(self._h)
+            self._h = ctypes.c_void_p()
+        return True
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.release()
+
+
+def _build_log():
+    return ctypes.create_string_buffer(16384)
+
+
+# ---------------------------------------------------------------------------
+class HipBayaderaFactory(_Handle):
+    """GCNBayaderaFactory (amd_gcn.clj:635-678): the EngineFactory."""
+    _destroy = "bh_ctx_destroy"
+
+    def __init__(self, device: int = 0, wgs: int = WGS_DEFAULT):
+        super().__init__()
+        call("bh_ctx_create", int(device), int(wgs), ctypes.byref(self._h))
+        cu = ctypes.c_int()
+        w = ctypes.c_uint32()
+        call("bh_ctx_info", self._h, ctypes.byref(cu), ctypes.byref(w))
+        self.device, self.compute_units, self.wgs = device, cu.value, w.value
+        self._dataset = HipDatasetEngine(self)
+        self._acor = HipAcorEngine(self)
+
+    # EngineFactory
+    def distribution_engine(self, model: DistributionModel) -> "HipDistributionEngine":
+        return HipDistributionEngine(self, model)
+
+    def likelihood_engine(self, model: LikelihoodModel) -> "HipLikelihoodEngine":
+        return HipLikelihoodEngine(self, model)
+
+    def dataset_engine(self) -> "HipDatasetEngine":
+        return self._dataset
+
+    def acor_engine(self) -> "HipAcorEngine":
+        return self._acor
+
+    def direct_sampler_engine(self, model: DistributionModel) -> "HipDirectSamplerEngine":
+        return HipDirectSamplerEngine(self, model)
+
+    def mcmc_factory(self, model: DistributionModel) -> "HipStretchFactory":
+        return HipStretchFactory(self, model)
+
+    def processing_elements(self) -> int:
+        return self.compute_units * self.wgs
+
+
+class HipDirectSamplerEngine:
+    """GCNDirectSamplerEngine (amd_gcn.clj:49-62); the uniform model."""
+
+    def __init__(self, factory: HipBayaderaFactory, model: DistributionModel):
+        if model.name != "uniform":
+            raise NotImplementedError(f"no direct sampler for {model.name}")
+        self.factory = factory
+
+    def sample(self, seed: int, params, n: int, out=None):
+        if n % 4:
+            raise ValueError("GCN direct sampler supports only matrices with ncols and offset "
+                             "that are multiple of 4.")
+        p = np.asarray(params, np.float32)
+        res, o = _out(out, (n,))
+        call("bh_uniform_sample", self.factory.handle, int(seed) & 0xFFFFFFFF, float(p[0]),
+             float(p[1]), o.ptr, n)
+        return res
+
+
+class HipDistributionEngine(_Handle):
+    """GCNDistributionEngine (amd_gcn.clj:66-93): DensityEngine."""
+    _destroy = "bh_dist_engine_destroy"
+
+    def __init__(self, factory: HipBayaderaFactory, model: DistributionModel):
+        super().__init__()
+        self.factory, self.dist_model = factory, model
+        log = _build_log()
+        call("bh_dist_engine_create", factory.handle, _source(model), model.logpdf.encode(),
+             int(model.params_size), ctypes.byref(self._h), log, len(log))
+
+    def model(self):
+        return self.dist_model
+
+    def _density(self, fn, params, x, out):
+        p = _Arr(params)
+        xa = _Arr(x)
+        n, dim = _shape(x)
+        res, o = _out(out, (n,))
+        call(fn, self._h, p.ptr, p.size, xa.ptr, dim, n, o.ptr)
+        return res
+
+    def log_density(self, params, x, out=None):
+        return self._density("bh_log_density", params, x, out)
+
+    def density(self, params, x, out=None):
+        return self._density("bh_density", params, x, out)
+
+
+class HipLikelihoodEngine(_Handle):
+    """GCNLikelihoodEngine (amd_gcn.clj:97-133): DensityEngine + LikelihoodEngine."""
+    _destroy = "bh_lik_engine_destroy"
+
+    def __init__(self, factory: HipBayaderaFactory, model: LikelihoodModel):
+        super().__init__()
+        self.factory, self.lik_model = factory, model
+        log = _build_log()
+        call("bh_lik_engine_create", factory.handle, _source(model), model.loglik.encode(),
+             ctypes.byref(self._h), log, len(log))
+
+    def model(self):
+        return self.lik_model
+
+    def _run(self, fn, data, x, out):
+        d = _Arr(data)
+        xa = _Arr(x)
+        n, dim = _shape(x)
+        res, o = _out(out, (n,))
+        call(fn, self._h, d.ptr, d.size, xa.ptr, dim, n, o.ptr)
+        return res
+
+    def log_density(self, data, x, out=None):
+        return self._run("bh_loglik", data, x, out)
+
+    def density(self, data, x, out=None):
+        return self._run("bh_lik", data, x, out)
+
+    def evidence(self, data, x) -> float:
+        d = _Arr(data)
+        xa = _Arr(x)
+        n, dim = _shape(x)
+        r = ctypes.c_double()
+        call("bh_evidence", self._h, d.ptr, d.size, xa.ptr, dim, n, ctypes.byref(r))
+        return r.value
+
+
+def _shape(x):
+    s = tuple(x.shape)
+    if len(s) == 1:
+        return s[0], 1
+    if len(s) != 2:
+        raise ValueError("expected a (n, DIM) array")
+    return s[0], s[1]
+
+
+class HipDatasetEngine:
+    """GCNDatasetEngine (amd_gcn.clj:137-214): DatasetEngine + EstimateEngine."""
+
+    def __init__(self, factory: HipBayaderaFactory):
+        self.factory = factory
+
+    def data_mean(self, data, out=None):
+        return self._moment("bh_dataset_mean", data, out)
+
+    def data_variance(self, data, out=None):
+        return self._moment("bh_dataset_variance", data, out)
+
+    def _moment(self, fn, data, out):
+        a = _Arr(data)
+        n, m = _shape(data)
+        res, o = _out(out, (m,))
+        call(fn, self.factory.handle, a.ptr, m, n, m, o.ptr)
+        return res
+
+    def histogram(self, data) -> Histogram:
+        a = _Arr(data)
+        n, m = _shape(data)
+        bins = self.factory.wgs
+        lim = np.zeros((m, 2), np.float32)
+        pdf = np.zeros((m, bins), np.float32)
+        ranks = np.zeros((m, bins), np.float32)
+        call("bh_dataset_histogram", self.factory.handle, a.ptr, m, n, m, lim.ctypes.data,
+             pdf.ctypes.data, ranks.ctypes.data)
+        return Histogram(lim, pdf, ranks)
+
+
+class HipAcorEngine:
+    """GCNAcorEngine (amd_gcn.clj:216-267)."""
+
+    def __init__(self, factory: HipBayaderaFactory):
+        self.factory = factory
+
+    def acor(self, series) -> Autocorrelation:
+        a = _Arr(series)
+        n, dim = _shape(series)
+        tau = np.zeros(dim, np.float32)
+        mean = np.zeros(dim, np.float32)
+        sigma = np.zeros(dim, np.float32)
+        lag = ctypes.c_uint32()
+        call("bh_acor", self.factory.handle, a.ptr, dim, n, tau.ctypes.data, mean.ctypes.data,
+             sigma.ctypes.data, ctypes.byref(lag))
+        return Autocorrelation(tau, mean, sigma, n, lag.value)
+
+
+# ---------------------------------------------------------------------------
+class HipStretchFactory(_Handle):
+    """GCNStretchFactory / gcn-stretch-factory (amd_gcn.clj:509-567, 623-631):
+    the model's mcmc-logpdf compiled with DIM and WGS into the engine kernels."""
+    _destroy = "bh_stretch_factory_destroy"
+
+    def __init__(self, factory: HipBayaderaFactory, model: DistributionModel):
+        super().__init__()
+        self.factory, self.dist_model = factory, model
+        log = _build_log()
+        call("bh_stretch_factory_create", factory.handle, _source(model),
+             model.mcmc_logpdf.encode(), int(model.dimension), int(model.params_size),
+             ctypes.byref(self._h), log, len(log))
+
+    def create_sampler(self, seed: int, walkers: int, params) -> "HipStretch":
+        return HipStretch(self, seed, walkers, params)
+
+
+class HipStretch(_Handle):
+    """GCNStretch (amd_gcn.clj:271-507): MCMC, MCMCStretch, RandomSampler,
+    EstimateEngine, Location, Spread."""
+    _destroy = "bh_stretch_destroy"
+
+    def __init__(self, sf: HipStretchFactory, seed: int, walkers: int, params):
+        super().__init__()
+        self.sf = sf
+        self.dim = sf.dist_model.dimension
+        self.walkers = int(walkers)
+        self.wgs = sf.factory.wgs
+        p = _Arr(params)
+        self._params = p
+        call("bh_stretch_create", sf.handle, int(seed) & 0xFFFFFFFF, self.walkers, p.ptr, p.size,
+             ctypes.byref(self._h))
+
+    def model(self):
+        return self.sf.dist_model
+
+    # engine options (no reference counterpart: the reference has one mode)
+    def set_partner(self, mode: int):
+        call("bh_stretch_set_partner", self._h, int(mode))
+        return self
+
+    def set_exec(self, mode: int):
+        call("bh_stretch_set_exec", self._h, int(mode))
+        return self
+
+    def set_timing(self, on: bool = True):
+        call("bh_stretch_set_timing", self._h, int(bool(on)))
+        return self
+
+    def last_gpu_ms(self) -> float:
+        ms = ctypes.c_double()
+        call("bh_stretch_last_gpu_ms", self._h, ctypes.byref(ms))
+        return ms.value
+
+    # Info
+    def info(self) -> dict:
+        w = ctypes.c_uint32()
+        it = ctypes.c_uint64()
+        call("bh_stretch_info", self._h, ctypes.byref(w), ctypes.byref(it))
+        return {"walker-count": w.value, "iteration-counter": it.value}
+
+    # MCMC
+    def init(self, seed: int):
+        call("bh_stretch_init", self._h, int(seed) & 0xFFFFFFFF)
+        return self
+
+    def init_position(self, seed_or_position, limits=None):
+        if isinstance(seed_or_position, HipStretch):
+            call("bh_stretch_init_position_from", self._h, seed_or_position.handle)
+        else:
+            lim = _Arr(np.asarray(limits, np.float32).reshape(-1))
+            if lim.size != 2 * self.dim:
+                raise ValueError(f"limits must be 2 x {self.dim}")
+            call("bh_stretch_init_position", self._h, int(seed_or_position) & 0xFFFFFFFF, lim.ptr)
+        return self
+
+    def burn_in(self, n: int, a: float):
+        call("bh_stretch_burn_in", self._h, int(n), float(a))
+        return self
+
+    def anneal(self, schedule, n: int, a: float):
+        temps = np.array([schedule(i) for i in range(int(n))], np.float64)
+        call("bh_stretch_anneal", self._h, temps.ctypes.data, int(n), float(a))
+        return self
+
+    def acc_rate(self, a: float) -> float:
+        r = ctypes.c_double()
+        call("bh_stretch_acc_rate", self._h, float(a), ctypes.byref(r))
+        return r.value
+
+    def run_sampler(self, n: int, a: float, means_out=None) -> dict:
+        D = self.dim
+        tau = np.zeros(D, np.float32)
+        mean = np.zeros(D, np.float32)
+        sigma = np.zeros(D, np.float32)
+        lag = ctypes.c_uint32()
+        acc = ctypes.c_double()
+        means, m = _out(means_out, (int(n), D))
+        rc = _lib.load().bh_stretch_run(self._h, int(n), float(a), ctypes.byref(acc), m.ptr,
+                                        tau.ctypes.data, mean.ctypes.data, sigma.ctypes.data,
+                                        ctypes.byref(lag))
+        _lib.check(rc, "run-sampler!")
+        return {"acceptance-rate": acc.value, "a": float(np.float32(a)),
+                "autocorrelation": Autocorrelation(tau, mean, sigma, int(n), lag.value),
+                "means": means}
+
+    # MCMCStretch
+    def init_move(self, a: float, steps: int = 1):
+        call("bh_stretch_init_move", self._h, float(a), int(steps))
+        return self
+
+    def move(self):
+        call("bh_stretch_move", self._h)
+        return self
+
+    def move_bare(self):
+        call("bh_stretch_move_bare", self._h)
+        return self
+
+    def set_temperature(self, t: float):
+        call("bh_stretch_set_temperature", self._h, float(t))
+        return self
+
+    def read_accounting(self, steps: int):
+        groups = self.walkers // 2 // self.wgs
+        acc = np.zeros(groups, np.uint32)
+        means = np.zeros(self.dim * groups * steps, np.float32)
+        call("bh_stretch_read_accounting", self._h, acc.ctypes.data, means.ctypes.data, int(steps))
+        return acc, means
+
+    # RandomSampler
+    def sample(self, n: int | None = None, out=None):
+        n = self.walkers if n is None else int(n)
+        res, o = _out(out, (n, self.dim))
+        call("bh_stretch_sample", self._h, o.ptr, n)
+        return res
+
+    # EstimateEngine
+    def histogram(self, cycles: int = 1) -> Histogram:
+        bins = self.wgs
+        lim = np.zeros((self.dim, 2), np.float32)
+        pdf = np.zeros((self.dim, bins), np.float32)
+        ranks = np.zeros((self.dim, bins), np.float32)
+        call("bh_stretch_histogram", self._h, int(cycles), lim.ctypes.data, pdf.ctypes.data,
+             ranks.ctypes.data)
+        return Histogram(lim, pdf, ranks)
+
+    # Location / Spread
+    def mean(self) -> np.ndarray:
+        out = np.zeros(self.dim, np.float32)
+        call("bh_stretch_mean", self._h, out.ctypes.data)
+        return out
+
+    def variance(self) -> np.ndarray:
+        out = np.zeros(self.dim, np.float32)
+        call("bh_stretch_variance", self._h, out.ctypes.data)
+        return out
+
+    def sd(self) -> np.ndarray:
+        return np.sqrt(self.variance())
+
+    # state for resume
+    def state(self):
+        xs = np.zeros((self.walkers, self.dim), np.float32)
+        lf = np.zeros(self.walkers, np.float32)
+        c = np.zeros(6, np.uint32)
+        call("bh_stretch_state_get", self._h, xs.ctypes.data, lf.ctypes.data, c.ctypes.data)
+        return xs, lf, c
+
+    def set_state(self, xs=None, logfn=None, counters=None):
+        x = None if xs is None else _Arr(xs)
+        lf = None if logfn is None else _Arr(logfn)
+        c = None if counters is None else _Arr(counters, np.uint32)
+        call("bh_stretch_state_set", self._h, None if x is None else x.ptr,
+             None if lf is None else lf.ptr, None if c is None else c.ptr)
+        return self

@@ -1,0 +1,212 @@
+/*
+ * bayadera_hip.h -- C-ABI of the MI355X engine for Bayadera's stretch-sampler
+ * path.
+ *
+ * This is the boundary a host binds through its FFI (JNA from Clojure,
+ * ctypes from Python): plain C types, opaque handles, caller-owned arrays.
+ * It replaces the OpenCL engine of the reference,
+ * src/clojure/uncomplicate/bayadera/internal/device/amd_gcn.clj (cited below
+ * as amd_gcn.clj), behind the protocols of
+ * src/clojure/uncomplicate/bayadera/internal/protocols.clj (protocols.clj).
+ *
+ * Conventions
+ *  - Every function returns BH_OK (0) or a negative BH_ERR_* code; the
+ *    calling thread's last error message is bh_last_error().
+ *  - Matrices are column-major DIM x n blocks, as neanderthal's `ge`
+ *    (element (d, j) at a[j*ld + d]).
+ *  - Array arguments may be host or device pointers (the engine asks the HIP
+ *    runtime which); results land where the pointer points.
+ *  - Each handle owns one HIP stream; calls on one handle are ordered and a
+ *    call returns after its results are written.  Handles are not
+ *    thread-safe (as the reference's engines are not, amd_gcn.clj:273-274).
+ *  - Model sources are the reference's C99 plugin dialect (REAL, native_exp,
+ *    native_log, pown, lgamma, NAN), compiled at run time with hipRTC, as
+ *    the reference compiles OpenCL C at run time (amd_gcn.clj:573-631).
+ */
+#ifndef BAYADERA_HIP_H
+#define BAYADERA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_OK 0
+#define BH_ERR_ARG (-1)     /* invalid argument (dragan-says-ex / IllegalArgumentException) */
+#define BH_ERR_HIP (-2)     /* HIP runtime error */
+#define BH_ERR_RTC (-3)     /* model source failed to compile */
+#define BH_ERR_RANGE (-4)   /* size rule violated (walker count, acor series too short) */
+#define BH_ERR_STATE (-5)   /* call out of order (e.g. move before init-move) */
+
+#define BH_PARTNER_FLAT 0   /* the reference's partner offset (amd-gcn-mcmc-stretch.cl:57) */
+#define BH_PARTNER_WALKER 1 /* whole-walker partner (Goodman-Weare as published) */
+
+#define BH_EXEC_AUTO 0      /* resident kernel when the ensemble fits in LDS */
+#define BH_EXEC_STEPWISE 1  /* two kernels per step, as the reference */
+#define BH_EXEC_RESIDENT 2  /* one work-group runs many steps from LDS */
+
+typedef struct bh_ctx bh_ctx;
+typedef struct bh_dist_engine bh_dist_engine;
+typedef struct bh_lik_engine bh_lik_engine;
+typedef struct bh_stretch_factory bh_stretch_factory;
+typedef struct bh_stretch bh_stretch;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* bh_last_error(void);
+/* ABI version: major*10000 + minor*100 + patch. */
+int bh_version(void);
+
+/* Compile a model source with the engine kernels for `arch` (e.g. "gfx950")
+ * without a device: the check a host runs before creating engines (the
+ * reference surfaces the same errors from build-program!, amd_gcn.clj:585-627).
+ * Any of the three function names may be NULL; dim applies to logfn. */
+int bh_rtc_compile_check(const char* source, const char* logfn_name, const char* logpdf_name,
+                         const char* loglik_name, uint32_t dim, uint32_t wgs, const char* arch,
+                         char* log, size_t log_size);
+
+/* ---------------------------------------------------------------------- */
+/* Factory: GCNBayaderaFactory / gcn-bayadera-factory (amd_gcn.clj:635-678) */
+/* ---------------------------------------------------------------------- */
+/* wgs = the reference's WGS: work-group size, histogram bin count and
+ * walker-count quantum (amd_gcn.clj:516, 678); a power of two in [64, 1024]. */
+int bh_ctx_create(int device, uint32_t wgs, bh_ctx** out);
+int bh_ctx_destroy(bh_ctx* ctx);
+/* compute units and WGS; processing-elements = cu * wgs (amd_gcn.clj:663-664) */
+int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs);
+
+/* ---------------------------------------------------------------------- */
+/* Direct sampler: GCNDirectSamplerEngine.sample for the uniform model      */
+/* (amd_gcn.clj:49-62, uniform-sampler.cl:19-34).  n a multiple of 4.      */
+/* ---------------------------------------------------------------------- */
+int bh_uniform_sample(bh_ctx* ctx, uint32_t seed, float lower, float upper, float* out,
+                      uint64_t n);
+
+/* ---------------------------------------------------------------------- */
+/* Distribution engine: gcn-distribution-engine + GCNDistributionEngine      */
+/* (amd_gcn.clj:599-605, 66-93).  logpdf_name has the signature             */
+/* REAL f(uint data_len, uint params_len, const REAL* params, uint dim,     */
+/*        const REAL* x).  data_len = params_count - params_len (:75-77).    */
+/* ---------------------------------------------------------------------- */
+int bh_dist_engine_create(bh_ctx* ctx, const char* source, const char* logpdf_name,
+                          uint32_t params_len, bh_dist_engine** out, char* log, size_t log_size);
+int bh_dist_engine_destroy(bh_dist_engine* e);
+/* DensityEngine.log-density / density: res[j] for the n columns of x (dim x n) */
+int bh_log_density(bh_dist_engine* e, const float* params, uint32_t params_count, const float* x,
+                   uint32_t dim, uint64_t n, float* res);
+int bh_density(bh_dist_engine* e, const float* params, uint32_t params_count, const float* x,
+               uint32_t dim, uint64_t n, float* res);
+
+/* ---------------------------------------------------------------------- */
+/* Likelihood engine: gcn-likelihood-engine + GCNLikelihoodEngine           */
+/* (amd_gcn.clj:607-613, 97-133).  loglik_name has the signature            */
+/* REAL f(uint data_len, const REAL* data, uint dim, const REAL* x).        */
+/* ---------------------------------------------------------------------- */
+int bh_lik_engine_create(bh_ctx* ctx, const char* source, const char* loglik_name,
+                         bh_lik_engine** out, char* log, size_t log_size);
+int bh_lik_engine_destroy(bh_lik_engine* e);
+int bh_loglik(bh_lik_engine* e, const float* data, uint32_t data_len, const float* x,
+              uint32_t dim, uint64_t n, float* res);
+int bh_lik(bh_lik_engine* e, const float* data, uint32_t data_len, const float* x, uint32_t dim,
+           uint64_t n, float* res);
+/* LikelihoodEngine.evidence: mean over the n columns of exp(loglik), FP64
+ * accumulator (amd_gcn.clj:122-133, 580) */
+int bh_evidence(bh_lik_engine* e, const float* data, uint32_t data_len, const float* x,
+                uint32_t dim, uint64_t n, double* out);
+
+/* ---------------------------------------------------------------------- */
+/* Dataset engine: GCNDatasetEngine (amd_gcn.clj:137-214)                   */
+/* ---------------------------------------------------------------------- */
+/* DatasetEngine.data-mean / data-variance (population variance, divide by
+ * n, :159, 180), FP64 accumulation; out has m floats. */
+int bh_dataset_mean(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                    float* out);
+int bh_dataset_variance(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                        float* out);
+/* EstimateEngine.histogram: Histogram{limits 2 x m, pdf WGS x m, bin-ranks
+ * WGS x m} (protocols.clj:13-18); WGS bins per row (amd-gcn-estimate.cl:1-25). */
+int bh_dataset_histogram(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
+                         float* limits, float* pdf, float* bin_ranks);
+
+/* ---------------------------------------------------------------------- */
+/* Autocorrelation: GCNAcorEngine.acor (amd_gcn.clj:216-267)                */
+/* series dim x n column-major (the per-step ensemble means).  Returns      */
+/* BH_ERR_RANGE when n < 5*lag (:264-267).  lag may be NULL.                */
+/* ---------------------------------------------------------------------- */
+int bh_acor(bh_ctx* ctx, const float* series, uint32_t dim, uint32_t n, float* tau, float* mean,
+            float* sigma, uint32_t* lag);
+
+/* ---------------------------------------------------------------------- */
+/* Stretch sampler factory: gcn-stretch-factory (amd_gcn.clj:623-631) with  */
+/* LOGFN = logfn_name (the model's mcmc-logpdf) and DIM = dim.              */
+/* ---------------------------------------------------------------------- */
+int bh_stretch_factory_create(bh_ctx* ctx, const char* source, const char* logfn_name,
+                              uint32_t dim, uint32_t params_len, bh_stretch_factory** out,
+                              char* log, size_t log_size);
+int bh_stretch_factory_destroy(bh_stretch_factory* f);
+
+/* SamplerFactory.create-sampler (amd_gcn.clj:514-567): walkers a multiple of
+ * 2*WGS (BH_ERR_RANGE otherwise); params = [data, hyperparameters]; calls
+ * init! with seed. */
+int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
+                      const float* params, uint32_t params_count, bh_stretch** out);
+int bh_stretch_destroy(bh_stretch* s);
+int bh_stretch_set_partner(bh_stretch* s, int partner_mode);
+int bh_stretch_set_exec(bh_stretch* s, int exec_mode);
+/* Info.info: {walker-count, iteration-counter} (amd_gcn.clj:321-324) */
+int bh_stretch_info(const bh_stretch* s, uint32_t* walkers, uint64_t* iteration_counter);
+
+/* MCMC (protocols.clj:97-103) */
+int bh_stretch_init(bh_stretch* s, uint32_t seed);                               /* init! :378-387 */
+int bh_stretch_init_position(bh_stretch* s, uint32_t seed, const float* limits); /* :394-402, limits 2 x DIM */
+int bh_stretch_init_position_from(bh_stretch* s, const bh_stretch* position);    /* :388-393 */
+int bh_stretch_burn_in(bh_stretch* s, uint64_t n, float a);                      /* :403-411 */
+/* anneal!: temperatures[i] is the schedule's value at step i (:412-420) */
+int bh_stretch_anneal(bh_stretch* s, const double* temperatures, uint64_t n, float a);
+/* run-sampler! (:421-443): acceptance rate, the per-step ensemble means
+ * (dim x n, may be NULL) and their autocorrelation (tau, mean, sigma: dim
+ * floats each, lag).  BH_ERR_RANGE when n is too short for acor. */
+int bh_stretch_run(bh_stretch* s, uint64_t n, float a, double* acceptance_rate, float* means,
+                   float* tau, float* mean, float* sigma, uint32_t* lag);
+int bh_stretch_acc_rate(bh_stretch* s, float a, double* out);                    /* :444-453 */
+
+/* MCMCStretch (protocols.clj:105-109) */
+int bh_stretch_init_move(bh_stretch* s, float a, uint64_t max_steps);            /* :329-339 */
+int bh_stretch_move(bh_stretch* s);                                              /* :340-346 */
+int bh_stretch_move_bare(bh_stretch* s);                                         /* :347-353 */
+int bh_stretch_set_temperature(bh_stretch* s, double t);                         /* :354-358 */
+/* the accounting buffers of the last init-move: accept[K/WGS] and
+ * means_acc[DIM*(K/WGS)*steps] (either may be NULL) */
+int bh_stretch_read_accounting(bh_stretch* s, uint32_t* accept, float* means_acc,
+                               uint64_t steps);
+
+/* RandomSampler.sample! (:360-376): out is DIM x ncols. */
+int bh_stretch_sample(bh_stretch* s, float* out, uint64_t ncols);
+/* EstimateEngine.histogram! (:457-487) */
+int bh_stretch_histogram(bh_stretch* s, uint32_t cycles, float* limits, float* pdf,
+                         float* bin_ranks);
+/* Location.mean / Spread.variance of the current walkers (:489-505) */
+int bh_stretch_mean(bh_stretch* s, float* out);
+int bh_stretch_variance(bh_stretch* s, float* out);
+
+/* GPU time of the sampler's own stream: with timing on, burn-in!, anneal!,
+ * run-sampler!, sample!, move! and move-bare! record HIP events around their
+ * kernels, and bh_stretch_last_gpu_ms returns the elapsed milliseconds of
+ * the last such call (run-sampler!: the steps only).  No reference
+ * counterpart: the reference has no timers (amd_gcn.clj:321-324). */
+int bh_stretch_set_timing(bh_stretch* s, int on);
+int bh_stretch_last_gpu_ms(const bh_stretch* s, double* ms);
+
+/* Sampler state for resume: positions (DIM x W), logfn (W), and the host
+ * counters {bare seed, move seed, bare counter, move counter, iteration
+ * counter low, high} (amd_gcn.clj:273-274, 330-334, 385-386). */
+int bh_stretch_state_get(const bh_stretch* s, float* xs, float* logfn, uint32_t counters[6]);
+int bh_stretch_state_set(bh_stretch* s, const float* xs, const float* logfn,
+                         const uint32_t counters[6]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BAYADERA_HIP_H */

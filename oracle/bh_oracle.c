@@ -525,7 +525,9 @@ static void acor_1d_group(const float* x, uint32_t stride, uint32_t dim_id, uint
     for (uint32_t l = 0; l < lsize; ++l) {
         const uint32_t gid = start + l;
         const int compute = gid + lag < gsize;
-        const int load_lag = (l < lag) && has_next;
+        /* the reference reads past the series when the next group holds
+         * fewer than lag points; that read is defined here as 0 */
+        const int load_lag = (l < lag) && has_next && (gid + lsize < gsize);
         lm[l] = compute ? x[(size_t)gid * stride + dim_id] : 0.0f;
         lm[l + lsize] = load_lag ? x[(size_t)(gid + lsize) * stride + dim_id] : 0.0f;
     }
