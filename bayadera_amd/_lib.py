@@ -25,6 +25,11 @@ BH_PARTNER_WALKER = 1
 BH_EXEC_AUTO = 0
 BH_EXEC_STEPWISE = 1
 BH_EXEC_RESIDENT = 2
+BH_GLM_LOGISTIC = 1
+BH_GLM_FP64 = 0
+BH_GLM_FP32 = 1
+BH_GLM_BF16 = 2
+BH_GLM_BF16X3 = 3
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -60,6 +65,9 @@ SIGNATURES = {
     "bh_acor": [_P, _P, _U32, _U32, _P, _P, _P, _P],
     "bh_stretch_factory_create": [_P, _S, _S, _U32, _U32, _PP, _P, _SZ],
     "bh_stretch_factory_destroy": [_P],
+    "bh_glm_stretch_factory_create": [_P, _I, _P, _P, _U64, _U32, _F, _PP, _P, _SZ],
+    "bh_glm_log_density": [_P, _P, _U64, _I, _P],
+    "bh_stretch_set_glm_precision": [_P, _I],
     "bh_stretch_create": [_P, _U32, _U32, _P, _U32, _PP],
     "bh_stretch_destroy": [_P],
     "bh_stretch_set_partner": [_P, _I],

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/bayadera_hip.h"
+#include "bh_glm.hpp"
 #include "bh_kernels.hpp"
 
 // Sources handed to hipRTC, embedded at build time as raw string literals.
@@ -260,11 +261,24 @@ struct bh_lik_engine {
     hipFunction_t loglik = nullptr, lik = nullptr, evidence = nullptr;
 };
 
+// A GLM posterior's data, resident in HBM for the factory's lifetime.
+struct GlmData {
+    int kind = 0;
+    unsigned long long N = 0;
+    uint32_t d = 0;
+    double prior_prec = 1.0;  // 1 / prior_sd^2
+    DevBuf X;                 // bf16, N x d row-major
+    DevBuf lin;               // double[d]: X'y
+};
+
 struct bh_stretch_factory {
     bh_ctx* ctx = nullptr;
     uint32_t dim = 0, params_len = 0;
     Module mod;
     hipFunction_t logfn = nullptr, bare = nullptr, accu = nullptr, resident = nullptr;
+    std::unique_ptr<GlmData> glm;
+    hipFunction_t glm_propose = nullptr, glm_split = nullptr, glm_logpost = nullptr,
+                  glm_accept = nullptr;
 };
 
 struct bh_stretch {
@@ -283,6 +297,9 @@ struct bh_stretch {
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // GLM samplers: log posteriors in double, proposal and partial-sum buffers
+    int glm_prec = BH_GLM_BF16X3;
+    DevBuf lp, gY, gz, gu2, gsplit, gpartial;
 };
 
 namespace {
@@ -334,7 +351,7 @@ size_t resident_lds(const bh_stretch* s, int mode) {
 constexpr size_t kResidentLdsMax = 64 * 1024;
 
 bool use_resident(const bh_stretch* s, int mode) {
-    if (s->exec == BH_EXEC_STEPWISE) return false;
+    if (s->exec == BH_EXEC_STEPWISE || s->f->glm) return false;
     const bool fits = resident_lds(s, mode) <= kResidentLdsMax;
     if (s->exec == BH_EXEC_RESIDENT) return fits;
     // a single work-group pays off while the ensemble is small
@@ -359,9 +376,114 @@ int run_resident(bh_stretch* s, int mode, uint32_t seed_odd, uint32_t seed_even,
                   args);
 }
 
+// ---------------------------------------------------------------- GLM
+constexpr double kLn2 = 0.69314718055994530942;
+constexpr float kLog2e = 1.44269504088896340736f;
+
+int glm_nsplit(int prec) { return prec == BH_GLM_BF16X3 ? 3 : (prec == BH_GLM_BF16 ? 1 : 0); }
+
+// row ranges of one likelihood launch over n points: a multiple of 8 that
+// puts about four blocks on every compute unit
+unsigned glm_ranges(const bh_ctx* c, const GlmData& g, int prec, unsigned n) {
+    const unsigned groups = n / bh::glm_point_quantum(prec);
+    unsigned r = (4u * (unsigned)c->cu + groups - 1) / std::max(groups, 1u);
+    r = (r + 7) / 8 * 8;
+    const unsigned long long max_r = std::max(8ull, (g.N / 64 + 7) / 8 * 8);
+    if (r > max_r) r = (unsigned)max_r;
+    return std::max(r, 8u);
+}
+
+// log posteriors of n points P (device, n x d) into lp (double, device).
+int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
+    bh_stretch_factory* f = s->f;
+    const GlmData& g = *f->glm;
+    const int prec = s->glm_prec;
+    const unsigned nranges = glm_ranges(s->ctx, g, prec, n);
+    const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
+    int nsplit = glm_nsplit(prec);
+    float yscale = kLog2e;
+    __bf16* ys = s->gsplit.as<__bf16>();
+    if (nsplit) {
+        void* a[] = {&P, &n, &ys, &yscale, &nsplit};
+        BH_TRY(launch(f->glm_split, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a));
+    }
+    double* partial = s->gpartial.as<double>();
+    BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, P, ys, n, nranges, rpr,
+                                   yscale, partial));
+    const double* lin = g.lin.as<double>();
+    double scale = prec == BH_GLM_FP64 ? 1.0 : kLn2;
+    double pp = g.prior_prec;
+    unsigned nr = nranges;
+    const double* pc = partial;
+    void* a[] = {&P, &n, &pc, &nr, &lin, &scale, &pp, &lp};
+    return launch(f->glm_logpost, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a);
+}
+
+// One GLM half-step: proposals, the likelihood contraction, the accept test.
+int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned step, float a,
+             float beta, int accu, unsigned mstep) {
+    bh_stretch_factory* f = s->f;
+    const GlmData& g = *f->glm;
+    const int prec = s->glm_prec;
+    unsigned K = s->K;
+    float* X = s->xs.as<float>() + (first ? 0 : (size_t)K * s->dim);
+    const float* S = s->xs.as<float>() + (first ? (size_t)K * s->dim : 0);
+    double* lp = s->lp.as<double>() + (first ? 0 : K);
+    float* Y = s->gY.as<float>();
+    float* z = s->gz.as<float>();
+    float* u2 = s->gu2.as<float>();
+    __bf16* ys = s->gsplit.as<__bf16>();
+    int partner = s->partner;
+    int nsplit = glm_nsplit(prec);
+    float yscale = kLog2e;
+    const float* Xc = X;
+    void* pa[] = {&seed, &oe, &S, &Xc, &a, &step, &K, &partner, &Y, &z, &u2, &ys, &yscale, &nsplit};
+    BH_TRY(launch(f->glm_propose, K / s->wgs, s->wgs, 0, s->stream, pa));
+    unsigned nranges = glm_ranges(s->ctx, g, prec, K);
+    const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
+    double* partial = s->gpartial.as<double>();
+    BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, Y, ys, K, nranges, rpr,
+                                   yscale, partial));
+    const double* lin = g.lin.as<double>();
+    double scale = prec == BH_GLM_FP64 ? 1.0 : kLn2;
+    double pp = g.prior_prec;
+    const float* Yc = Y;
+    const float* zc = z;
+    const float* u2c = u2;
+    const double* pc = partial;
+    unsigned* acc = accu ? s->accept.as<unsigned>() : nullptr;
+    float* means = accu ? s->means_acc.as<float>() : nullptr;
+    void* aa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &X, &lp, &beta, &K, &accu,
+                  &acc, &means, &mstep};
+    return launch(f->glm_accept, K / s->wgs, s->wgs, 0, s->stream, aa);
+}
+
+// Work buffers of a GLM sampler (or of a log-density call) for up to n
+// points per launch.
+int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
+    const GlmData& g = *s->f->glm;
+    size_t maxr = 0;
+    for (int prec = BH_GLM_FP64; prec <= BH_GLM_BF16X3; ++prec)
+        for (unsigned n : {n_points, k_half})
+            if (n && n % bh::glm_point_quantum(prec) == 0)
+                maxr = std::max(maxr, (size_t)glm_ranges(s->ctx, g, prec, n) * n);
+    BH_TRY(s->gpartial.alloc(std::max<size_t>(maxr, 1) * sizeof(double)));
+    BH_TRY(s->gsplit.alloc((size_t)3 * n_points * g.d * sizeof(uint16_t)));
+    if (k_half) {
+        BH_TRY(s->gY.alloc((size_t)k_half * g.d * sizeof(float)));
+        BH_TRY(s->gz.alloc((size_t)k_half * sizeof(float)));
+        BH_TRY(s->gu2.alloc((size_t)k_half * sizeof(float)));
+    }
+    return BH_OK;
+}
+
 // One bare half-step pair (move-bare!, amd_gcn.clj:347-353): odd half with
 // seed and constant 3333, even half with seed+1 and 4444 (:547-550).
 int bare_pair(bh_stretch* s, float beta) {
+    if (s->f->glm) {
+        BH_TRY(glm_half(s, true, s->bare_seed, 3333, s->bare_counter, s->bare_a, beta, 0, 0));
+        return glm_half(s, false, s->bare_seed + 1u, 4444, s->bare_counter, s->bare_a, beta, 0, 0);
+    }
     unsigned data_len = s->data_len, params_len = s->params_len, K = s->K;
     const float* params = s->params.as<float>();
     float* x0 = s->xs.as<float>();
@@ -384,6 +506,12 @@ int bare_pair(bh_stretch* s, float beta) {
 
 // One accounting pair (move!, amd_gcn.clj:340-346): constants 1111/2222.
 int accu_pair(bh_stretch* s) {
+    if (s->f->glm) {
+        BH_TRY(glm_half(s, true, s->move_seed, 1111, s->move_counter, s->accu_a, 1.0f, 1,
+                        s->move_counter));
+        return glm_half(s, false, s->move_seed + 1u, 2222, s->move_counter, s->accu_a, 1.0f, 1,
+                        s->move_counter);
+    }
     unsigned data_len = s->data_len, params_len = s->params_len, K = s->K;
     const float* params = s->params.as<float>();
     float* x0 = s->xs.as<float>();
@@ -854,6 +982,81 @@ int bh_stretch_factory_destroy(bh_stretch_factory* f) {
     return BH_OK;
 }
 
+int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, const float* y,
+                                  uint64_t N, uint32_t d, float prior_sd,
+                                  bh_stretch_factory** out, char* log, size_t log_size) {
+    BH_ARG(ctx && X_bf16 && y && out, "NULL argument");
+    BH_ARG(kind == BH_GLM_LOGISTIC, "unknown GLM kind %d", kind);
+    BH_ARG(bh::glm_supported_dim(d), "GLM dimension %u not supported (32 or 64)", d);
+    BH_ARG(N > 0 && N < (1ull << 40), "bad row count");
+    BH_ARG(prior_sd > 0.0f, "prior sd must be positive");
+    BH_HIP(hipSetDevice(ctx->device));
+    std::vector<char> code;
+    BH_TRY(rtc_compile(ctx->arch, "",
+                       {"-DWGS=" + std::to_string(ctx->wgs), "-DDIM=" + std::to_string(d),
+                        "-DBH_GLM=1"},
+                       code, log, log_size));
+    auto f = std::make_unique<bh_stretch_factory>();
+    f->ctx = ctx;
+    f->dim = d;
+    f->params_len = 0;
+    BH_TRY(f->mod.load(code));
+    BH_TRY(f->mod.fn("bh_glm_propose", &f->glm_propose));
+    BH_TRY(f->mod.fn("bh_glm_split_points", &f->glm_split));
+    BH_TRY(f->mod.fn("bh_glm_logpost", &f->glm_logpost));
+    BH_TRY(f->mod.fn("bh_glm_accept", &f->glm_accept));
+    auto g = std::make_unique<GlmData>();
+    g->kind = kind;
+    g->N = N;
+    g->d = d;
+    g->prior_prec = 1.0 / ((double)prior_sd * (double)prior_sd);
+    BH_TRY(g->X.alloc((size_t)N * d * sizeof(uint16_t)));
+    BH_HIP(hipMemcpyAsync(g->X.p, X_bf16, (size_t)N * d * sizeof(uint16_t), hipMemcpyDefault,
+                          ctx->stream));
+    DevIn dy;
+    BH_TRY(dy.init(y, (size_t)N * sizeof(float), ctx->stream));
+    BH_TRY(g->lin.alloc((size_t)d * sizeof(double)));
+    const unsigned chunks = 1024;
+    DevBuf part;
+    BH_TRY(part.alloc((size_t)chunks * d * sizeof(double)));
+    BH_HIP(bh::launch_glm_lin(ctx->stream, g->X.p, dy.as<float>(), N, d, part.as<double>(),
+                              chunks, g->lin.as<double>()));
+    BH_TRY(sync(ctx->stream));
+    f->glm = std::move(g);
+    *out = f.release();
+    return BH_OK;
+}
+
+int bh_glm_log_density(bh_stretch_factory* f, const float* beta, uint64_t n, int precision,
+                       double* out) {
+    BH_ARG(f && beta && out, "NULL argument");
+    BH_ARG(f->glm, "not a GLM factory");
+    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    BH_ARG(n > 0 && n < (1ull << 26), "bad point count");
+    bh_ctx* c = f->ctx;
+    BH_HIP(hipSetDevice(c->device));
+    const unsigned q = bh::glm_point_quantum(precision);
+    const unsigned npad = (unsigned)((n + q - 1) / q * q);
+    bh_stretch t;  // work buffers and stream of this call
+    t.f = f;
+    t.ctx = c;
+    t.stream = c->stream;
+    t.wgs = c->wgs;
+    t.dim = f->dim;
+    t.glm_prec = precision;
+    BH_TRY(glm_alloc(&t, npad, 0));
+    DevBuf P, lp;
+    BH_TRY(P.alloc((size_t)npad * f->dim * sizeof(float)));
+    BH_HIP(hipMemsetAsync(P.p, 0, (size_t)npad * f->dim * sizeof(float), c->stream));
+    BH_HIP(hipMemcpyAsync(P.p, beta, (size_t)n * f->dim * sizeof(float), hipMemcpyDefault,
+                          c->stream));
+    BH_TRY(lp.alloc((size_t)npad * sizeof(double)));
+    BH_TRY(glm_logpost_points(&t, P.as<float>(), npad, lp.as<double>()));
+    BH_HIP(hipMemcpyAsync(out, lp.p, n * sizeof(double), hipMemcpyDefault, c->stream));
+    t.stream = nullptr;  // owned by the context
+    return sync(c->stream);
+}
+
 int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
                       const float* params, uint32_t params_count, bh_stretch** out) {
     BH_ARG(f && out, "NULL argument");
@@ -875,7 +1078,15 @@ int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
     s->params_count = params_count;
     s->params_len = f->params_len;
     s->data_len = params_count > f->params_len ? params_count - f->params_len : 0u;
+    if (f->glm && walkers % 1024 != 0)
+        return fail(BH_ERR_RANGE, "GLM samplers need a multiple of 1024 walkers, got %u",
+                    walkers);
     BH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if (f->glm) {
+        BH_TRY(glm_alloc(s.get(), walkers, walkers / 2));
+        BH_TRY(s->lp.alloc((size_t)walkers * sizeof(double)));
+        BH_HIP(hipMemsetAsync(s->lp.p, 0, (size_t)walkers * sizeof(double), s->stream));
+    }
     BH_TRY(s->xs.alloc((size_t)walkers * f->dim * sizeof(float)));
     BH_TRY(s->logfn.alloc((size_t)walkers * sizeof(float)));
     BH_TRY(s->params.alloc((size_t)std::max(params_count, 1u) * sizeof(float)));
@@ -937,6 +1148,14 @@ int bh_stretch_set_exec(bh_stretch* s, int mode) {
     return BH_OK;
 }
 
+int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
+    BH_ARG(s, "NULL sampler");
+    BH_ARG(s->f->glm, "not a GLM sampler");
+    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    s->glm_prec = precision;
+    return BH_OK;
+}
+
 int bh_stretch_info(const bh_stretch* s, uint32_t* walkers, uint64_t* iteration_counter) {
     BH_ARG(s, "NULL sampler");
     if (walkers) *walkers = s->W;
@@ -955,6 +1174,7 @@ int bh_stretch_init(bh_stretch* s, uint32_t seed) {
 }
 
 static int logfn_all(bh_stretch* s) {
+    if (s->f->glm) return glm_logpost_points(s, s->xs.as<float>(), s->W, s->lp.as<double>());
     unsigned data_len = s->data_len, params_len = s->params_len, n = s->W;
     const float* params = s->params.as<float>();
     const float* xs = s->xs.as<float>();
@@ -1210,9 +1430,19 @@ int bh_stretch_state_get(const bh_stretch* s, float* xs, float* logfn, uint32_t 
     if (xs)
         BH_HIP(hipMemcpyAsync(xs, s->xs.p, (size_t)s->W * s->dim * sizeof(float), hipMemcpyDefault,
                               s->stream));
-    if (logfn)
+    if (logfn && s->f->glm) {
+        std::vector<double> h(s->W);
+        BH_HIP(hipMemcpyAsync(h.data(), s->lp.p, (size_t)s->W * sizeof(double),
+                              hipMemcpyDeviceToHost, s->stream));
+        BH_TRY(sync(s->stream));
+        std::vector<float> hf(h.begin(), h.end());
+        BH_HIP(hipMemcpyAsync(logfn, hf.data(), (size_t)s->W * sizeof(float), hipMemcpyDefault,
+                              s->stream));
+        BH_TRY(sync(s->stream));
+    } else if (logfn) {
         BH_HIP(hipMemcpyAsync(logfn, s->logfn.p, (size_t)s->W * sizeof(float), hipMemcpyDefault,
                               s->stream));
+    }
     if (counters) {
         counters[0] = s->bare_seed;
         counters[1] = s->move_seed;
@@ -1231,11 +1461,15 @@ int bh_stretch_state_set(bh_stretch* s, const float* xs, const float* logfn,
     if (xs)
         BH_HIP(hipMemcpyAsync(s->xs.p, xs, (size_t)s->W * s->dim * sizeof(float), hipMemcpyDefault,
                               s->stream));
-    if (logfn)
+    if (s->f->glm) {
+        // log posteriors are kept in double: recompute them from positions
+        if (xs) BH_TRY(logfn_all(s));
+    } else if (logfn) {
         BH_HIP(hipMemcpyAsync(s->logfn.p, logfn, (size_t)s->W * sizeof(float), hipMemcpyDefault,
                               s->stream));
-    else if (xs)
+    } else if (xs) {
         BH_TRY(logfn_all(s));
+    }
     if (counters) {
         s->bare_seed = counters[0];
         s->move_seed = counters[1];

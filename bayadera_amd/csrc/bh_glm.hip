@@ -1,0 +1,300 @@
+// Logistic-regression log-likelihood of a batch of n parameter vectors,
+//   ll(b) = y.(X b) - sum_i softplus(x_i . b),
+// the dense part of which is the contraction U = X.B (N x n) over the d
+// features.  These kernels produce, for each row range r and point k,
+//   partial[r][k] = sum_{i in r} softplus(x_i . b_k)
+// (in log2 units for the fp32 and MFMA kernels, which contract against
+// log2(e)*b; in natural units for the fp64 kernel).  The linear term y.(X b)
+// = (X'y).b and the prior are added per point by the walker-side kernels.
+//
+// X is bf16, row-major N x D (the data as stored).  U is never materialised:
+// each tile is reduced through the softplus epilogue as it comes out of the
+// matrix cores.
+#include <hip/hip_runtime.h>
+
+#include "bh_glm.hpp"
+
+namespace bh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// log2(1 + 2^u) = max(u, 0) + log2(1 + 2^-|u|): one v_exp_f32 and one
+// v_log_f32, both on arguments where they are accurate to about an ulp.
+__device__ __forceinline__ float softplus2(float u) {
+    const float e = __builtin_amdgcn_exp2f(-__builtin_fabsf(u));
+    return __builtin_fmaxf(u, 0.0f) + __builtin_amdgcn_logf(1.0f + e);
+}
+
+// Block index -> (walker group, row range), XCD-aware: blocks are dealt to
+// the 8 XCDs round-robin, so the consecutive blocks one XCD receives take all
+// walker groups of one row range in turn and read the same rows of X through
+// that XCD's L2.
+__device__ __forceinline__ void glm_tile(unsigned groups, unsigned& group, unsigned& range) {
+    const unsigned b = blockIdx.x;
+    const unsigned idx = b >> 3;
+    group = idx % groups;
+    range = (idx / groups) * 8 + (b & 7);
+}
+
+// MFMA kernel: v_mfma_f32_16x16x32_bf16.  A block of 8 waves owns 512
+// points, 64 per wave as four 16-point column blocks whose B fragments (all
+// D features, NS split terms) stay in registers for the whole row range.
+// Lane layout (16x16x32): A lane l holds row l%16, features (l/16)*8..+7 of a
+// 32-feature chunk; B lane l holds point l%16, the same features; the
+// accumulator lane l holds rows (l/16)*4..+3 of point l%16.
+template <int D, int NS>
+__global__ void __launch_bounds__(512)
+glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned long long N,
+                         const __bf16* __restrict__ ys, unsigned n, unsigned groups,
+                         unsigned nranges, unsigned long long rows_per_range,
+                         double* __restrict__ partial) {
+    constexpr int KC = D / 32;
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned wave = threadIdx.x >> 6;
+    unsigned group, range;
+    glm_tile(groups, group, range);
+    if (range >= nranges) return;
+    const unsigned w0 = group * 512u + wave * 64u;
+    const unsigned lr = lane & 15u, lg = lane >> 4;
+
+    bf16x8 bfrag[4][KC][NS];
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                bfrag[wb][kc][s] = *reinterpret_cast<const bf16x8*>(
+                    ys + ((size_t)s * n + w0 + wb * 16 + lr) * D + kc * 32 + lg * 8);
+
+    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
+    unsigned long long r1 = r0 + rows_per_range;
+    if (r1 > N) r1 = N;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+
+    bf16x8 anext[KC];
+    auto load_a = [&](unsigned long long rb, bf16x8 (&a)[KC]) {
+        const unsigned long long row = rb + lr;
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            if (row < r1) {
+                a[kc] = *reinterpret_cast<const bf16x8*>(X + row * D + kc * 32 + lg * 8);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[kc][j] = (__bf16)0.0f;
+            }
+        }
+    };
+    if (r0 < r1) load_a(r0, anext);
+    for (unsigned long long rb = r0; rb < r1; rb += 16) {
+        bf16x8 a[KC];
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) a[kc] = anext[kc];
+        if (rb + 16 < r1) load_a(rb + 16, anext);
+        const unsigned long long orow = rb + lg * 4;
+#pragma unroll
+        for (int wb = 0; wb < 4; ++wb) {
+            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                for (int s = NS - 1; s >= 0; --s)
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], c, 0, 0, 0);
+            float s4 = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (orow + i < r1) s4 += softplus2(c[i]);
+            acc[wb] += (double)s4;
+        }
+    }
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb) {
+        double v = acc[wb];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (lg == 0) partial[(size_t)range * n + w0 + wb * 16 + lr] = v;
+    }
+}
+
+// Plain fp32 kernel (no matrix cores): one point per thread, its D
+// coefficients (times log2 e) in registers; tiles of 32 rows of X are
+// converted to fp32 in LDS and read back as broadcasts.
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_logistic_fp32_kernel(const __bf16* __restrict__ X, unsigned long long N,
+                         const float* __restrict__ P, unsigned n, unsigned groups,
+                         unsigned nranges, unsigned long long rows_per_range, float yscale,
+                         double* __restrict__ partial) {
+    constexpr int TR = 32;
+    __shared__ float4 xs[TR * D / 4];
+    unsigned group, range;
+    glm_tile(groups, group, range);
+    if (range >= nranges) return;
+    const unsigned k = group * 256u + threadIdx.x;
+    float y[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) y[j] = P[(size_t)k * D + j] * yscale;
+    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
+    unsigned long long r1 = r0 + rows_per_range;
+    if (r1 > N) r1 = N;
+    double acc = 0.0;
+    for (unsigned long long rb = r0; rb < r1; rb += TR) {
+        __syncthreads();
+        for (unsigned e = threadIdx.x; e < TR * D; e += 256) {
+            const unsigned long long row = rb + e / D;
+            reinterpret_cast<float*>(xs)[e] = row < r1 ? (float)X[row * D + e % D] : 0.0f;
+        }
+        __syncthreads();
+        const int rows = (int)((r1 - rb) < TR ? (r1 - rb) : TR);
+        float s4 = 0.0f;
+        for (int r = 0; r < rows; ++r) {
+            float u = 0.0f;
+#pragma unroll
+            for (int j4 = 0; j4 < D / 4; ++j4) {
+                const float4 x = xs[r * (D / 4) + j4];
+                u = __builtin_fmaf(x.x, y[4 * j4 + 0], u);
+                u = __builtin_fmaf(x.y, y[4 * j4 + 1], u);
+                u = __builtin_fmaf(x.z, y[4 * j4 + 2], u);
+                u = __builtin_fmaf(x.w, y[4 * j4 + 3], u);
+            }
+            s4 += softplus2(u);
+            if ((r & 3) == 3) {
+                acc += (double)s4;
+                s4 = 0.0f;
+            }
+        }
+        acc += (double)s4;
+    }
+    partial[(size_t)range * n + k] = acc;
+}
+
+// fp64 reference kernel: the products of bf16 data and fp32 coefficients
+// are exact in double, the sum and softplus(eta) = max(eta,0) +
+// log1p(exp(-|eta|)) run in double.  Natural-log units.
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_logistic_fp64_kernel(const __bf16* __restrict__ X, unsigned long long N,
+                         const float* __restrict__ P, unsigned n, unsigned groups,
+                         unsigned nranges, unsigned long long rows_per_range,
+                         double* __restrict__ partial) {
+    constexpr int TR = 16;
+    __shared__ double xs[TR * D];
+    unsigned group, range;
+    glm_tile(groups, group, range);
+    if (range >= nranges) return;
+    const unsigned k = group * 256u + threadIdx.x;
+    double y[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) y[j] = (double)P[(size_t)k * D + j];
+    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
+    unsigned long long r1 = r0 + rows_per_range;
+    if (r1 > N) r1 = N;
+    double acc = 0.0;
+    for (unsigned long long rb = r0; rb < r1; rb += TR) {
+        __syncthreads();
+        for (unsigned e = threadIdx.x; e < TR * D; e += 256) {
+            const unsigned long long row = rb + e / D;
+            xs[e] = row < r1 ? (double)(float)X[row * D + e % D] : 0.0;
+        }
+        __syncthreads();
+        const int rows = (int)((r1 - rb) < TR ? (r1 - rb) : TR);
+        for (int r = 0; r < rows; ++r) {
+            double eta = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) eta = __builtin_fma(xs[r * D + j], y[j], eta);
+            acc += __builtin_fmax(eta, 0.0) + log1p(exp(-__builtin_fabs(eta)));
+        }
+    }
+    partial[(size_t)range * n + k] = acc;
+}
+
+// lin[j] = sum_i y_i x_ij in double: one block per chunk of rows, one
+// thread per feature; chunk partials summed in chunk order by lin_final.
+__global__ void __launch_bounds__(256)
+glm_lin_partial_kernel(const __bf16* __restrict__ X, const float* __restrict__ yv,
+                       unsigned long long N, unsigned d, unsigned long long rows_per_chunk,
+                       double* __restrict__ part) {
+    const unsigned j = threadIdx.x;
+    if (j >= d) return;
+    const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_chunk;
+    unsigned long long r1 = r0 + rows_per_chunk;
+    if (r1 > N) r1 = N;
+    double s = 0.0;
+    for (unsigned long long i = r0; i < r1; ++i) s = __builtin_fma((double)yv[i], (double)(float)X[i * d + j], s);
+    part[(size_t)blockIdx.x * d + j] = s;
+}
+
+__global__ void __launch_bounds__(256)
+glm_lin_final_kernel(const double* __restrict__ part, unsigned chunks, unsigned d,
+                     double* __restrict__ lin) {
+    const unsigned j = threadIdx.x;
+    if (j >= d) return;
+    double s = 0.0;
+    for (unsigned c = 0; c < chunks; ++c) s += part[(size_t)c * d + j];
+    lin[j] = s;
+}
+
+// --------------------------------------------------------------------------
+template <int D>
+static hipError_t launch_d(int precision, hipStream_t s, const __bf16* X, unsigned long long N,
+                           const float* P, const __bf16* ys, unsigned n, unsigned nranges,
+                           unsigned long long rpr, float yscale, double* partial) {
+    switch (precision) {
+    case kGlmBf16: {
+        const unsigned groups = n / 512;
+        glm_logistic_mfma_kernel<D, 1><<<groups * nranges, 512, 0, s>>>(X, N, ys, n, groups,
+                                                                        nranges, rpr, partial);
+        break;
+    }
+    case kGlmBf16x3: {
+        const unsigned groups = n / 512;
+        glm_logistic_mfma_kernel<D, 3><<<groups * nranges, 512, 0, s>>>(X, N, ys, n, groups,
+                                                                        nranges, rpr, partial);
+        break;
+    }
+    case kGlmFp32: {
+        const unsigned groups = n / 256;
+        glm_logistic_fp32_kernel<D><<<groups * nranges, 256, 0, s>>>(X, N, P, n, groups, nranges,
+                                                                     rpr, yscale, partial);
+        break;
+    }
+    case kGlmFp64: {
+        const unsigned groups = n / 256;
+        glm_logistic_fp64_kernel<D><<<groups * nranges, 256, 0, s>>>(X, N, P, n, groups, nranges,
+                                                                     rpr, partial);
+        break;
+    }
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+bool glm_supported_dim(unsigned d) { return d == 32 || d == 64; }
+
+unsigned glm_point_quantum(int precision) {
+    return (precision == kGlmBf16 || precision == kGlmBf16x3) ? 512u : 256u;
+}
+
+hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsigned long long N,
+                               unsigned d, const float* P, const void* ys, unsigned n,
+                               unsigned nranges, unsigned long long rows_per_range, float yscale,
+                               double* partial) {
+    const __bf16* x = static_cast<const __bf16*>(X);
+    const __bf16* y = static_cast<const __bf16*>(ys);
+    if (d == 64) return launch_d<64>(precision, s, x, N, P, y, n, nranges, rows_per_range, yscale, partial);
+    if (d == 32) return launch_d<32>(precision, s, x, N, P, y, n, nranges, rows_per_range, yscale, partial);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_glm_lin(hipStream_t s, const void* X, const float* yv, unsigned long long N,
+                          unsigned d, double* part, unsigned chunks, double* lin) {
+    const unsigned long long rpc = (N + chunks - 1) / chunks;
+    glm_lin_partial_kernel<<<chunks, 256, 0, s>>>(static_cast<const __bf16*>(X), yv, N, d, rpc,
+                                                  part);
+    glm_lin_final_kernel<<<1, 256, 0, s>>>(part, chunks, d, lin);
+    return hipGetLastError();
+}
+
+}  // namespace bh

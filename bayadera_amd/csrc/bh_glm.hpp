@@ -1,0 +1,26 @@
+// Launchers of the GLM likelihood kernels (bh_glm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace bh {
+
+// precision of the contraction X.B (values of BH_GLM_* in bayadera_hip.h)
+constexpr int kGlmFp64 = 0;
+constexpr int kGlmFp32 = 1;
+constexpr int kGlmBf16 = 2;
+constexpr int kGlmBf16x3 = 3;
+
+bool glm_supported_dim(unsigned d);
+// the number of points n of a launch must be a multiple of this
+unsigned glm_point_quantum(int precision);
+// partial[r][k] for r < nranges (a multiple of 8), k < n; X bf16 N x d,
+// P fp32 n x d (fp32/fp64 kernels), ys the bf16 split of yscale*P
+// (NS x n x d, MFMA kernels).
+hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsigned long long N,
+                               unsigned d, const float* P, const void* ys, unsigned n,
+                               unsigned nranges, unsigned long long rows_per_range, float yscale,
+                               double* partial);
+hipError_t launch_glm_lin(hipStream_t s, const void* X, const float* yv, unsigned long long N,
+                          unsigned d, double* part, unsigned chunks, double* lin);
+
+}  // namespace bh

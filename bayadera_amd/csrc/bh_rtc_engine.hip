@@ -71,30 +71,26 @@ __device__ __forceinline__ unsigned bh_slice_count(bool flag, unsigned* wave_cou
     return total;
 }
 
-#ifdef LOGFN
+#if defined(LOGFN) || defined(BH_GLM)
 
-// One walker's stretch move (amd-gcn-mcmc-stretch.cl:31-81).
+// The proposal half of the stretch move (amd-gcn-mcmc-stretch.cl:46-65):
 //   counter {k, step, odd_or_even, 0xbeeff00d}, key {seed, 0xdecafbad}
 //   z  = fma((a-2+1/a)*u1, u1, (2(1-1/a))*u1) + 1/a                     (:53-54)
 //   Y  = fma(z, Xk - Xj, Xj)                                           (:64)
-//   q  = isfinite(ly) ? z^(DIM-1) exp(beta (ly - lX)) : 0; accept iff u2 <= q (:68-71)
 // Partner: FLAT reads the complementary half at element offset
 // j0 = (uint)((u0*K)*DIM) (:57), wrapping modulo K*DIM where the reference
 // reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1).
-// xk holds the walker's coordinates on entry and its new ones on exit.
-__device__ __forceinline__ bool bh_stretch_walker(unsigned k, unsigned K, unsigned seed,
-                                                  unsigned odd_or_even, unsigned step, float a,
-                                                  float beta, int partner, const float* S,
-                                                  float (&xk)[DIM], float& lx,
-                                                  unsigned data_len, unsigned params_len,
-                                                  const float* params) {
+// Returns u2, the uniform of the accept test.
+__device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned seed,
+                                           unsigned odd_or_even, unsigned step, float a,
+                                           int partner, const float* S, const float (&xk)[DIM],
+                                           float (&y)[DIM], float& z) {
     const bh_f32x4 u = bh_u01x4(
         bh_philox4x32_10(bh_u32x4{k, step, odd_or_even, BH_CTR3}, seed, BH_KEY1_STRETCH));
     const float inv_a = 1.0f / a;
     const float ca = a - 2.0f + inv_a;
     const float cb = 2.0f * (1.0f - inv_a);
-    const float z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
-    float y[DIM];
+    z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
     if (partner == BH_PARTNER_FLAT) {
         const unsigned total = K * DIM;
         unsigned e = (unsigned)((u.x * (float)K) * (float)DIM);
@@ -115,26 +111,15 @@ __device__ __forceinline__ bool bh_stretch_walker(unsigned k, unsigned K, unsign
             y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
         }
     }
-    const float ly = LOGFN(data_len, params_len, params, DIM, y);
+    return u.z;
+}
+
+// z^(DIM-1) by repeated multiplication
+__device__ __forceinline__ float bh_zpow(float z) {
     float zp = 1.0f;
 #pragma unroll
     for (int i = 0; i < DIM - 1; ++i) zp *= z;
-    const float q = __builtin_isfinite(ly) ? zp * expf(beta * (ly - lx)) : 0.0f;
-    const bool accepted = u.z <= q;
-    if (accepted) {
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) xk[i] = y[i];
-        lx = ly;
-    }
-    return accepted;
-}
-
-// logfn kernel (amd-gcn-mcmc-stretch.cl:161-167): res[w] = LOGFN(x + DIM*w).
-extern "C" __global__ void __launch_bounds__(WGS)
-bh_logfn(unsigned data_len, unsigned params_len, const float* __restrict__ params,
-         const float* __restrict__ x, float* __restrict__ res, unsigned n) {
-    const unsigned w = blockIdx.x * WGS + threadIdx.x;
-    if (w < n) res[w] = LOGFN(data_len, params_len, params, DIM, x + (size_t)w * DIM);
+    return zp;
 }
 
 // Accounting of stretch_move_accu (amd-gcn-mcmc-stretch.cl:98-114) for the
@@ -163,6 +148,47 @@ __device__ __forceinline__ void bh_accu(const float (&xk)[DIM], bool accepted, u
                 if (d0 + c < DIM) m[(size_t)(d0 + c) * groups] += sums[c];
         }
     }
+}
+
+#endif  // LOGFN || BH_GLM
+
+#ifdef LOGFN
+
+// One walker's stretch move (amd-gcn-mcmc-stretch.cl:31-81).
+//   counter {k, step, odd_or_even, 0xbeeff00d}, key {seed, 0xdecafbad}
+//   z  = fma((a-2+1/a)*u1, u1, (2(1-1/a))*u1) + 1/a                     (:53-54)
+//   Y  = fma(z, Xk - Xj, Xj)                                           (:64)
+//   q  = isfinite(ly) ? z^(DIM-1) exp(beta (ly - lX)) : 0; accept iff u2 <= q (:68-71)
+// Partner: FLAT reads the complementary half at element offset
+// j0 = (uint)((u0*K)*DIM) (:57), wrapping modulo K*DIM where the reference
+// reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1).
+// xk holds the walker's coordinates on entry and its new ones on exit.
+__device__ __forceinline__ bool bh_stretch_walker(unsigned k, unsigned K, unsigned seed,
+                                                  unsigned odd_or_even, unsigned step, float a,
+                                                  float beta, int partner, const float* S,
+                                                  float (&xk)[DIM], float& lx,
+                                                  unsigned data_len, unsigned params_len,
+                                                  const float* params) {
+    float y[DIM];
+    float z;
+    const float u2 = bh_propose(k, K, seed, odd_or_even, step, a, partner, S, xk, y, z);
+    const float ly = LOGFN(data_len, params_len, params, DIM, y);
+    const float q = __builtin_isfinite(ly) ? bh_zpow(z) * expf(beta * (ly - lx)) : 0.0f;
+    const bool accepted = u2 <= q;
+    if (accepted) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) xk[i] = y[i];
+        lx = ly;
+    }
+    return accepted;
+}
+
+// logfn kernel (amd-gcn-mcmc-stretch.cl:161-167): res[w] = LOGFN(x + DIM*w).
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_logfn(unsigned data_len, unsigned params_len, const float* __restrict__ params,
+         const float* __restrict__ x, float* __restrict__ res, unsigned n) {
+    const unsigned w = blockIdx.x * WGS + threadIdx.x;
+    if (w < n) res[w] = LOGFN(data_len, params_len, params, DIM, x + (size_t)w * DIM);
 }
 
 // One half-step over K walkers, one thread per walker, WGS walkers per block
@@ -286,6 +312,127 @@ bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsig
 }
 
 #endif  // LOGFN
+
+#ifdef BH_GLM
+// Generalised linear model posteriors evaluated for a whole batch of walkers
+// at once (the likelihood is a dense contraction X.Y over data rows, run by
+// the ahead-of-time kernels of bh_glm.hip).  These kernels are the
+// walker-side halves: the proposal, and the accept test with its accounting,
+// with the same arithmetic as bh_stretch_walker.
+
+// Writes yscale*Y as nsplit bf16 terms, hi + mid + lo, each the nearest
+// bf16 of what the previous terms leave (exact residuals in fp32).
+__device__ __forceinline__ void bh_glm_split(const float (&y)[DIM], float yscale, int nsplit,
+                                             __bf16* out, unsigned k, unsigned n) {
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        float r = y[i] * yscale;
+        for (int sp = 0; sp < nsplit; ++sp) {
+            const __bf16 h = (__bf16)r;
+            out[((size_t)sp * n + k) * DIM + i] = h;
+            r = r - (float)h;
+        }
+    }
+}
+
+// Proposals of one half-step: Y (fp32), z and u2 per walker, and the bf16
+// split of log2(e)*Y for the MFMA kernels.
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
+               const float* __restrict__ X, float a, unsigned step, unsigned K, int partner,
+               float* __restrict__ Y, float* __restrict__ zbuf, float* __restrict__ u2buf,
+               __bf16* __restrict__ ysplit, float yscale, int nsplit) {
+    const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    float xk[DIM], y[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
+    float z;
+    const float u2 = bh_propose(k, K, seed, odd_or_even, step, a, partner, S, xk, y, z);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) Y[(size_t)k * DIM + i] = y[i];
+    zbuf[k] = z;
+    u2buf[k] = u2;
+    if (nsplit > 0) bh_glm_split(y, yscale, nsplit, ysplit, k, K);
+}
+
+// bf16 split of n given points (for the initial log-densities).
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_split_points(const float* __restrict__ P, unsigned n, __bf16* __restrict__ ysplit,
+                    float yscale, int nsplit) {
+    const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    if (k >= n) return;
+    float y[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) y[i] = P[(size_t)k * DIM + i];
+    bh_glm_split(y, yscale, nsplit, ysplit, k, n);
+}
+
+// log posterior of point k from the per-range row sums of the likelihood
+// kernel: lin.Y - scale * sum_r partial[r][k] - |Y|^2 / (2 s^2), in double
+// (the sum over ranges runs in range order).
+__device__ __forceinline__ double bh_glm_logpost(const float (&y)[DIM], unsigned k, unsigned n,
+                                                 const double* __restrict__ partial,
+                                                 unsigned nranges, const double* __restrict__ lin,
+                                                 double scale, double prior_prec) {
+    double s = 0.0;
+    for (unsigned r = 0; r < nranges; ++r) s += partial[(size_t)r * n + k];
+    double l = 0.0, q = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        l = __builtin_fma(lin[i], (double)y[i], l);
+        q = __builtin_fma((double)y[i], (double)y[i], q);
+    }
+    return l - scale * s - 0.5 * prior_prec * q;
+}
+
+// Initial log posteriors of n points.
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_logpost(const float* __restrict__ P, unsigned n, const double* __restrict__ partial,
+               unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
+               double* __restrict__ lp) {
+    const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    if (k >= n) return;
+    float y[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) y[i] = P[(size_t)k * DIM + i];
+    lp[k] = bh_glm_logpost(y, k, n, partial, nranges, lin, scale, prior_prec);
+}
+
+// Accept test of one half-step (amd-gcn-mcmc-stretch.cl:66-78): q =
+// z^(DIM-1) exp(beta (lp(Y) - lp(X))), the difference taken in double; with
+// accu != 0 also the accounting of stretch_move_accu.
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_accept(const float* __restrict__ Y, const float* __restrict__ zbuf,
+              const float* __restrict__ u2buf, const double* __restrict__ partial,
+              unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
+              float* __restrict__ X, double* __restrict__ lp, float beta, unsigned K, int accu,
+              unsigned* __restrict__ accept, float* __restrict__ means, unsigned mstep) {
+    const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    float y[DIM], xk[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
+    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
+    const float d = (float)(lpy - lp[k]);
+    const float q = __builtin_isfinite(lpy) ? bh_zpow(zbuf[k]) * expf(beta * d) : 0.0f;
+    const bool accepted = u2buf[k] <= q;
+    if (accepted) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            X[(size_t)k * DIM + i] = y[i];
+            xk[i] = y[i];
+        }
+        lp[k] = lpy;
+    } else if (accu) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
+    }
+    if (accu) {
+        __shared__ float red[BH_ACCU_CHUNK * WGS];
+        __shared__ unsigned wave_counts[WGS / 64];
+        bh_accu(xk, accepted, blockIdx.x, gridDim.x, mstep, accept, means, red, wave_counts);
+    }
+}
+#endif  // BH_GLM
 
 #ifdef LOGPDF
 // logpdf / pdf kernels (amd-gcn-distribution.cl:1-14).

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call
-from .models import DistributionModel, LikelihoodModel
+from .models import DistributionModel, GlmModel, LikelihoodModel
 
 WGS_DEFAULT = 256
 
@@ -139,7 +139,9 @@ class HipBayaderaFactory(_Handle):
     def direct_sampler_engine(self, model: DistributionModel) -> "HipDirectSamplerEngine":
         return HipDirectSamplerEngine(self, model)
 
-    def mcmc_factory(self, model: DistributionModel) -> "HipStretchFactory":
+    def mcmc_factory(self, model) -> "HipStretchFactory":
+        if isinstance(model, GlmModel):
+            return HipGlmStretchFactory(self, model)
         return HipStretchFactory(self, model)
 
     def processing_elements(self) -> int:
@@ -307,6 +309,30 @@ class HipStretchFactory(_Handle):
         return HipStretch(self, seed, walkers, params)
 
 
+class HipGlmStretchFactory(HipStretchFactory):
+    """A stretch factory for a GLM posterior declared by form (GlmModel): the
+    likelihood of a whole half-ensemble runs as one X.B contraction."""
+
+    def __init__(self, factory: HipBayaderaFactory, model: GlmModel):
+        _Handle.__init__(self)
+        self.factory, self.dist_model = factory, model
+        kind = {"logistic": _lib.BH_GLM_LOGISTIC}[model.kind]
+        X = _Arr(model.X, np.uint16) if not hasattr(model.X, "data_ptr") else _Arr(model.X)
+        y = _Arr(model.y)
+        n, d = model.X.shape
+        log = _build_log()
+        call("bh_glm_stretch_factory_create", factory.handle, kind, X.ptr, y.ptr, int(n), int(d),
+             float(model.prior_sd), ctypes.byref(self._h), log, len(log))
+
+    def log_density(self, beta, precision: int = _lib.BH_GLM_FP64) -> np.ndarray:
+        """Log posterior (likelihood + prior) of the rows of beta (n, d)."""
+        b = _Arr(beta)
+        n = b.size // self.dist_model.dimension
+        out = np.zeros(n, np.float64)
+        call("bh_glm_log_density", self._h, b.ptr, n, int(precision), out.ctypes.data)
+        return out
+
+
 class HipStretch(_Handle):
     """GCNStretch (amd_gcn.clj:271-507): MCMC, MCMCStretch, RandomSampler,
     EstimateEngine, Location, Spread."""
@@ -333,6 +359,10 @@ class HipStretch(_Handle):
 
     def set_exec(self, mode: int):
         call("bh_stretch_set_exec", self._h, int(mode))
+        return self
+
+    def set_glm_precision(self, precision: int):
+        call("bh_stretch_set_glm_precision", self._h, int(precision))
         return self
 
     def set_timing(self, on: bool = True):

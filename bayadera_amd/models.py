@@ -503,6 +503,52 @@ def likelihood_models() -> dict:
             "binomial": likelihood_model(BINOMIAL_SOURCE, name="binomial")}
 
 
+@dataclass
+class GlmModel:
+    """A GLM posterior declared by form, not by source: the engine evaluates
+    its likelihood for a whole half-ensemble as one contraction X.B on the
+    matrix cores (bh_glm_stretch_factory_create).  ``X`` holds bf16 bit
+    patterns (uint16, N x d row-major), ``y`` labels in {0, 1}; the prior is
+    beta ~ N(0, prior_sd^2 I)."""
+    kind: str
+    X: np.ndarray
+    y: np.ndarray
+    prior_sd: float
+    name: str = "glm"
+
+    @property
+    def dimension(self) -> int:
+        return int(self.X.shape[1])
+
+    @property
+    def params_size(self) -> int:
+        return 0
+
+    def model(self):
+        return self
+
+
+def glm_logistic_model(X_bf16, y, prior_sd: float = 5.0) -> GlmModel:
+    X = np.ascontiguousarray(X_bf16)
+    if X.dtype != np.uint16 or X.ndim != 2:
+        raise ValueError("X must be an (N, d) array of bf16 bit patterns (uint16)")
+    yv = np.ascontiguousarray(y, np.float32).reshape(-1)
+    if yv.size != X.shape[0]:
+        raise ValueError("y must have one label per row of X")
+    return GlmModel("logistic", X, yv, float(prior_sd))
+
+
+def to_bf16(a) -> np.ndarray:
+    """float32 -> bf16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32)
+    r = ((u >> 16) & 1) + 0x7FFF
+    return ((u + r) >> 16).astype(np.uint16)
+
+
+def from_bf16(b) -> np.ndarray:
+    return (np.ascontiguousarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
 def mvn_model(dim: int) -> DistributionModel:
     return distribution_model(MVN_SOURCE, name="mvn", dimension=dim, params_size=dim + dim * dim,
                               limits=[[-FLT_MAX, FLT_MAX]] * dim)

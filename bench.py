@@ -144,11 +144,176 @@ def cpu_baseline(params, limits, budget_s=25.0):
                                         / t, 1)}
 
 
+# ---------------------------------------------------------------------------
+# Logistic regression, precision form: X 1,000,000 x 64 (rows N(0,1)/8,
+# stored bf16), beta* ~ N(0,1), y ~ Bernoulli(sigmoid(X beta*)), prior
+# N(0, 5^2), 16384 walkers.  The likelihood of each half-ensemble is one
+# 1M x 64 . 64 x 8192 contraction with a softplus epilogue.
+# ---------------------------------------------------------------------------
+C3_N, C3_D, C3_W = 1_000_000, 64, 16384
+C3_BURN, C3_RUN = 500, 2000
+C3_PRIOR_SD = 5.0
+MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
+
+
+def c3_problem(N=C3_N, d=C3_D, seed=1003):
+    from bayadera_amd import models
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((N, d), dtype=np.float32) / np.float32(np.sqrt(d)))
+    Xb = models.to_bf16(X)
+    Xd = models.from_bf16(Xb).reshape(N, d).astype(np.float64)
+    beta_true = rng.standard_normal(d)
+    eta = Xd @ beta_true
+    y = (rng.random(N) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float32)
+    # a Laplace box for init-position!: Newton steps to the posterior mode,
+    # walkers uniform within 4 sd of it (setup, not timed)
+    b = np.zeros(d)
+    for _ in range(8):
+        p = 1.0 / (1.0 + np.exp(-(Xd @ b)))
+        g = Xd.T @ (y - p) - b / C3_PRIOR_SD ** 2
+        H = (Xd * (p * (1 - p))[:, None]).T @ Xd + np.eye(d) / C3_PRIOR_SD ** 2
+        b = b + np.linalg.solve(H, g)
+    sd = np.sqrt(np.diag(np.linalg.inv(H)))
+    limits = np.stack([b - 4 * sd, b + 4 * sd], 1).astype(np.float32).reshape(-1)
+    return Xb, Xd, y, b, sd, limits
+
+
+def c3_run(sf, prec, limits, burn, run, seed=123):
+    from bayadera_amd import _lib
+    s = sf.create_sampler(seed, C3_W, np.zeros(1, np.float32))
+    s.set_glm_precision(prec).set_partner(_lib.BH_PARTNER_WALKER)
+    s.init(seed)
+    s.init_position(seed + 1, limits)
+    t0 = time.perf_counter()
+    s.burn_in(burn, A)
+    s.set_timing(True)
+    res = s.run_sampler(run, A)
+    run_gpu_ms = s.last_gpu_ms()
+    s.set_timing(False)
+    var = s.variance()
+    t1 = time.perf_counter()
+    s.release()
+    ac = res["autocorrelation"]
+    ess = ess_from(var, ac.sigma)
+    return {"wall_s": t1 - t0, "run_gpu_ms": run_gpu_ms, "ess": ess, "means": res["means"],
+            "grand_mean": res["means"].astype(np.float64).mean(0), "sigma": ac.sigma,
+            "tau": ac.tau, "acc": res["acceptance-rate"]}
+
+
+def c3_kernel_roofline(sf, limits, prec):
+    """Average duration of the likelihood contraction for one half-ensemble
+    (8192 points), from the sampler's stream events over single move-bare!
+    calls minus the walker-side kernels measured the same way at N/16."""
+    from bayadera_amd import _lib
+    s = sf.create_sampler(321, C3_W, np.zeros(1, np.float32))
+    s.set_glm_precision(prec).set_partner(_lib.BH_PARTNER_WALKER)
+    s.init_position(322, limits)
+    s.burn_in(2, A)
+    s.set_timing(True)
+    ms = []
+    for _ in range(10):
+        s.move_bare()
+        ms.append(s.last_gpu_ms())
+    s.set_timing(False)
+    s.release()
+    return float(np.median(ms)) / 2.0  # per half-step (propose + contraction + accept)
+
+
+def c3_cpu_baseline(Xd, y, limits, budget_s=20.0):
+    """Walker-steps/s of the oracle's fp64 GLM sampler on a bounded sample:
+    walkers x one half-step over the full 1M x 64 data, OpenMP over walkers."""
+    import oracle as O
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS"):
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    Xf = np.ascontiguousarray(Xd, np.float32)
+    W = 1024  # the sampler's smallest ensemble; half of it moves per half-step
+    g = O.GlmStretch(Xf, y, C3_PRIOR_SD, W, 5, partner=O.PARTNER_WALKER, threads=threads)
+    # positions only (skip the initial log posteriors of all walkers)
+    g.xs[:] = O.init_walkers(6, limits.reshape(-1, 2), C3_D, W)
+    t0 = time.perf_counter()
+    n_half = 0
+    while n_half == 0 or (time.perf_counter() - t0 < budget_s and n_half < 4):
+        g._half(5, 3333, n_half, True, A, 1.0)
+        n_half += 1
+    t = time.perf_counter() - t0
+    return {"walker_steps_per_s": (W // 2) * n_half / 2 / t, "threads": threads,
+            "sample": f"{n_half} half-step(s) of {W // 2} walkers over the full "
+                      f"{C3_N}x{C3_D} data in {t:.1f} s"}
+
+
+def main_c3(args):
+    import torch
+    from bayadera_amd import _lib, engine, models
+    t_setup = time.perf_counter()
+    Xb, Xd, y, b_map, sd_map, limits = c3_problem()
+    t_setup = time.perf_counter() - t_setup
+    fac = engine.HipBayaderaFactory(0, 256)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=C3_PRIOR_SD))
+    burn, run = args.c3_burn, args.c3_run
+    # warm-up (compiles nothing further; touches every kernel once)
+    c3_run(sf, _lib.BH_GLM_BF16X3, limits, 2, 64)
+    torch.cuda.synchronize()
+    res = {}
+    for name, prec in (("bf16x3", _lib.BH_GLM_BF16X3), ("fp32", _lib.BH_GLM_FP32),
+                       ("bf16", _lib.BH_GLM_BF16)):
+        res[name] = c3_run(sf, prec, limits, burn, run)
+    ref = c3_run(sf, _lib.BH_GLM_FP64, limits, burn, run) if not args.no_fp64 else None
+    out = {}
+    for name, r in res.items():
+        ess = float(r["ess"].min())
+        o = {"ess_per_s": round(ess / r["wall_s"], 2), "ess_min": round(ess, 1),
+             "walker_steps_per_s": round(C3_W * (burn + run) / r["wall_s"], 1),
+             "wall_s": round(r["wall_s"], 3), "acceptance_rate": round(r["acc"], 4),
+             "tau_max": round(float(r["tau"].max()), 2),
+             "ms_per_step_run": round(r["run_gpu_ms"] / run, 3)}
+        if ref is not None:
+            err = np.abs(r["grand_mean"] - ref["grand_mean"]) / ref["sigma"].astype(np.float64)
+            o["mean_err_over_mcse_max"] = round(float(err.max()), 4)
+        out[name] = o
+    half_ms = c3_kernel_roofline(sf, limits, _lib.BH_GLM_BF16X3)
+    flops = 2.0 * C3_N * C3_D * (C3_W // 2)
+    achieved = flops / (half_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC, "value": out["bf16x3"]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
+        "steps": burn + run, "warmup": 64, "ms_per_step": round(res["bf16x3"]["wall_s"] /
+                                                                 (burn + run) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "data": "synthetic (seeded logistic-regression data, X stored bf16)",
+        "config": {"workload": f"logistic regression {C3_N}x{C3_D}, {C3_W} walkers, "
+                               f"{burn} burn-in + {run} run-sampler steps (a=2), prior N(0,5^2)",
+                   "likelihood": "MFMA split-bf16x3, fp32 accumulate, fp64 row sums",
+                   "partner": "walker"},
+        "mfma_vs_fp32_ess_per_s": round(out["bf16x3"]["ess_per_s"] / out["fp32"]["ess_per_s"], 2),
+        "precisions": out,
+        "roofline": {"kernel": "half-step (propose + glm_logistic_mfma<64,3> + accept), 8192 "
+                               "points", "bound": "mfma", "achieved": round(achieved, 1),
+                     "peak": MFMA_BF16_PEAK, "unit": "TFLOP/s",
+                     "frac": round(achieved / MFMA_BF16_PEAK, 4), "traffic": None,
+                     "avg_half_step_ms": round(half_ms, 4),
+                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": 3},
+        "setup_s": round(t_setup, 1),
+    }
+    if not args.no_cpu_baseline:
+        cb = c3_cpu_baseline(Xd, y, limits)
+        ess_per_ws = out["bf16x3"]["ess_min"] / (C3_W * (burn + run))
+        line["cpu_baseline"] = {
+            "value": round(cb["walker_steps_per_s"] * ess_per_ws, 4), "unit": "ESS/s",
+            "cores": cb["threads"], "kind": "port",
+            "sample": cb["sample"] + "; ESS per walker-step taken from the GPU run",
+            "walker_steps_per_s": round(cb["walker_steps_per_s"], 2)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c1", "c3"], default="c1")
+    ap.add_argument("--c3-burn", type=int, default=C3_BURN)
+    ap.add_argument("--c3-run", type=int, default=C3_RUN)
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     args = ap.parse_args()
@@ -156,6 +321,8 @@ def main():
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(sharding across GPUs is designed in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        return main_c3(args)
 
     import torch
 

@@ -47,6 +47,15 @@ extern "C" {
 #define BH_EXEC_STEPWISE 1  /* two kernels per step, as the reference */
 #define BH_EXEC_RESIDENT 2  /* one work-group runs many steps from LDS */
 
+/* GLM posteriors (a capability the reference does not have: its models
+ * loop over data rows serially inside each walker's LOGFN, e.g.
+ * distributions/gaussian.cl:29-42).  Precision of the contraction X.B: */
+#define BH_GLM_LOGISTIC 1   /* y_i ~ Bernoulli(sigmoid(x_i . beta)), beta ~ N(0, s^2 I) */
+#define BH_GLM_FP64 0       /* fp64 VALU: the accuracy reference */
+#define BH_GLM_FP32 1       /* plain fp32 VALU contraction and softplus */
+#define BH_GLM_BF16 2       /* MFMA, bf16 x bf16 -> fp32, one product */
+#define BH_GLM_BF16X3 3     /* MFMA, proposals split in three bf16 terms (fp32-accurate) */
+
 typedef struct bh_ctx bh_ctx;
 typedef struct bh_dist_engine bh_dist_engine;
 typedef struct bh_lik_engine bh_lik_engine;
@@ -146,6 +155,19 @@ int bh_stretch_factory_create(bh_ctx* ctx, const char* source, const char* logfn
                               char* log, size_t log_size);
 int bh_stretch_factory_destroy(bh_stretch_factory* f);
 
+/* A stretch-sampler factory whose LOGFN is a GLM posterior evaluated for a
+ * whole half-ensemble at once (no reference counterpart; it sits behind the
+ * same SamplerFactory.create-sampler, amd_gcn.clj:514-567).  X: N x d bf16
+ * (IEEE bfloat16 bit patterns, row-major), y: N labels in {0, 1}; d is 32
+ * or 64; the data stay in HBM until the factory is destroyed.  Samplers
+ * need a walker count that is a multiple of 1024. */
+int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, const float* y,
+                                  uint64_t N, uint32_t d, float prior_sd,
+                                  bh_stretch_factory** out, char* log, size_t log_size);
+/* Log posteriors (likelihood + prior, FP64) of n points beta (d x n). */
+int bh_glm_log_density(bh_stretch_factory* f, const float* beta, uint64_t n, int precision,
+                       double* out);
+
 /* SamplerFactory.create-sampler (amd_gcn.clj:514-567): walkers a multiple of
  * 2*WGS (BH_ERR_RANGE otherwise); params = [data, hyperparameters]; calls
  * init! with seed. */
@@ -154,6 +176,8 @@ int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
 int bh_stretch_destroy(bh_stretch* s);
 int bh_stretch_set_partner(bh_stretch* s, int partner_mode);
 int bh_stretch_set_exec(bh_stretch* s, int exec_mode);
+/* GLM samplers: precision of the likelihood contraction (BH_GLM_*). */
+int bh_stretch_set_glm_precision(bh_stretch* s, int precision);
 /* Info.info: {walker-count, iteration-counter} (amd_gcn.clj:321-324) */
 int bh_stretch_info(const bh_stretch* s, uint32_t* walkers, uint64_t* iteration_counter);
 

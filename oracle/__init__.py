@@ -109,6 +109,14 @@ def lib():
                                ctypes.POINTER(_c_u32)]
         L.bho_acor.restype = ctypes.c_int
         L.bho_set_variant.argtypes = [ctypes.c_int]
+        L.bho_glm_logpost.argtypes = [_f32p, _f32p, ctypes.c_uint64, _c_u32, ctypes.c_double,
+                                      _f32p]
+        L.bho_glm_logpost.restype = ctypes.c_double
+        L.bho_glm_stretch_half.argtypes = [
+            _c_u32, _c_u32, _c_u32, _c_u32, _c_u32, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p, _c_f32, _c_f32, ctypes.c_int, _f32p, _f32p, ctypes.c_uint64,
+            ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
+        L.bho_glm_stretch_half.restype = _c_u32
         _lib = L
     return _lib
 
@@ -473,3 +481,52 @@ class Stretch:
 
     def sd(self) -> np.ndarray:
         return np.sqrt(self.variance())
+
+
+# --------------------------------------------------------------------------
+# GLM (logistic regression) posterior, all in double
+# --------------------------------------------------------------------------
+def glm_logpost(Xd: np.ndarray, y: np.ndarray, prior_sd: float, beta: np.ndarray) -> np.ndarray:
+    """Log posteriors of the rows of beta (n, d); Xd holds the bf16 data as
+    float32 (N, d), y the labels."""
+    X = np.ascontiguousarray(Xd, np.float32)
+    yy = np.ascontiguousarray(y, np.float32)
+    B = np.ascontiguousarray(beta, np.float32).reshape(-1, X.shape[1])
+    pp = 1.0 / (float(prior_sd) ** 2)
+    return np.array([lib().bho_glm_logpost(X.reshape(-1), yy, X.shape[0], X.shape[1], pp, b)
+                     for b in B])
+
+
+class GlmStretch(Stretch):
+    """The engine's GLM sampler restated: the moves of ``Stretch`` with the
+    logistic log posterior computed in double and kept in double."""
+
+    def __init__(self, Xd, y, prior_sd, walkers, seed, wgs=256, partner=PARTNER_FLAT,
+                 threads=1):
+        self.Xd = np.ascontiguousarray(Xd, np.float32)
+        self.yv = np.ascontiguousarray(y, np.float32)
+        self.prior_prec = 1.0 / (float(prior_sd) ** 2)
+        super().__init__(-1, self.Xd.shape[1], walkers, np.zeros(1, np.float32), seed, wgs=wgs,
+                         partner=partner, threads=threads)
+        self.lp = np.zeros(walkers, np.float64)
+
+    def init_position(self, seed=None, limits=None, position=None):
+        if position is not None:
+            self.xs[:] = position.xs
+        else:
+            self.xs[:] = init_walkers(seed, np.asarray(limits, np.float32), self.dim, self.W)
+        self.lp = glm_logpost(self.Xd, self.yv, 1.0 / np.sqrt(self.prior_prec),
+                              self.xs.reshape(self.W, self.dim))
+        self.iteration_counter = 0
+        return self
+
+    def _half(self, seed, odd_or_even, step, first, a, beta, accepted=None):
+        K, D = self.K, self.dim
+        xo = 0 if first else K * D
+        so = K * D if first else 0
+        lo = 0 if first else K
+        return lib().bho_glm_stretch_half(
+            seed & 0xFFFFFFFF, odd_or_even, step & 0xFFFFFFFF, K, D, _ptr(self.xs, so),
+            _ptr(self.xs, xo), _ptr(self.lp, lo), np.float32(a), np.float32(beta), self.partner,
+            self.Xd.reshape(-1), self.yv, self.Xd.shape[0], self.prior_prec,
+            None if accepted is None else _ptr(accepted), self.threads)

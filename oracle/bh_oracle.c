@@ -224,11 +224,9 @@ static inline float pown_f(float z, int n) {
  *   q = isfinite(ly) ? z^(DIM-1) exp(beta (ly - lX)) : 0 ; accept iff u2 <= q (:68-71)
  * The wrap modulo K*DIM defines the reference's out-of-bounds read at
  * j0+i >= K*DIM (it only occurs for DIM > 1). */
-static inline int stretch_one(uint32_t k, uint32_t seed, uint32_t odd_or_even, uint32_t step,
-                              uint32_t K, uint32_t dim, const float* Scompl, float* X,
-                              float* logfn_X, float a, float beta, int partner_mode, int model,
-                              uint32_t data_len, uint32_t params_len, const float* params,
-                              float* Y) {
+static inline float propose_one(uint32_t k, uint32_t seed, uint32_t odd_or_even, uint32_t step,
+                                uint32_t K, uint32_t dim, const float* Scompl, const float* X,
+                                float a, int partner_mode, float* Y, float* z_out) {
     const uint32_t key[2] = {seed, 0xdecafbadu};
     const uint32_t ctr[4] = {k, step, odd_or_even, 0xbeeff00du};
     float u[4];
@@ -257,11 +255,23 @@ static inline int stretch_one(uint32_t k, uint32_t seed, uint32_t odd_or_even, u
             Y[i] = (g_variant & 2) ? xj + z * (X[k0 + i] - xj) : fmaf(z, X[k0 + i] - xj, xj);
         }
     }
+    *z_out = z;
+    return u[2];
+}
+
+static inline int stretch_one(uint32_t k, uint32_t seed, uint32_t odd_or_even, uint32_t step,
+                              uint32_t K, uint32_t dim, const float* Scompl, float* X,
+                              float* logfn_X, float a, float beta, int partner_mode, int model,
+                              uint32_t data_len, uint32_t params_len, const float* params,
+                              float* Y) {
+    float z;
+    const float u2 = propose_one(k, seed, odd_or_even, step, K, dim, Scompl, X, a, partner_mode,
+                                 Y, &z);
     const float ly = bho_model_logfn(model, data_len, params_len, params, dim, Y);
     const float q = isfinite(ly) ? pown_f(z, (int)dim - 1) * expf(beta * (ly - logfn_X[k])) : 0.0f;
-    const int accepted = u[2] <= q;
+    const int accepted = u2 <= q;
     if (accepted) {
-        for (uint32_t i = 0; i < dim; ++i) X[k0 + i] = Y[i];
+        for (uint32_t i = 0; i < dim; ++i) X[k * dim + i] = Y[i];
         logfn_X[k] = ly;
     }
     return accepted;
@@ -652,4 +662,64 @@ BHO_EXPORT int bho_acor(float* series, uint32_t dim, uint32_t n, uint32_t wgs, f
     free(c0acc);
     free(dacc);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Logistic-regression posterior (GLM path of the engine).  Not in the       */
+/* reference: its models loop over data rows inside LOGFN.  Everything in    */
+/* double: eta_i = x_i . beta (the bf16 data times fp32 coefficients are     */
+/* exact in double), ll = sum_i y_i eta_i - softplus(eta_i), prior          */
+/* beta ~ N(0, 1/prior_prec I).                                             */
+/* ------------------------------------------------------------------------ */
+BHO_EXPORT double bho_glm_logpost(const float* Xd, const float* y, uint64_t N, uint32_t d,
+                                  double prior_prec, const float* beta) {
+    double ll = 0.0;
+    for (uint64_t i = 0; i < N; ++i) {
+        double eta = 0.0;
+        for (uint32_t j = 0; j < d; ++j) eta = fma((double)Xd[i * d + j], (double)beta[j], eta);
+        ll += (double)y[i] * eta - (fmax(eta, 0.0) + log1p(exp(-fabs(eta))));
+    }
+    double q = 0.0;
+    for (uint32_t j = 0; j < d; ++j) q = fma((double)beta[j], (double)beta[j], q);
+    return ll - 0.5 * prior_prec * q;
+}
+
+/* One GLM half-step: the proposals of stretch_one, log posteriors kept in
+ * double, accept iff u2 <= z^(d-1) exp(beta * (float)(lp(Y) - lp(X))). */
+BHO_EXPORT uint32_t bho_glm_stretch_half(uint32_t seed, uint32_t odd_or_even, uint32_t step,
+                                         uint32_t K, uint32_t dim, const float* Scompl, float* X,
+                                         double* lp, float a, float beta, int partner_mode,
+                                         const float* Xd, const float* y, uint64_t N,
+                                         double prior_prec, uint8_t* accepted, int threads) {
+    uint32_t total = 0;
+#ifdef _OPENMP
+    if (threads <= 0) threads = 1;
+#pragma omp parallel num_threads(threads) reduction(+ : total)
+#endif
+    {
+        float* Y = (float*)malloc(sizeof(float) * dim);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t kk = 0; kk < (int64_t)K; ++kk) {
+            const uint32_t k = (uint32_t)kk;
+            float z;
+            const float u2 = propose_one(k, seed, odd_or_even, step, K, dim, Scompl, X, a,
+                                         partner_mode, Y, &z);
+            const double lpy = bho_glm_logpost(Xd, y, N, dim, prior_prec, Y);
+            const float q = isfinite(lpy)
+                                ? pown_f(z, (int)dim - 1) * expf(beta * (float)(lpy - lp[k]))
+                                : 0.0f;
+            const int acc = u2 <= q;
+            if (acc) {
+                for (uint32_t i = 0; i < dim; ++i) X[k * dim + i] = Y[i];
+                lp[k] = lpy;
+            }
+            if (accepted) accepted[k] = (uint8_t)acc;
+            total += (uint32_t)acc;
+        }
+        free(Y);
+    }
+    (void)threads;
+    return total;
 }

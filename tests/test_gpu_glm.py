@@ -1,0 +1,92 @@
+"""The GLM (logistic regression) path on the GPU against the fp64 oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bayadera_amd import _lib, engine, models
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(N, d, seed):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((N, d)) / np.sqrt(d)).astype(np.float32)
+    Xb = models.to_bf16(X)
+    Xd = models.from_bf16(Xb).reshape(N, d)
+    beta = rng.standard_normal(d)
+    p = 1.0 / (1.0 + np.exp(-(Xd.astype(np.float64) @ beta)))
+    y = (rng.random(N) < p).astype(np.float32)
+    return Xb, Xd, y
+
+
+@pytest.fixture(scope="module")
+def fac():
+    f = engine.HipBayaderaFactory(0, 256)
+    yield f
+    f.release()
+
+
+@pytest.mark.parametrize("d", [32, 64])
+@pytest.mark.parametrize("N", [1, 1000, 20011])
+def test_glm_log_density_precisions(fac, N, d):
+    Xb, Xd, y = _problem(N, d, 7 + N + d)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
+    rng = np.random.default_rng(3)
+    beta = (rng.standard_normal((700, d)) * 0.7).astype(np.float32)
+    ref = O.glm_logpost(Xd, y, 5.0, beta)
+    scale = np.abs(ref) + 1.0
+    f64 = sf.log_density(beta, _lib.BH_GLM_FP64)
+    assert np.max(np.abs(f64 - ref) / scale) < 1e-12
+    f32 = sf.log_density(beta, _lib.BH_GLM_FP32)
+    x3 = sf.log_density(beta, _lib.BH_GLM_BF16X3)
+    b1 = sf.log_density(beta, _lib.BH_GLM_BF16)
+    # fp32-accurate contractions: error of a few 1e-7 of |ll| per point
+    assert np.max(np.abs(f32 - ref) / scale) < 2e-6
+    assert np.max(np.abs(x3 - ref) / scale) < 2e-6
+    # one bf16 product: the proposals lose 16 mantissa bits
+    assert np.max(np.abs(b1 - ref) / scale) < 2e-2
+
+
+def test_glm_sampler_fp64_matches_oracle_trajectory(fac):
+    N, d, W = 3000, 32, 1024
+    Xb, Xd, y = _problem(N, d, 11)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
+    s = sf.create_sampler(5, W, np.zeros(1, np.float32)).set_glm_precision(_lib.BH_GLM_FP64)
+    s.set_partner(_lib.BH_PARTNER_WALKER)
+    s.init(5)
+    lim = np.tile(np.array([-0.5, 0.5], np.float32), d)
+    s.init_position(6, lim)
+    ref = O.GlmStretch(Xd, y, 5.0, W, 5, partner=O.PARTNER_WALKER, threads=8)
+    ref.init(5)
+    ref.init_position(6, lim.reshape(d, 2))
+    xs, lp, _ = s.state()
+    assert np.array_equal(xs.reshape(-1), ref.xs)
+    assert np.allclose(lp, ref.lp.astype(np.float32), rtol=1e-6)
+    s.burn_in(3, 2.0)
+    ref.burn_in(3, 2.0)
+    same = np.all(s.state()[0].reshape(-1, d) == ref.xs.reshape(-1, d), axis=1).mean()
+    assert same > 0.99, same
+
+
+@pytest.mark.parametrize("prec", [_lib.BH_GLM_FP32, _lib.BH_GLM_BF16X3, _lib.BH_GLM_BF16])
+def test_glm_sampler_runs_and_accounts(fac, prec):
+    N, d, W = 4096, 64, 2048
+    Xb, Xd, y = _problem(N, d, 13)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
+    s = sf.create_sampler(8, W, np.zeros(1, np.float32)).set_glm_precision(prec)
+    s.set_partner(_lib.BH_PARTNER_WALKER)
+    s.init_position(9, np.tile(np.array([-0.1, 0.1], np.float32), d))
+    s.burn_in(20, 2.0)
+    res = s.run_sampler(60, 2.0)
+    assert 0.05 < res["acceptance-rate"] < 0.9
+    assert np.all(np.isfinite(res["means"]))
+    xs, lp, _ = s.state()
+    ref = O.glm_logpost(Xd, y, 5.0, xs[:64])
+    assert np.allclose(lp[:64], ref, rtol=1e-5, atol=1e-2)
+
+
+def test_glm_walker_rule(fac):
+    Xb, Xd, y = _problem(100, 32, 1)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y))
+    with pytest.raises(_lib.ArgumentError, match="multiple of 1024"):
+        sf.create_sampler(1, 512, np.zeros(1, np.float32))
