@@ -50,6 +50,7 @@ SIGNATURES = {
     "bh_ctx_destroy": [_P],
     "bh_ctx_info": [_P, _P, _P],
     "bh_uniform_sample": [_P, _U32, _F, _F, _P, _U64],
+    "bh_direct_sample": [_P, _I, _U32, _P, _U32, _P, _U64],
     "bh_dist_engine_create": [_P, _S, _S, _U32, _PP, _P, _SZ],
     "bh_dist_engine_destroy": [_P],
     "bh_log_density": [_P, _P, _U32, _P, _U32, _U64, _P],

@@ -737,6 +737,26 @@ int bh_uniform_sample(bh_ctx* ctx, uint32_t seed, float lower, float upper, floa
     return sync(ctx->stream);
 }
 
+int bh_direct_sample(bh_ctx* ctx, int kind, uint32_t seed, const float* params,
+                     uint32_t params_count, float* out, uint64_t n) {
+    BH_ARG(ctx && params && out, "NULL argument");
+    BH_ARG(kind >= BH_DIRECT_UNIFORM && kind <= BH_DIRECT_ERLANG, "unknown sampler %d", kind);
+    const uint32_t need = (kind == BH_DIRECT_EXPONENTIAL) ? 1u : 2u;
+    BH_ARG(params_count >= need, "sampler %d needs %u params", kind, need);
+    BH_ARG(n % 4 == 0 && n / 4 <= 0xffffffffull,
+           "the direct sampler needs a multiple of 4 samples (amd_gcn.clj:55-62), got %llu",
+           (unsigned long long)n);
+    BH_HIP(hipSetDevice(ctx->device));
+    float p[2] = {0.0f, 0.0f};
+    BH_HIP(hipMemcpy(p, params, need * sizeof(float), hipMemcpyDefault));
+    DevOut o;
+    BH_TRY(o.init(out, n * sizeof(float)));
+    BH_HIP(bh::launch_direct_sample(ctx->stream, kind, seed, p[0], p[1], (unsigned)(n / 4),
+                                    o.as<float>()));
+    BH_TRY(o.finish(ctx->stream));
+    return sync(ctx->stream);
+}
+
 // ---------------------------------------------------------------- density
 int bh_dist_engine_create(bh_ctx* ctx, const char* source, const char* logpdf_name,
                           uint32_t params_len, bh_dist_engine** out, char* log,

@@ -52,6 +52,54 @@ uniform_sample_kernel(unsigned seed, float lower, float upper, unsigned quads,
     reinterpret_cast<float4*>(x)[g] = v;
 }
 
+// Gaussian (Box-Muller), exponential and Erlang direct samplers
+// (gaussian-sampler.cl:17-44, exponential-sampler.cl:16-29,
+// erlang-sampler.cl:20-40); kinds as BH_DIRECT_* in bayadera_hip.h.
+__global__ void __launch_bounds__(256)
+direct_sample_kernel(int kind, unsigned seed, float p0, float p1, unsigned quads,
+                     float* __restrict__ x) {
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= quads) return;
+    float r[4];
+    if (kind == 3) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (unsigned i = 0; (float)i < p1; ++i) {
+            const bh_f32x4 u = bh_u01x4(
+                bh_philox4x32_10(bh_u32x4{g, BH_CTR1_INIT, BH_CTR2_INIT, i}, seed, BH_KEY1_INIT));
+            acc[0] += logf(u.x);
+            acc[1] += logf(u.y);
+            acc[2] += logf(u.z);
+            acc[3] += logf(u.w);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) r[m] = -(acc[m] / p0);
+    } else {
+        const bh_f32x4 u = bh_u01x4(
+            bh_philox4x32_10(bh_u32x4{g, BH_CTR1_INIT, BH_CTR2_INIT, BH_CTR3}, seed, BH_KEY1_INIT));
+        if (kind == 1) {
+            const float two_pi = 6.2831855f;
+            const float ra = sqrtf(-2.0f * logf(u.y)), rb = sqrtf(-2.0f * logf(u.w));
+            r[0] = __builtin_fmaf(sinf(two_pi * u.x) * ra, p1, p0);
+            r[1] = __builtin_fmaf(cosf(two_pi * u.x) * ra, p1, p0);
+            r[2] = __builtin_fmaf(sinf(two_pi * u.z) * rb, p1, p0);
+            r[3] = __builtin_fmaf(cosf(two_pi * u.z) * rb, p1, p0);
+        } else if (kind == 2) {
+            const float c = -1.0f / p0;
+            r[0] = c * logf(1.0f - u.x);
+            r[1] = c * logf(1.0f - u.y);
+            r[2] = c * logf(1.0f - u.z);
+            r[3] = c * logf(1.0f - u.w);
+        } else {
+            const float range = p1 - p0;
+            r[0] = __builtin_fmaf(u.x, range, p0);
+            r[1] = __builtin_fmaf(u.y, range, p0);
+            r[2] = __builtin_fmaf(u.z, range, p0);
+            r[3] = __builtin_fmaf(u.w, range, p0);
+        }
+    }
+    reinterpret_cast<float4*>(x)[g] = make_float4(r[0], r[1], r[2], r[3]);
+}
+
 // --------------------------------------------------------------------------
 // Dataset engine (amd-gcn-estimate.cl; host amd_gcn.clj:137-214)
 // --------------------------------------------------------------------------
@@ -498,6 +546,12 @@ hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits
 hipError_t launch_uniform_sample(hipStream_t s, unsigned seed, float lo, float hi, unsigned quads,
                                  float* x) {
     uniform_sample_kernel<<<(quads + 255) / 256, 256, 0, s>>>(seed, lo, hi, quads, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_direct_sample(hipStream_t s, int kind, unsigned seed, float p0, float p1,
+                                unsigned quads, float* x) {
+    direct_sample_kernel<<<(quads + 255) / 256, 256, 0, s>>>(kind, seed, p0, p1, quads, x);
     return hipGetLastError();
 }
 

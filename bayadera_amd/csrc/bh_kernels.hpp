@@ -10,6 +10,8 @@ hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits
                                unsigned walkers, float* xs);
 hipError_t launch_uniform_sample(hipStream_t s, unsigned seed, float lo, float hi, unsigned quads,
                                  float* x);
+hipError_t launch_direct_sample(hipStream_t s, int kind, unsigned seed, float p0, float p1,
+                                unsigned quads, float* x);
 size_t dataset_scratch_bytes(unsigned m, unsigned long long n);
 hipError_t launch_min_max(hipStream_t s, const float* a, unsigned m, unsigned long long n,
                           unsigned ld, float* limits, void* scratch);

@@ -149,21 +149,23 @@ class HipBayaderaFactory(_Handle):
 
 
 class HipDirectSamplerEngine:
-    """GCNDirectSamplerEngine (amd_gcn.clj:49-62); the uniform model."""
+    """GCNDirectSamplerEngine (amd_gcn.clj:49-62) for the library models with
+    direct samplers (rng/NAME-sampler.cl): uniform, gaussian, exponential, erlang."""
+    KINDS = {"uniform": 0, "gaussian": 1, "exponential": 2, "erlang": 3}
 
     def __init__(self, factory: HipBayaderaFactory, model: DistributionModel):
-        if model.name != "uniform":
+        if model.name not in self.KINDS:
             raise NotImplementedError(f"no direct sampler for {model.name}")
-        self.factory = factory
+        self.factory, self.kind = factory, self.KINDS[model.name]
 
     def sample(self, seed: int, params, n: int, out=None):
         if n % 4:
             raise ValueError("GCN direct sampler supports only matrices with ncols and offset "
                              "that are multiple of 4.")
-        p = np.asarray(params, np.float32)
+        p = _Arr(np.asarray(params, np.float32).reshape(-1))
         res, o = _out(out, (n,))
-        call("bh_uniform_sample", self.factory.handle, int(seed) & 0xFFFFFFFF, float(p[0]),
-             float(p[1]), o.ptr, n)
+        call("bh_direct_sample", self.factory.handle, self.kind, int(seed) & 0xFFFFFFFF, p.ptr,
+             p.size, o.ptr, n)
         return res
 
 

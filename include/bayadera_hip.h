@@ -91,6 +91,15 @@ int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs);
 /* ---------------------------------------------------------------------- */
 int bh_uniform_sample(bh_ctx* ctx, uint32_t seed, float lower, float upper, float* out,
                       uint64_t n);
+/* The other direct samplers of GCNDirectSamplerEngine (rng/NAME-sampler.cl):
+ * kind BH_DIRECT_GAUSSIAN params {mu, sigma}, BH_DIRECT_EXPONENTIAL {lambda},
+ * BH_DIRECT_ERLANG {lambda, k}, BH_DIRECT_UNIFORM {lower, upper}. */
+#define BH_DIRECT_UNIFORM 0
+#define BH_DIRECT_GAUSSIAN 1
+#define BH_DIRECT_EXPONENTIAL 2
+#define BH_DIRECT_ERLANG 3
+int bh_direct_sample(bh_ctx* ctx, int kind, uint32_t seed, const float* params,
+                     uint32_t params_count, float* out, uint64_t n);
 
 /* ---------------------------------------------------------------------- */
 /* Distribution engine: gcn-distribution-engine + GCNDistributionEngine      */

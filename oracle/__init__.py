@@ -109,6 +109,7 @@ def lib():
                                ctypes.POINTER(_c_u32)]
         L.bho_acor.restype = ctypes.c_int
         L.bho_set_variant.argtypes = [ctypes.c_int]
+        L.bho_direct_sample.argtypes = [ctypes.c_int, _c_u32, _c_f32, _c_f32, _c_u32, _f32p]
         L.bho_glm_logpost.argtypes = [_f32p, _f32p, ctypes.c_uint64, _c_u32, ctypes.c_double,
                                       _f32p]
         L.bho_glm_logpost.restype = ctypes.c_double
@@ -149,6 +150,19 @@ def uniform_sample(seed: int, lower: float, upper: float, n: int) -> np.ndarray:
         raise ValueError("n must be a multiple of 4")
     out = np.zeros(n, np.float32)
     lib().bho_uniform_sample(seed, np.float32(lower), np.float32(upper), n // 4, out)
+    return out
+
+
+DIRECT_UNIFORM, DIRECT_GAUSSIAN, DIRECT_EXPONENTIAL, DIRECT_ERLANG = 0, 1, 2, 3
+
+
+def direct_sample(kind: int, seed: int, params, n: int) -> np.ndarray:
+    """The direct samplers of rng/NAME-sampler.cl for n (a multiple of 4) draws."""
+    if n % 4:
+        raise ValueError("n must be a multiple of 4")
+    p = list(np.asarray(params, np.float32)) + [np.float32(0)]
+    out = np.zeros(n, np.float32)
+    lib().bho_direct_sample(kind, seed & 0xFFFFFFFF, p[0], p[1], n // 4, out)
     return out
 
 

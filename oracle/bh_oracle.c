@@ -89,6 +89,49 @@ BHO_EXPORT void bho_uniform_sample(uint32_t seed, float lower, float upper,
     }
 }
 
+/* Direct samplers of rng/NAME-sampler.cl (the uniform one above):
+ *   kind 1 gaussian    (gaussian-sampler.cl:17-44): Box-Muller, sin/cos of
+ *          2pi*u0 (2pi*u2) times sqrt(-2 log u1) (sqrt(-2 log u3)), then
+ *          fma(., sigma, mu) with params {mu, sigma}
+ *   kind 2 exponential (exponential-sampler.cl:16-29): (-1/lambda) log(1-u)
+ *   kind 3 erlang      (erlang-sampler.cl:20-40): -(sum_{i<k} log u_i)/lambda,
+ *          counter word 3 = i, params {lambda, k}
+ * The reference uses native_sin/cos/log there; this uses the C library's. */
+BHO_EXPORT void bho_direct_sample(int kind, uint32_t seed, float p0, float p1, uint32_t n_quads,
+                                  float* out) {
+    const uint32_t key[2] = {seed, 0xdecafaaau};
+    const float two_pi = 6.2831855f;
+    for (uint32_t g = 0; g < n_quads; ++g) {
+        float r[4];
+        if (kind == 3) {
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t i = 0; (float)i < p1; ++i) {
+                const uint32_t ctr[4] = {g, 0xf00dcafeu, 0xdeadbeefu, i};
+                float u[4];
+                u01x4(ctr, key, u);
+                for (int m = 0; m < 4; ++m) acc[m] += logf(u[m]);
+            }
+            for (int m = 0; m < 4; ++m) r[m] = -(acc[m] / p0);
+        } else {
+            const uint32_t ctr[4] = {g, 0xf00dcafeu, 0xdeadbeefu, 0xbeeff00du};
+            float u[4];
+            u01x4(ctr, key, u);
+            if (kind == 1) {
+                const float ra = sqrtf(-2.0f * logf(u[1])), rb = sqrtf(-2.0f * logf(u[3]));
+                const float bm[4] = {sinf(two_pi * u[0]) * ra, cosf(two_pi * u[0]) * ra,
+                                     sinf(two_pi * u[2]) * rb, cosf(two_pi * u[2]) * rb};
+                for (int m = 0; m < 4; ++m) r[m] = fmaf(bm[m], p1, p0);
+            } else if (kind == 2) {
+                const float c = -1.0f / p0;
+                for (int m = 0; m < 4; ++m) r[m] = c * logf(1.0f - u[m]);
+            } else {
+                for (int m = 0; m < 4; ++m) r[m] = fmaf(u[m], p1 - p0, p0);
+            }
+        }
+        for (int m = 0; m < 4; ++m) out[4 * g + m] = r[m];
+    }
+}
+
 /* init_walkers, stretch.cl:138-159: work-item g writes xs[4g .. 4g+3];
  * coordinate i gets limits row (i % dim); x = u*hi + (1-u)*lo, contracted
  * to fma(u, hi, (1-u)*lo).  Global size is dim*walkers/4 (amd_gcn.clj:399). */

@@ -266,3 +266,15 @@ def test_device_tensors_in_place(fac, gaussian_sf):
     assert torch.isfinite(out).all().item()
     m = fac.dataset_engine().data_mean(out)
     assert abs(float(m[0]) - float(out.double().mean())) < 1e-5
+
+
+@pytest.mark.parametrize("name,kind", [("gaussian", 1), ("exponential", 2), ("erlang", 3)])
+def test_direct_samplers_vs_oracle_and_goldens(fac, goldens, name, kind):
+    g = goldens["G2_direct_samplers"][name]
+    model = {"gaussian": models.gaussian_model, "exponential": models.exponential_model,
+             "erlang": models.erlang_model}[name]()
+    x = fac.direct_sampler_engine(model).sample(123, g["params"], 10000)
+    r = O.direct_sample(kind, 123, g["params"], 10000)
+    assert np.allclose(x, r, rtol=2e-6, atol=1e-6)
+    assert np.allclose(x[:4], g["first4"], rtol=2e-6, atol=0)
+    assert np.allclose(x[9996:], g["at9996"], rtol=2e-6, atol=0)

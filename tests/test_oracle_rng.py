@@ -61,3 +61,29 @@ def test_uniform_sampler_contraction_is_pinned(goldens):
     lo, hi = np.float32(g["params"][0]), np.float32(g["params"][1])
     unfused = (u * (hi - lo) + lo).astype(np.float32)
     assert unfused.tolist() != g["first4"]
+
+
+KINDS = {"gaussian": 1, "exponential": 2, "erlang": 3}
+
+
+@pytest.mark.parametrize("name", sorted(KINDS))
+def test_direct_samplers_goldens(goldens, name):
+    g = goldens["G2_direct_samplers"][name]
+    x = O.direct_sample(KINDS[name], 123, g["params"], 10000)
+    # the reference's native_sin/cos/log differ from the C library's in the
+    # last bits: 2e-6 relative
+    assert np.allclose(x[:4], g["first4"], rtol=2e-6, atol=0)
+    assert np.allclose(x[9996:], g["at9996"], rtol=2e-6, atol=0)
+    assert float(x.max()) == pytest.approx(g["max"], rel=2e-6)
+    assert float(x.min()) == pytest.approx(g["min"], rel=2e-6)
+    assert float(x.astype(np.float64).mean()) == pytest.approx(g["mean"], rel=1e-6)
+
+
+def test_direct_samplers_moments():
+    # closed forms: N(100, 3^2), Exp(4) mean 1/4, Erlang(2, 3) mean 3/2 var 3/4
+    x = O.direct_sample(1, 5, [100.0, 3.0], 400000).astype(np.float64)
+    assert abs(x.mean() - 100) < 0.03 and abs(x.std() - 3) < 0.02
+    x = O.direct_sample(2, 6, [4.0], 400000).astype(np.float64)
+    assert abs(x.mean() - 0.25) < 0.002
+    x = O.direct_sample(3, 7, [2.0, 3.0], 400000).astype(np.float64)
+    assert abs(x.mean() - 1.5) < 0.01 and abs(x.var() - 0.75) < 0.01
