@@ -90,6 +90,14 @@ SIGNATURES = {
     "bh_stretch_histogram": [_P, _U32, _P, _P, _P],
     "bh_stretch_mean": [_P, _P],
     "bh_stretch_variance": [_P, _P],
+    "bh_batch_create": [_P, _S, _S, _U32, _U32, _P, _P, _P, _P, _U32, _PP, _P, _SZ],
+    "bh_batch_destroy": [_P],
+    "bh_batch_set_partner": [_P, _I],
+    "bh_batch_init_position": [_P, _U32, _P],
+    "bh_batch_burn_in": [_P, _U32, _U64, _F],
+    "bh_batch_run": [_P, _U32, _U64, _F, _P, _P, _P],
+    "bh_batch_state_get": [_P, _P, _P],
+    "bh_batch_last_gpu_ms": [_P, _P],
     "bh_stretch_set_timing": [_P, _I],
     "bh_stretch_last_gpu_ms": [_P, _P],
     "bh_stretch_state_get": [_P, _P, _P, _P],
@@ -125,6 +133,13 @@ def load(path: str = LIB_PATH):
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and
+    # only works with that one, while this library needs any libamdhip64.so.7;
+    # whichever loads first serves both, so let torch's load first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise ImportError(
             f"{path} is missing: build the HIP engine first (python -c "

@@ -281,6 +281,19 @@ struct bh_stretch_factory {
                   glm_accept = nullptr;
 };
 
+struct bh_batch {
+    bh_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    Module mod;
+    hipFunction_t run = nullptr, init = nullptr, logfn = nullptr;
+    uint32_t max_dim = 0, problems = 0, W = 0, nb = 0, dim_hi = 0;
+    uint32_t bare_counter = 0;
+    int partner = BH_PARTNER_FLAT;
+    DevBuf dims, poff, dlen, plen, params, xs, logfn_buf, accept, sums;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
 struct bh_stretch {
     bh_stretch_factory* f = nullptr;
     bh_ctx* ctx = nullptr;
@@ -1008,7 +1021,7 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
     BH_ARG(ctx && X_bf16 && y && out, "NULL argument");
     BH_ARG(kind == BH_GLM_LOGISTIC, "unknown GLM kind %d", kind);
     BH_ARG(bh::glm_supported_dim(d), "GLM dimension %u not supported (32 or 64)", d);
-    BH_ARG(N > 0 && N < (1ull << 40), "bad row count");
+    BH_ARG(N > 0 && N < (1ull << 31), "bad row count");
     BH_ARG(prior_sd > 0.0f, "prior sd must be positive");
     BH_HIP(hipSetDevice(ctx->device));
     std::vector<char> code;
@@ -1098,9 +1111,6 @@ int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
     s->params_count = params_count;
     s->params_len = f->params_len;
     s->data_len = params_count > f->params_len ? params_count - f->params_len : 0u;
-    if (f->glm && walkers % 1024 != 0)
-        return fail(BH_ERR_RANGE, "GLM samplers need a multiple of 1024 walkers, got %u",
-                    walkers);
     BH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     if (f->glm) {
         BH_TRY(glm_alloc(s.get(), walkers, walkers / 2));
@@ -1173,6 +1183,217 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
     BH_ARG(s->f->glm, "not a GLM sampler");
     BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
     s->glm_prec = precision;
+    return BH_OK;
+}
+
+// ------------------------------------------------------------------ batch
+static size_t batch_lds(const bh_batch* b) {
+    return (size_t)2 * b->max_dim * sizeof(double) + (size_t)8 * b->nb * sizeof(float) +
+           ((size_t)b->W * b->dim_hi + b->W + (size_t)b->nb * b->dim_hi) * sizeof(float);
+}
+
+int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uint32_t max_dim,
+                    uint32_t problems, const uint32_t* dims, const uint64_t* offsets,
+                    const uint32_t* hyper_len, const float* params, uint32_t walkers,
+                    bh_batch** out, char* log, size_t log_size) {
+    BH_ARG(ctx && source && dims && offsets && hyper_len && params && out, "NULL argument");
+    BH_ARG(valid_name(logfn_name), "invalid logfn function name");
+    BH_ARG(max_dim >= 1 && max_dim <= 256, "max_dim %u outside [1, 256]", max_dim);
+    BH_ARG(problems >= 1 && problems <= (1u << 20), "bad problem count %u", problems);
+    if (!(walkers >= 128 && walkers % 128 == 0 && walkers <= 8192))
+        return fail(BH_ERR_RANGE, "batched samplers need a multiple of 128 walkers (<= 8192), got %u",
+                    walkers);
+    BH_HIP(hipSetDevice(ctx->device));
+    auto b = std::make_unique<bh_batch>();
+    b->ctx = ctx;
+    b->max_dim = max_dim;
+    b->problems = problems;
+    b->W = walkers;
+    const uint32_t K = walkers / 2;
+    uint32_t nb = 1;
+    while (nb * 2 <= std::min(K, 1024u) && K % (nb * 2) == 0) nb *= 2;
+    b->nb = std::max(nb, 64u);
+    std::vector<uint32_t> dl(problems), pl(problems);
+    for (uint32_t i = 0; i < problems; ++i) {
+        BH_ARG(dims[i] >= 1 && dims[i] <= max_dim, "problem %u: dim %u outside [1, %u]", i,
+               dims[i], max_dim);
+        BH_ARG((uint64_t)dims[i] * walkers % 4 == 0, "problem %u: dim*walkers not a multiple of 4", i);
+        BH_ARG(offsets[i + 1] >= offsets[i] && offsets[i + 1] - offsets[i] >= hyper_len[i],
+               "problem %u: bad params range", i);
+        pl[i] = hyper_len[i];
+        dl[i] = (uint32_t)(offsets[i + 1] - offsets[i] - hyper_len[i]);
+        b->dim_hi = std::max(b->dim_hi, dims[i]);
+    }
+    int lds_max = 0;
+    BH_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    if (batch_lds(b.get()) > (size_t)lds_max)
+        return fail(BH_ERR_RANGE, "%u walkers x %u dims need %zu bytes of LDS per problem (%d available)",
+                    walkers, b->dim_hi, batch_lds(b.get()), lds_max);
+    std::vector<char> code;
+    BH_TRY(rtc_compile(ctx->arch, source,
+                       {"-DWGS=" + std::to_string(ctx->wgs), "-DDIM=" + std::to_string(max_dim),
+                        std::string("-DLOGFN=") + logfn_name, "-DBH_BATCH=1"},
+                       code, log, log_size));
+    BH_TRY(b->mod.load(code));
+    BH_TRY(b->mod.fn("bh_batch_run", &b->run));
+    BH_TRY(b->mod.fn("bh_batch_init", &b->init));
+    BH_TRY(b->mod.fn("bh_batch_logfn", &b->logfn));
+    BH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BH_HIP(hipEventCreate(&b->ev0));
+    BH_HIP(hipEventCreate(&b->ev1));
+    const uint64_t total = offsets[problems];
+    BH_TRY(b->dims.alloc((size_t)problems * 4));
+    BH_TRY(b->dlen.alloc((size_t)problems * 4));
+    BH_TRY(b->plen.alloc((size_t)problems * 4));
+    BH_TRY(b->poff.alloc((size_t)problems * 8));
+    BH_TRY(b->params.alloc(std::max<size_t>(total, 1) * sizeof(float)));
+    BH_TRY(b->xs.alloc((size_t)problems * walkers * max_dim * sizeof(float)));
+    BH_TRY(b->logfn_buf.alloc((size_t)problems * walkers * sizeof(float)));
+    BH_TRY(b->accept.alloc((size_t)problems * 8));
+    BH_TRY(b->sums.alloc((size_t)problems * max_dim * 2 * sizeof(double)));
+    BH_HIP(hipMemcpyAsync(b->dims.p, dims, (size_t)problems * 4, hipMemcpyDefault, b->stream));
+    BH_HIP(hipMemcpyAsync(b->dlen.p, dl.data(), (size_t)problems * 4, hipMemcpyDefault, b->stream));
+    BH_HIP(hipMemcpyAsync(b->plen.p, pl.data(), (size_t)problems * 4, hipMemcpyDefault, b->stream));
+    BH_HIP(hipMemcpyAsync(b->poff.p, offsets, (size_t)problems * 8, hipMemcpyDefault, b->stream));
+    if (total)
+        BH_HIP(hipMemcpyAsync(b->params.p, params, total * sizeof(float), hipMemcpyDefault,
+                              b->stream));
+    BH_HIP(hipMemsetAsync(b->xs.p, 0, (size_t)problems * walkers * max_dim * sizeof(float),
+                          b->stream));
+    BH_HIP(hipMemsetAsync(b->logfn_buf.p, 0, (size_t)problems * walkers * sizeof(float), b->stream));
+    BH_TRY(sync(b->stream));
+    *out = b.release();
+    return BH_OK;
+}
+
+int bh_batch_destroy(bh_batch* b) {
+    if (!b) return BH_OK;
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipStreamDestroy(b->stream);
+    }
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+    return BH_OK;
+}
+
+int bh_batch_set_partner(bh_batch* b, int mode) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(mode == BH_PARTNER_FLAT || mode == BH_PARTNER_WALKER, "unknown partner mode %d", mode);
+    b->partner = mode;
+    return BH_OK;
+}
+
+int bh_batch_init_position(bh_batch* b, uint32_t seed, const float* limits) {
+    BH_ARG(b && limits, "NULL argument");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    DevIn dl;
+    BH_TRY(dl.init(limits, (size_t)b->problems * b->max_dim * 2 * sizeof(float), b->stream));
+    const unsigned* dims = b->dims.as<unsigned>();
+    const unsigned long long* poff = b->poff.as<unsigned long long>();
+    const unsigned* dlen = b->dlen.as<unsigned>();
+    const unsigned* plen = b->plen.as<unsigned>();
+    const float* params = b->params.as<float>();
+    const float* lim = dl.as<float>();
+    float* xs = b->xs.as<float>();
+    float* lf = b->logfn_buf.as<float>();
+    unsigned W = b->W;
+    void* ai[] = {&seed, &dims, &poff, &dlen, &plen, &params, &lim, &xs, &lf, &W};
+    const unsigned gx = std::max(1u, (b->dim_hi * W / 4 + 255) / 256);
+    BH_HIP(hipModuleLaunchKernel(b->init, gx, b->problems, 1, 256, 1, 1, 0, b->stream, ai, nullptr));
+    void* al[] = {&dims, &poff, &dlen, &plen, &params, &xs, &lf, &W};
+    BH_HIP(hipModuleLaunchKernel(b->logfn, (W + 255) / 256, b->problems, 1, 256, 1, 1, 0, b->stream,
+                                 al, nullptr));
+    b->bare_counter = 0;
+    return sync(b->stream);
+}
+
+static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mode) {
+    BH_HIP(hipSetDevice(b->ctx->device));
+    const unsigned* dims = b->dims.as<unsigned>();
+    const unsigned long long* poff = b->poff.as<unsigned long long>();
+    const unsigned* dlen = b->dlen.as<unsigned>();
+    const unsigned* plen = b->plen.as<unsigned>();
+    const float* params = b->params.as<float>();
+    float* xs = b->xs.as<float>();
+    float* lf = b->logfn_buf.as<float>();
+    unsigned W = b->W;
+    int partner = b->partner;
+    unsigned long long* acc = b->accept.as<unsigned long long>();
+    double* sums = b->sums.as<double>();
+    BH_HIP(hipEventRecord(b->ev0, b->stream));
+    uint64_t done = 0;
+    while (done < n) {
+        unsigned step0 = b->bare_counter;
+        unsigned chunk = (unsigned)std::min<uint64_t>(n - done, 1u << 20);
+        void* args[] = {&seed0, &dims, &poff, &dlen, &plen, &params, &xs, &lf, &W, &a,
+                        &step0, &chunk, &partner, &mode, &acc, &sums};
+        BH_HIP(hipModuleLaunchKernel(b->run, b->problems, 1, 1, b->nb, 1, 1,
+                                     (unsigned)batch_lds(b), b->stream, args, nullptr));
+        b->bare_counter += chunk;
+        done += chunk;
+    }
+    BH_HIP(hipEventRecord(b->ev1, b->stream));
+    BH_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    BH_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    return BH_OK;
+}
+
+int bh_batch_burn_in(bh_batch* b, uint32_t seed0, uint64_t n, float a) {
+    BH_ARG(b, "NULL batch");
+    BH_HIP(hipMemsetAsync(b->accept.p, 0, (size_t)b->problems * 8, b->stream));
+    return batch_steps(b, seed0, n, a, 0);
+}
+
+int bh_batch_run(bh_batch* b, uint32_t seed0, uint64_t n, float a, double* acceptance_rate,
+                 double* mean, double* sd) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(n > 0, "no steps");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    BH_HIP(hipMemsetAsync(b->accept.p, 0, (size_t)b->problems * 8, b->stream));
+    BH_HIP(hipMemsetAsync(b->sums.p, 0, (size_t)b->problems * b->max_dim * 2 * sizeof(double),
+                          b->stream));
+    BH_TRY(batch_steps(b, seed0, n, a, 1));
+    std::vector<unsigned long long> acc(b->problems);
+    std::vector<double> sums((size_t)b->problems * b->max_dim * 2);
+    std::vector<uint32_t> dims(b->problems);
+    BH_HIP(hipMemcpyAsync(acc.data(), b->accept.p, acc.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    BH_HIP(hipMemcpyAsync(sums.data(), b->sums.p, sums.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    BH_HIP(hipMemcpyAsync(dims.data(), b->dims.p, dims.size() * 4, hipMemcpyDeviceToHost, b->stream));
+    BH_TRY(sync(b->stream));
+    const double cnt = (double)b->W * (double)n;
+    for (uint32_t p = 0; p < b->problems; ++p) {
+        if (acceptance_rate) acceptance_rate[p] = (double)acc[p] / cnt;
+        for (uint32_t i = 0; i < b->max_dim; ++i) {
+            const size_t o = (size_t)p * b->max_dim + i;
+            const bool valid = i < dims[p];
+            const double m = valid ? sums[o * 2] / cnt : 0.0;
+            const double v = valid ? std::max(0.0, sums[o * 2 + 1] / cnt - m * m) : 0.0;
+            if (mean) mean[o] = m;
+            if (sd) sd[o] = std::sqrt(v);
+        }
+    }
+    return BH_OK;
+}
+
+int bh_batch_state_get(const bh_batch* b, float* xs, float* logfn) {
+    BH_ARG(b, "NULL batch");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    if (xs)
+        BH_HIP(hipMemcpyAsync(xs, b->xs.p, (size_t)b->problems * b->W * b->max_dim * sizeof(float),
+                              hipMemcpyDefault, b->stream));
+    if (logfn)
+        BH_HIP(hipMemcpyAsync(logfn, b->logfn_buf.p, (size_t)b->problems * b->W * sizeof(float),
+                              hipMemcpyDefault, b->stream));
+    return sync(b->stream);
+}
+
+int bh_batch_last_gpu_ms(const bh_batch* b, double* ms) {
+    BH_ARG(b && ms, "NULL argument");
+    *ms = b->last_ms;
     return BH_OK;
 }
 

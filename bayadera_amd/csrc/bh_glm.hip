@@ -37,25 +37,65 @@ __device__ __forceinline__ void glm_tile(unsigned groups, unsigned& group, unsig
     range = (idx / groups) * 8 + (b & 7);
 }
 
-// MFMA kernel: v_mfma_f32_16x16x32_bf16.  A block of 8 waves owns 512
+// MFMA kernel: v_mfma_f32_16x16x32_bf16.  A block of 4 waves owns 256
 // points, 64 per wave as four 16-point column blocks whose B fragments (all
 // D features, NS split terms) stay in registers for the whole row range.
-// Lane layout (16x16x32): A lane l holds row l%16, features (l/16)*8..+7 of a
-// 32-feature chunk; B lane l holds point l%16, the same features; the
-// accumulator lane l holds rows (l/16)*4..+3 of point l%16.
+// Lane layout (16x16x32, checked on the device): A lane l holds row l%16,
+// features (l/16)*8..+7 of a 32-feature chunk; B lane l holds point l%16,
+// the same features; the accumulator lane l holds rows (l/16)*4..+3 of
+// point l%16.  The loop is software-pipelined: the matrix-core products of
+// row block i+1 are issued while the softplus epilogue of block i runs on
+// the VALU, and the scheduler is asked to interleave them.  Split terms are
+// accumulated smallest first.  Rows are masked only in the last block.
 template <int D, int NS>
-__global__ void __launch_bounds__(512)
-glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned long long N,
+__device__ __forceinline__ void glm_mfma_block(const bf16x8 (&a)[D / 32],
+                                               const bf16x8 (&bfrag)[4][D / 32][NS],
+                                               f32x4 (&c)[4]) {
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb) {
+        c[wb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = NS - 1; s >= 0; --s)
+#pragma unroll
+            for (int kc = 0; kc < D / 32; ++kc)
+                c[wb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], c[wb], 0,
+                                                                0, 0);
+    }
+}
+
+__device__ __forceinline__ void glm_epilogue(const f32x4 (&c)[4], double (&acc)[4]) {
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb) {
+        const float s4 = (softplus2(c[wb][0]) + softplus2(c[wb][1])) +
+                         (softplus2(c[wb][2]) + softplus2(c[wb][3]));
+        acc[wb] += (double)s4;
+    }
+}
+
+__device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double (&acc)[4],
+                                                    unsigned orow, unsigned r1) {
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb) {
+        float s4 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (orow + i < r1) s4 += softplus2(c[wb][i]);
+        acc[wb] += (double)s4;
+    }
+}
+
+template <int D, int NS>
+__global__ void __launch_bounds__(256)
+glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned N,
                          const __bf16* __restrict__ ys, unsigned n, unsigned groups,
-                         unsigned nranges, unsigned long long rows_per_range,
-                         double* __restrict__ partial) {
+                         unsigned nranges, unsigned rows_per_range, double* __restrict__ partial) {
     constexpr int KC = D / 32;
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
     unsigned group, range;
     glm_tile(groups, group, range);
     if (range >= nranges) return;
-    const unsigned w0 = group * 512u + wave * 64u;
+    const unsigned w0 = group * 256u + wave * 64u;
     const unsigned lr = lane & 15u, lg = lane >> 4;
 
     bf16x8 bfrag[4][KC][NS];
@@ -68,45 +108,63 @@ glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned long long N,
                 bfrag[wb][kc][s] = *reinterpret_cast<const bf16x8*>(
                     ys + ((size_t)s * n + w0 + wb * 16 + lr) * D + kc * 32 + lg * 8);
 
-    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
-    unsigned long long r1 = r0 + rows_per_range;
-    if (r1 > N) r1 = N;
+    const unsigned r0 = range * rows_per_range;
+    const unsigned r1 = (r0 >= N) ? r0 : ((N - r0 < rows_per_range) ? N : r0 + rows_per_range);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-
-    bf16x8 anext[KC];
-    auto load_a = [&](unsigned long long rb, bf16x8 (&a)[KC]) {
-        const unsigned long long row = rb + lr;
+    if (r0 >= r1) {
+        if (lg == 0)
+#pragma unroll
+            for (int wb = 0; wb < 4; ++wb) partial[(size_t)range * n + w0 + wb * 16 + lr] = 0.0;
+        return;
+    }
+    // full 16-row blocks [r0, rfull), then one masked block
+    const unsigned rfull = r0 + (r1 - r0) / 16 * 16;
+    const __bf16* xl = X + (size_t)lr * D + lg * 8;
+    auto load_full = [&](unsigned rb, bf16x8 (&a)[KC]) {
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+            a[kc] = *reinterpret_cast<const bf16x8*>(xl + (size_t)rb * D + kc * 32);
+    };
+    f32x4 c[4];
+    if (rfull > r0) {
+        const unsigned rlast = rfull - 16;
+        bf16x8 a[KC], an[KC];
+        load_full(r0, a);
+        load_full(r0 + 16 <= rlast ? r0 + 16 : rlast, an);
+        glm_mfma_block<D, NS>(a, bfrag, c);
+        for (unsigned rb = r0 + 16; rb < rfull; rb += 16) {
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) a[kc] = an[kc];
+            // prefetch the next block (clamped: the last iteration reloads
+            // the last block, which is in range)
+            load_full(rb + 16 <= rlast ? rb + 16 : rlast, an);
+            f32x4 cn[4];
+            glm_mfma_block<D, NS>(a, bfrag, cn);
+            glm_epilogue(c, acc);
+#pragma unroll
+            for (int i = 0; i < 4 * KC * NS; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+            }
+#pragma unroll
+            for (int wb = 0; wb < 4; ++wb) c[wb] = cn[wb];
+        }
+        glm_epilogue(c, acc);
+    }
+    if (rfull < r1) {
+        bf16x8 a[KC];
+        const unsigned row = rfull + lr;
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
             if (row < r1) {
-                a[kc] = *reinterpret_cast<const bf16x8*>(X + row * D + kc * 32 + lg * 8);
+                a[kc] = *reinterpret_cast<const bf16x8*>(X + (size_t)row * D + kc * 32 + lg * 8);
             } else {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) a[kc][j] = (__bf16)0.0f;
             }
         }
-    };
-    if (r0 < r1) load_a(r0, anext);
-    for (unsigned long long rb = r0; rb < r1; rb += 16) {
-        bf16x8 a[KC];
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc) a[kc] = anext[kc];
-        if (rb + 16 < r1) load_a(rb + 16, anext);
-        const unsigned long long orow = rb + lg * 4;
-#pragma unroll
-        for (int wb = 0; wb < 4; ++wb) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-                for (int s = NS - 1; s >= 0; --s)
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], c, 0, 0, 0);
-            float s4 = 0.0f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (orow + i < r1) s4 += softplus2(c[i]);
-            acc[wb] += (double)s4;
-        }
+        glm_mfma_block<D, NS>(a, bfrag, c);
+        glm_epilogue_masked(c, acc, rfull + lg * 4, r1);
     }
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
@@ -242,15 +300,15 @@ static hipError_t launch_d(int precision, hipStream_t s, const __bf16* X, unsign
                            unsigned long long rpr, float yscale, double* partial) {
     switch (precision) {
     case kGlmBf16: {
-        const unsigned groups = n / 512;
-        glm_logistic_mfma_kernel<D, 1><<<groups * nranges, 512, 0, s>>>(X, N, ys, n, groups,
-                                                                        nranges, rpr, partial);
+        const unsigned groups = n / 256;
+        glm_logistic_mfma_kernel<D, 1><<<groups * nranges, 256, 0, s>>>(
+            X, (unsigned)N, ys, n, groups, nranges, (unsigned)rpr, partial);
         break;
     }
     case kGlmBf16x3: {
-        const unsigned groups = n / 512;
-        glm_logistic_mfma_kernel<D, 3><<<groups * nranges, 512, 0, s>>>(X, N, ys, n, groups,
-                                                                        nranges, rpr, partial);
+        const unsigned groups = n / 256;
+        glm_logistic_mfma_kernel<D, 3><<<groups * nranges, 256, 0, s>>>(
+            X, (unsigned)N, ys, n, groups, nranges, (unsigned)rpr, partial);
         break;
     }
     case kGlmFp32: {
@@ -274,7 +332,8 @@ static hipError_t launch_d(int precision, hipStream_t s, const __bf16* X, unsign
 bool glm_supported_dim(unsigned d) { return d == 32 || d == 64; }
 
 unsigned glm_point_quantum(int precision) {
-    return (precision == kGlmBf16 || precision == kGlmBf16x3) ? 512u : 256u;
+    (void)precision;
+    return 256u;
 }
 
 hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsigned long long N,

@@ -73,6 +73,39 @@ __device__ __forceinline__ unsigned bh_slice_count(bool flag, unsigned* wave_cou
 
 #if defined(LOGFN) || defined(BH_GLM)
 
+// A walker's DIM coordinates.  Walkers start at multiples of DIM floats from
+// 256-byte aligned bases (and half-ensembles at K*DIM floats, K a multiple of
+// 256), so for DIM % 4 == 0 they are 16-byte aligned: one 128-bit access per
+// four coordinates.
+__device__ __forceinline__ void bh_load_walker(const float* p, float (&v)[DIM]) {
+    if constexpr (DIM % 4 == 0) {
+        const float4* q = reinterpret_cast<const float4*>(__builtin_assume_aligned(p, 16));
+#pragma unroll
+        for (int i = 0; i < DIM / 4; ++i) {
+            const float4 t = q[i];
+            v[4 * i] = t.x;
+            v[4 * i + 1] = t.y;
+            v[4 * i + 2] = t.z;
+            v[4 * i + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) v[i] = p[i];
+    }
+}
+
+__device__ __forceinline__ void bh_store_walker(float* p, const float (&v)[DIM]) {
+    if constexpr (DIM % 4 == 0) {
+        float4* q = reinterpret_cast<float4*>(__builtin_assume_aligned(p, 16));
+#pragma unroll
+        for (int i = 0; i < DIM / 4; ++i)
+            q[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) p[i] = v[i];
+    }
+}
+
 // The proposal half of the stretch move (amd-gcn-mcmc-stretch.cl:46-65):
 //   counter {k, step, odd_or_even, 0xbeeff00d}, key {seed, 0xdecafbad}
 //   z  = fma((a-2+1/a)*u1, u1, (2(1-1/a))*u1) + 1/a                     (:53-54)
@@ -81,9 +114,10 @@ __device__ __forceinline__ unsigned bh_slice_count(bool flag, unsigned* wave_cou
 // j0 = (uint)((u0*K)*DIM) (:57), wrapping modulo K*DIM where the reference
 // reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1).
 // Returns u2, the uniform of the accept test.
+template <int PARTNER>
 __device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned seed,
                                            unsigned odd_or_even, unsigned step, float a,
-                                           int partner, const float* S, const float (&xk)[DIM],
+                                           const float* S, const float (&xk)[DIM],
                                            float (&y)[DIM], float& z) {
     const bh_f32x4 u = bh_u01x4(
         bh_philox4x32_10(bh_u32x4{k, step, odd_or_even, BH_CTR3}, seed, BH_KEY1_STRETCH));
@@ -91,7 +125,7 @@ __device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned see
     const float ca = a - 2.0f + inv_a;
     const float cb = 2.0f * (1.0f - inv_a);
     z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
-    if (partner == BH_PARTNER_FLAT) {
+    if constexpr (PARTNER == BH_PARTNER_FLAT) {
         const unsigned total = K * DIM;
         unsigned e = (unsigned)((u.x * (float)K) * (float)DIM);
         if (e >= total) e -= total;
@@ -104,12 +138,10 @@ __device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned see
     } else {
         unsigned j = (unsigned)(u.x * (float)K);
         j = j < K ? j : K - 1;
-        const float* sj = S + (size_t)j * DIM;
+        float sj[DIM];
+        bh_load_walker(S + (size_t)j * DIM, sj);
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-            const float xj = sj[i];
-            y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
-        }
+        for (int i = 0; i < DIM; ++i) y[i] = __builtin_fmaf(z, xk[i] - sj[i], sj[i]);
     }
     return u.z;
 }
@@ -163,15 +195,16 @@ __device__ __forceinline__ void bh_accu(const float (&xk)[DIM], bool accepted, u
 // j0 = (uint)((u0*K)*DIM) (:57), wrapping modulo K*DIM where the reference
 // reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1).
 // xk holds the walker's coordinates on entry and its new ones on exit.
+template <int PARTNER>
 __device__ __forceinline__ bool bh_stretch_walker(unsigned k, unsigned K, unsigned seed,
                                                   unsigned odd_or_even, unsigned step, float a,
-                                                  float beta, int partner, const float* S,
+                                                  float beta, const float* S,
                                                   float (&xk)[DIM], float& lx,
                                                   unsigned data_len, unsigned params_len,
                                                   const float* params) {
     float y[DIM];
     float z;
-    const float u2 = bh_propose(k, K, seed, odd_or_even, step, a, partner, S, xk, y, z);
+    const float u2 = bh_propose<PARTNER>(k, K, seed, odd_or_even, step, a, S, xk, y, z);
     const float ly = LOGFN(data_len, params_len, params, DIM, y);
     const float q = __builtin_isfinite(ly) ? bh_zpow(z) * expf(beta * (ly - lx)) : 0.0f;
     const bool accepted = u2 <= q;
@@ -193,26 +226,24 @@ bh_logfn(unsigned data_len, unsigned params_len, const float* __restrict__ param
 
 // One half-step over K walkers, one thread per walker, WGS walkers per block
 // (stretch_move_bare / stretch_move_accu, amd-gcn-mcmc-stretch.cl:83-134).
-template <bool ACCU>
+template <bool ACCU, int PARTNER>
 __device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_even,
                                                 unsigned data_len, unsigned params_len,
                                                 const float* __restrict__ params,
                                                 const float* __restrict__ S, float* __restrict__ X,
                                                 float* __restrict__ logfn_X, float a, float beta,
-                                                unsigned step, unsigned K, int partner,
+                                                unsigned step, unsigned K,
                                                 unsigned* __restrict__ accept,
                                                 float* __restrict__ means) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float xk[DIM];
     float* xp = X + (size_t)k * DIM;
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) xk[i] = xp[i];
+    bh_load_walker(xp, xk);
     float lx = logfn_X[k];
-    const bool accepted = bh_stretch_walker(k, K, seed, odd_or_even, step, a, beta, partner, S,
-                                            xk, lx, data_len, params_len, params);
+    const bool accepted = bh_stretch_walker<PARTNER>(k, K, seed, odd_or_even, step, a, beta, S,
+                                                     xk, lx, data_len, params_len, params);
     if (accepted) {
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) xp[i] = xk[i];
+        bh_store_walker(xp, xk);
         logfn_X[k] = lx;
     }
     if (ACCU) {
@@ -227,8 +258,13 @@ bh_stretch_move_bare(unsigned seed, unsigned odd_or_even, unsigned data_len, uns
                      const float* __restrict__ params, const float* __restrict__ S,
                      float* __restrict__ X, float* __restrict__ logfn_X, float a, float beta,
                      unsigned step, unsigned K, int partner) {
-    bh_stretch_half<false>(seed, odd_or_even, data_len, params_len, params, S, X, logfn_X, a,
-                           beta, step, K, partner, nullptr, nullptr);
+    if (partner == BH_PARTNER_FLAT)
+        bh_stretch_half<false, BH_PARTNER_FLAT>(seed, odd_or_even, data_len, params_len, params, S,
+                                                X, logfn_X, a, beta, step, K, nullptr, nullptr);
+    else
+        bh_stretch_half<false, BH_PARTNER_WALKER>(seed, odd_or_even, data_len, params_len, params,
+                                                  S, X, logfn_X, a, beta, step, K, nullptr,
+                                                  nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(WGS)
@@ -238,8 +274,12 @@ bh_stretch_move_accu(unsigned seed, unsigned odd_or_even, unsigned data_len,
                      float* __restrict__ logfn_X, unsigned* __restrict__ accept,
                      float* __restrict__ means, float a, unsigned step, unsigned K,
                      int partner) {
-    bh_stretch_half<true>(seed, odd_or_even, data_len, params_len, params, S, X, logfn_X, a,
-                          1.0f, step, K, partner, accept, means);
+    if (partner == BH_PARTNER_FLAT)
+        bh_stretch_half<true, BH_PARTNER_FLAT>(seed, odd_or_even, data_len, params_len, params, S,
+                                               X, logfn_X, a, 1.0f, step, K, accept, means);
+    else
+        bh_stretch_half<true, BH_PARTNER_WALKER>(seed, odd_or_even, data_len, params_len, params,
+                                                 S, X, logfn_X, a, 1.0f, step, K, accept, means);
 }
 
 // Resident multi-step kernel: one work-group holds the whole ensemble
@@ -251,14 +291,13 @@ bh_stretch_move_accu(unsigned seed, unsigned odd_or_even, unsigned data_len,
 //   mode 0: bare steps (burn-in!, anneal! with betas[s], sample! with out)
 //   mode 1: accounting steps (move!): accept/means as stretch_move_accu
 // blockDim.x is a multiple of WGS dividing K.
-extern "C" __global__ void __launch_bounds__(1024)
-bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsigned c_even,
-                    unsigned data_len, unsigned params_len, const float* __restrict__ params,
-                    float* __restrict__ xs, float* __restrict__ logfn, unsigned K, float a,
-                    float beta, const float* __restrict__ betas, unsigned step0,
-                    unsigned nsteps, int partner, int mode, unsigned* __restrict__ accept,
-                    float* __restrict__ means, float* __restrict__ sample_out,
-                    unsigned long long sample_floats) {
+template <int PARTNER>
+__device__ __forceinline__ void bh_resident_impl(
+    unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsigned c_even, unsigned data_len,
+    unsigned params_len, const float* __restrict__ params, float* __restrict__ xs,
+    float* __restrict__ logfn, unsigned K, float a, float beta, const float* __restrict__ betas,
+    unsigned step0, unsigned nsteps, int mode, unsigned* __restrict__ accept,
+    float* __restrict__ means, float* __restrict__ sample_out, unsigned long long sample_floats) {
     extern __shared__ float lds[];
     const unsigned W = 2 * K;
     const unsigned nb = blockDim.x;
@@ -287,9 +326,9 @@ bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsig
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
                 float lx = L[k];
-                const bool acc = bh_stretch_walker(k, K, seed, oe, step, a, mode == 1 ? 1.0f : b,
-                                                   partner, S, xk, lx, data_len, params_len,
-                                                   params);
+                const bool acc = bh_stretch_walker<PARTNER>(k, K, seed, oe, step, a,
+                                                            mode == 1 ? 1.0f : b, S, xk, lx,
+                                                            data_len, params_len, params);
                 if (acc) {
 #pragma unroll
                     for (int i = 0; i < DIM; ++i) X[(size_t)k * DIM + i] = xk[i];
@@ -311,7 +350,202 @@ bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsig
     for (unsigned i = t; i < W; i += nb) logfn[i] = sl[i];
 }
 
+extern "C" __global__ void __launch_bounds__(1024)
+bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsigned c_even,
+                    unsigned data_len, unsigned params_len, const float* __restrict__ params,
+                    float* __restrict__ xs, float* __restrict__ logfn, unsigned K, float a,
+                    float beta, const float* __restrict__ betas, unsigned step0,
+                    unsigned nsteps, int partner, int mode, unsigned* __restrict__ accept,
+                    float* __restrict__ means, float* __restrict__ sample_out,
+                    unsigned long long sample_floats) {
+    if (partner == BH_PARTNER_FLAT)
+        bh_resident_impl<BH_PARTNER_FLAT>(seed_odd, seed_even, c_odd, c_even, data_len,
+                                          params_len, params, xs, logfn, K, a, beta, betas, step0,
+                                          nsteps, mode, accept, means, sample_out, sample_floats);
+    else
+        bh_resident_impl<BH_PARTNER_WALKER>(seed_odd, seed_even, c_odd, c_even, data_len,
+                                            params_len, params, xs, logfn, K, a, beta, betas,
+                                            step0, nsteps, mode, accept, means, sample_out,
+                                            sample_floats);
+}
+
 #endif  // LOGFN
+
+#ifdef BH_BATCH
+// Many independent posteriors of one model source in one launch: one
+// work-group per problem, its W walkers resident in LDS for all steps.  A
+// problem b with dimension dims[b] <= DIM and params params + poff[b] runs
+// exactly the single-sampler algorithm (move-bare!, constants 3333/4444)
+// with bare seed seed0 + 2b: the same streams, moves and accept tests as a
+// sampler created for that problem alone.  Positions are packed with the
+// problem's own dimension; a proposal is built in the thread's LDS slot,
+// where the model reads it.  Mode 1 also accumulates per problem, in double
+// and in a fixed order, the sums over steps of the ensemble sums of x and
+// x^2 (each step's ensemble sums are float tree sums over the walkers).
+// LDS: W*dim + W + nb*dim floats + 2*DIM doubles + 8*nb floats.
+extern "C" __global__ void __launch_bounds__(1024)
+bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
+             const unsigned long long* __restrict__ poff, const unsigned* __restrict__ dlen,
+             const unsigned* __restrict__ plen, const float* __restrict__ params,
+             float* __restrict__ xs, float* __restrict__ logfn, unsigned W, float a,
+             unsigned step0, unsigned nsteps, int partner, int mode,
+             unsigned long long* __restrict__ accept, double* __restrict__ sums) {
+    extern __shared__ double lds_d[];
+    const unsigned b = blockIdx.x;
+    const unsigned dim = dims[b];
+    const unsigned K = W / 2;
+    const unsigned t = threadIdx.x, nb = blockDim.x;
+    const float* prm = params + poff[b];
+    const unsigned data_len = dlen[b], params_len = plen[b];
+    float* gx = xs + (size_t)b * W * DIM;  // problem slot: W x DIM floats, packed by dim
+    float* gl = logfn + (size_t)b * W;
+    double* acc1 = lds_d;                  // DIM
+    double* acc2 = acc1 + DIM;             // DIM
+    float* red = reinterpret_cast<float*>(acc2 + DIM);  // 8 * nb
+    float* sx = red + 8 * nb;              // W * dim
+    float* sl = sx + (size_t)W * dim;      // W
+    float* yb = sl + W;                    // nb * dim
+    for (unsigned i = t; i < W * dim; i += nb) sx[i] = gx[i];
+    for (unsigned i = t; i < W; i += nb) sl[i] = gl[i];
+    for (unsigned i = t; i < 2 * DIM; i += nb) acc1[i] = 0.0;
+    __syncthreads();
+    const unsigned seed = seed0 + 2u * b;
+    unsigned long long nacc = 0;
+    const float inv_a = 1.0f / a;
+    const float ca = a - 2.0f + inv_a;
+    const float cb = 2.0f * (1.0f - inv_a);
+    float* y = yb + (size_t)t * dim;
+    for (unsigned st = 0; st < nsteps; ++st) {
+        const unsigned step = step0 + st;
+        for (int h = 0; h < 2; ++h) {
+            float* X = sx + (h ? (size_t)K * dim : 0);
+            const float* S = sx + (h ? 0 : (size_t)K * dim);
+            float* L = sl + (h ? K : 0);
+            for (unsigned k = t; k < K; k += nb) {
+                const bh_f32x4 u = bh_u01x4(bh_philox4x32_10(
+                    bh_u32x4{k, step, h ? 4444u : 3333u, BH_CTR3}, seed + (unsigned)h,
+                    BH_KEY1_STRETCH));
+                const float z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
+                const float* xk = X + (size_t)k * dim;
+                if (partner == BH_PARTNER_FLAT) {
+                    const unsigned total = K * dim;
+                    unsigned e = (unsigned)((u.x * (float)K) * (float)dim);
+                    if (e >= total) e -= total;
+                    for (unsigned i = 0; i < dim; ++i) {
+                        const float xj = S[e];
+                        y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+                        e = (e + 1 == total) ? 0u : e + 1;
+                    }
+                } else {
+                    unsigned j = (unsigned)(u.x * (float)K);
+                    j = j < K ? j : K - 1;
+                    for (unsigned i = 0; i < dim; ++i) {
+                        const float xj = S[(size_t)j * dim + i];
+                        y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+                    }
+                }
+                const float ly = LOGFN(data_len, params_len, prm, dim, y);
+                float zp = 1.0f;
+                for (unsigned i = 1; i < dim; ++i) zp *= z;
+                const float q = __builtin_isfinite(ly) ? zp * expf(ly - L[k]) : 0.0f;
+                if (u.z <= q) {
+                    for (unsigned i = 0; i < dim; ++i) X[(size_t)k * dim + i] = y[i];
+                    L[k] = ly;
+                    nacc += 1;
+                }
+            }
+            __syncthreads();
+        }
+        if (mode == 1) {
+            for (unsigned d0 = 0; d0 < dim; d0 += 4) {
+                float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                for (unsigned w = t; w < W; w += nb) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        if (d0 + c < dim) {
+                            const float x = sx[(size_t)w * dim + d0 + c];
+                            v[c] += x;
+                            v[4 + c] = __builtin_fmaf(x, x, v[4 + c]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 8; ++c) red[c * nb + t] = v[c];
+                __syncthreads();
+                for (unsigned r = nb / 2; r >= 1; r >>= 1) {
+                    if (t < r) {
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) red[c * nb + t] += red[c * nb + t + r];
+                    }
+                    __syncthreads();
+                }
+                if (t < 4 && d0 + t < dim) {
+                    acc1[d0 + t] += (double)red[t * nb];
+                    acc2[d0 + t] += (double)red[(4 + t) * nb];
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (unsigned i = t; i < W * dim; i += nb) gx[i] = sx[i];
+    for (unsigned i = t; i < W; i += nb) gl[i] = sl[i];
+    // accept count: block reduce in the (now free) staging area
+    __syncthreads();
+    unsigned long long* ra = reinterpret_cast<unsigned long long*>(red);
+    ra[t] = nacc;
+    __syncthreads();
+    for (unsigned r = nb / 2; r >= 1; r >>= 1) {
+        if (t < r) ra[t] += ra[t + r];
+        __syncthreads();
+    }
+    if (t == 0) accept[b] += ra[0];
+    if (mode == 1) {
+        for (unsigned i = t; i < dim; i += nb) {
+            sums[((size_t)b * DIM + i) * 2] += acc1[i];
+            sums[((size_t)b * DIM + i) * 2 + 1] += acc2[i];
+        }
+    }
+}
+
+// Initial positions and log-densities of every problem: init_walkers'
+// stream (key {seed0 + b, 0xdecafaaa}) over the problem's W*dim values,
+// limits (DIM x 2 per problem) taken row i % dim.
+extern "C" __global__ void __launch_bounds__(256)
+bh_batch_init(unsigned seed0, const unsigned* __restrict__ dims,
+              const unsigned long long* __restrict__ poff, const unsigned* __restrict__ dlen,
+              const unsigned* __restrict__ plen, const float* __restrict__ params,
+              const float* __restrict__ limits, float* __restrict__ xs, float* __restrict__ logfn,
+              unsigned W) {
+    const unsigned b = blockIdx.y;
+    const unsigned dim = dims[b];
+    float* gx = xs + (size_t)b * W * DIM;
+    const float* lim = limits + (size_t)b * DIM * 2;
+    const unsigned items = dim * W / 4;
+    for (unsigned g = blockIdx.x * 256 + threadIdx.x; g < items; g += gridDim.x * 256) {
+        const bh_f32x4 u = bh_u01x4(bh_philox4x32_10(
+            bh_u32x4{g, BH_CTR1_INIT, BH_CTR2_INIT, BH_CTR3}, seed0 + b, BH_KEY1_INIT));
+        const float uu[4] = {u.x, u.y, u.z, u.w};
+        for (unsigned m = 0; m < 4; ++m) {
+            const unsigned i = 4 * g + m;
+            const unsigned r = i % dim;
+            gx[i] = __builtin_fmaf(uu[m], lim[2 * r + 1], (1.0f - uu[m]) * lim[2 * r]);
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+bh_batch_logfn(const unsigned* __restrict__ dims, const unsigned long long* __restrict__ poff,
+               const unsigned* __restrict__ dlen, const unsigned* __restrict__ plen,
+               const float* __restrict__ params, const float* __restrict__ xs,
+               float* __restrict__ logfn, unsigned W) {
+    const unsigned b = blockIdx.y;
+    const unsigned w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= W) return;
+    const unsigned dim = dims[b];
+    const float* x = xs + (size_t)b * W * DIM + (size_t)w * dim;
+    logfn[(size_t)b * W + w] = LOGFN(dlen[b], plen[b], params + poff[b], dim, x);
+}
+#endif  // BH_BATCH
 
 #ifdef BH_GLM
 // Generalised linear model posteriors evaluated for a whole batch of walkers
@@ -347,7 +581,10 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
 #pragma unroll
     for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
     float z;
-    const float u2 = bh_propose(k, K, seed, odd_or_even, step, a, partner, S, xk, y, z);
+    const float u2 = partner == BH_PARTNER_FLAT
+                         ? bh_propose<BH_PARTNER_FLAT>(k, K, seed, odd_or_even, step, a, S, xk, y, z)
+                         : bh_propose<BH_PARTNER_WALKER>(k, K, seed, odd_or_even, step, a, S, xk,
+                                                         y, z);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) Y[(size_t)k * DIM + i] = y[i];
     zbuf[k] = z;

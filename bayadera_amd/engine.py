@@ -139,6 +139,9 @@ class HipBayaderaFactory(_Handle):
     def direct_sampler_engine(self, model: DistributionModel) -> "HipDirectSamplerEngine":
         return HipDirectSamplerEngine(self, model)
 
+    def batch(self, model: DistributionModel, params, dims, hyper_len, walkers: int) -> "HipBatch":
+        return HipBatch(self, model, params, dims, hyper_len, walkers)
+
     def mcmc_factory(self, model) -> "HipStretchFactory":
         if isinstance(model, GlmModel):
             return HipGlmStretchFactory(self, model)
@@ -333,6 +336,65 @@ class HipGlmStretchFactory(HipStretchFactory):
         out = np.zeros(n, np.float64)
         call("bh_glm_log_density", self._h, b.ptr, n, int(precision), out.ctypes.data)
         return out
+
+
+class HipBatch(_Handle):
+    """Many independent posteriors of one model in one launch (no reference
+    counterpart).  ``params``: one array per problem ([data, hyperparameters]
+    as create-sampler takes them), ``dims`` their dimensions (<= the model's
+    dimension), ``hyper_len`` each problem's hyperparameter count."""
+    _destroy = "bh_batch_destroy"
+
+    def __init__(self, factory: HipBayaderaFactory, model: DistributionModel, params, dims,
+                 hyper_len, walkers: int):
+        super().__init__()
+        self.factory, self.dist_model = factory, model
+        self.problems = len(params)
+        self.dims = np.ascontiguousarray(dims, np.uint32)
+        self.walkers = int(walkers)
+        self.max_dim = int(model.dimension)
+        offs = np.zeros(self.problems + 1, np.uint64)
+        offs[1:] = np.cumsum([len(p) for p in params])
+        flat = np.concatenate([np.asarray(p, np.float32) for p in params]).astype(np.float32)
+        hl = np.broadcast_to(np.asarray(hyper_len, np.uint32), (self.problems,)).copy()
+        log = _build_log()
+        call("bh_batch_create", factory.handle, _source(model), model.mcmc_logpdf.encode(),
+             self.max_dim, self.problems, self.dims.ctypes.data, offs.ctypes.data,
+             hl.ctypes.data, flat.ctypes.data, self.walkers, ctypes.byref(self._h), log, len(log))
+
+    def set_partner(self, mode: int):
+        call("bh_batch_set_partner", self._h, int(mode))
+        return self
+
+    def init_position(self, seed: int, limits):
+        lim = _Arr(np.asarray(limits, np.float32).reshape(self.problems, self.max_dim, 2))
+        call("bh_batch_init_position", self._h, int(seed) & 0xFFFFFFFF, lim.ptr)
+        return self
+
+    def burn_in(self, seed0: int, n: int, a: float):
+        call("bh_batch_burn_in", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a))
+        return self
+
+    def run(self, seed0: int, n: int, a: float) -> dict:
+        acc = np.zeros(self.problems, np.float64)
+        mean = np.zeros((self.problems, self.max_dim), np.float64)
+        sd = np.zeros((self.problems, self.max_dim), np.float64)
+        call("bh_batch_run", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a),
+             acc.ctypes.data, mean.ctypes.data, sd.ctypes.data)
+        return {"acceptance-rate": acc, "mean": mean, "sd": sd}
+
+    def state(self):
+        xs = np.zeros((self.problems, self.walkers * self.max_dim), np.float32)
+        lf = np.zeros((self.problems, self.walkers), np.float32)
+        call("bh_batch_state_get", self._h, xs.ctypes.data, lf.ctypes.data)
+        # problem b's positions are packed with its own dimension
+        return [xs[b, :self.walkers * d].reshape(self.walkers, d)
+                for b, d in enumerate(self.dims)], lf
+
+    def last_gpu_ms(self) -> float:
+        ms = ctypes.c_double()
+        call("bh_batch_last_gpu_ms", self._h, ctypes.byref(ms))
+        return ms.value
 
 
 class HipStretch(_Handle):

@@ -436,6 +436,34 @@ inline REAL gaussian_iid_logpdf(const uint data_len, const uint params_len, cons
 }
 """
 
+HBB_SOURCE = """
+// Hierarchical beta-binomial: x = [log alpha, log beta, t_1 .. t_G] with
+// theta_g = 1 / (1 + exp(-t_g)); data [n_1 .. n_G, y_1 .. y_G], hyperparameter
+// [s0]: log alpha, log beta ~ N(0, s0^2).  The logit transform's Jacobian
+// theta (1 - theta) is folded into the Beta(alpha, beta) prior's exponents.
+inline REAL hbb_softplus(const REAL t) {
+    return fmax(t, 0.0f) + native_log(1.0f + native_exp(-fabs(t)));
+}
+
+inline REAL hbb_logpdf(const uint data_len, const uint params_len, const REAL* params,
+                       const uint dim, const REAL* x) {
+    const uint G = dim - 2;
+    const REAL* n = params;
+    const REAL* y = params + G;
+    const REAL s0 = params[data_len];
+    const REAL la = x[0];
+    const REAL lb = x[1];
+    const REAL a = native_exp(la);
+    const REAL b = native_exp(lb);
+    REAL acc = -(REAL)G * (lgamma(a) + lgamma(b) - lgamma(a + b));
+    for (uint g = 0; g < G; g++) {
+        const REAL t = x[2 + g];
+        acc -= (y[g] + a) * hbb_softplus(-t) + (n[g] - y[g] + b) * hbb_softplus(t);
+    }
+    return acc - 0.5f * (la * la + lb * lb) / (s0 * s0);
+}
+"""
+
 _LIM_ALL = [[-FLT_MAX, FLT_MAX]]
 
 
@@ -547,6 +575,12 @@ def to_bf16(a) -> np.ndarray:
 
 def from_bf16(b) -> np.ndarray:
     return (np.ascontiguousarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def hbb_model(max_groups: int = 50) -> DistributionModel:
+    """Hierarchical beta-binomial with up to max_groups groups (dimension
+    max_groups + 2; each batched problem has its own G)."""
+    return distribution_model(HBB_SOURCE, name="hbb", dimension=max_groups + 2, params_size=1)
 
 
 def mvn_model(dim: int) -> DistributionModel:

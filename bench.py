@@ -316,6 +316,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the move-bare! roofline run, plus a 256 MiB device copy whose "
+                         "FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -323,6 +326,18 @@ def main():
         sys.exit(2)
     if args.workload == "c3":
         return main_c3(args)
+    if args.roofline_only:
+        import torch
+        from bayadera_amd import engine, models
+        _, _, params, _ = c1_problem()
+        fac = engine.HipBayaderaFactory(0, 256)
+        r = roofline(fac, models.mvn_model(DIM), params)
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline": r, "calibration_copy_bytes": src.numel() * 4}), flush=True)
+        return
 
     import torch
 

@@ -168,8 +168,7 @@ int bh_stretch_factory_destroy(bh_stretch_factory* f);
  * whole half-ensemble at once (no reference counterpart; it sits behind the
  * same SamplerFactory.create-sampler, amd_gcn.clj:514-567).  X: N x d bf16
  * (IEEE bfloat16 bit patterns, row-major), y: N labels in {0, 1}; d is 32
- * or 64; the data stay in HBM until the factory is destroyed.  Samplers
- * need a walker count that is a multiple of 1024. */
+ * or 64, N < 2^31; the data stay in HBM until the factory is destroyed. */
 int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, const float* y,
                                   uint64_t N, uint32_t d, float prior_sd,
                                   bh_stretch_factory** out, char* log, size_t log_size);
@@ -230,6 +229,38 @@ int bh_stretch_variance(bh_stretch* s, float* out);
  * counterpart: the reference has no timers (amd_gcn.clj:321-324). */
 int bh_stretch_set_timing(bh_stretch* s, int on);
 int bh_stretch_last_gpu_ms(const bh_stretch* s, double* ms);
+
+/* ---------------------------------------------------------------------- */
+/* Batched posteriors (no reference counterpart: the reference runs one      */
+/* posterior per sampler).  `problems` independent posteriors of one model   */
+/* source, problem b with dimension dims[b] <= max_dim and params            */
+/* params[offsets[b] .. offsets[b+1]) of which the last hyper_len[b] are     */
+/* hyperparameters (data_len = the rest, as create-sampler splits them).     */
+/* Every problem has `walkers` walkers (a multiple of 128) and runs the     */
+/* single-sampler algorithm with bare seed seed0 + 2b; one launch runs all   */
+/* problems, one work-group each, walkers resident in LDS.                   */
+/* ---------------------------------------------------------------------- */
+typedef struct bh_batch bh_batch;
+int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uint32_t max_dim,
+                    uint32_t problems, const uint32_t* dims, const uint64_t* offsets,
+                    const uint32_t* hyper_len, const float* params, uint32_t walkers,
+                    bh_batch** out, char* log, size_t log_size);
+int bh_batch_destroy(bh_batch* b);
+int bh_batch_set_partner(bh_batch* b, int partner_mode);
+/* init-position! of every problem: key {seed + b, 0xdecafaaa}; limits
+ * problems x max_dim x 2 (rows beyond a problem's dim are ignored). */
+int bh_batch_init_position(bh_batch* b, uint32_t seed, const float* limits);
+int bh_batch_burn_in(bh_batch* b, uint32_t seed0, uint64_t n, float a);
+/* n steps with accounting: per problem the acceptance rate and, over the n
+ * steps and all walkers, the posterior mean and sd of each coordinate
+ * (problems x max_dim; NULL to skip). */
+int bh_batch_run(bh_batch* b, uint32_t seed0, uint64_t n, float a, double* acceptance_rate,
+                 double* mean, double* sd);
+/* positions (problems x walkers x max_dim slots, each problem packed by its
+ * own dim) and logfn values (problems x walkers) */
+int bh_batch_state_get(const bh_batch* b, float* xs, float* logfn);
+/* GPU time of the last burn-in or run call (HIP events on the batch's stream) */
+int bh_batch_last_gpu_ms(const bh_batch* b, double* ms);
 
 /* Sampler state for resume: positions (DIM x W), logfn (W), and the host
  * counters {bare seed, move seed, bare counter, move counter, iteration

@@ -101,3 +101,30 @@ def test_bad_wgs_rejected():
     h = ctypes.c_void_p()
     assert _lib.load().bh_ctx_create(0, 100, ctypes.byref(h)) == _lib.BH_ERR_ARG
     assert "power of two" in _lib.last_error()
+
+
+@pytest.mark.parametrize("defines,source", [
+    (["-DWGS=256", "-DDIM=64", "-DBH_GLM=1"], ""),
+    (["-DWGS=256", "-DDIM=32", "-DBH_GLM=1"], ""),
+    (["-DWGS=256", "-DDIM=52", "-DBH_BATCH=1", "-DLOGFN=hbb_logpdf"], models.HBB_SOURCE),
+    (["-DWGS=256", "-DDIM=6", "-DBH_BATCH=1", "-DLOGFN=mvn_logpdf"], models.MVN_SOURCE),
+])
+def test_engine_variants_compile(tmp_path, defines, source):
+    """The GLM and batched variants of the run-time-compiled engine source
+    build for gfx950 (hipcc on the same text hipRTC receives)."""
+    import os
+    import shutil
+    import subprocess
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not available")
+    csrc = os.path.join(os.path.dirname(_lib.HERE), "bayadera_amd", "csrc")
+    text = ('#include "bh_rtc_prelude.h"\n#pragma clang force_cuda_host_device begin\n' + source
+            + "\n#pragma clang force_cuda_host_device end\n"
+            + open(os.path.join(csrc, "bh_rtc_engine.hip")).read())
+    src = tmp_path / "engine.hip"
+    src.write_text(text)
+    res = subprocess.run([hipcc, "--offload-arch=gfx950", "--cuda-device-only", "-O1",
+                          "-std=c++17", "-ffp-contract=on", "-I", csrc, *defines, "-c", str(src),
+                          "-o", str(tmp_path / "engine.o")], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-3000:]

@@ -1,0 +1,96 @@
+"""Batched posteriors: each problem of one launch equals the oracle's
+single sampler with the problem's seeds."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bayadera_amd import _lib, engine, models
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fac():
+    f = engine.HipBayaderaFactory(0, 256)
+    yield f
+    f.release()
+
+
+def _mvn_problems(B, seed):
+    rng = np.random.default_rng(seed)
+    dims, params = [], []
+    for _ in range(B):
+        d = int(rng.integers(1, 7))
+        mu = rng.standard_normal(d)
+        A = rng.standard_normal((d, d))
+        P = np.linalg.inv(A @ A.T + np.eye(d))
+        dims.append(d)
+        params.append(np.concatenate([mu, P.reshape(-1)]).astype(np.float32))
+    return dims, params
+
+
+@pytest.mark.parametrize("partner", [_lib.BH_PARTNER_FLAT, _lib.BH_PARTNER_WALKER])
+def test_batch_problems_equal_single_oracle_samplers(fac, partner):
+    B, W, MAXD = 12, 256, 6
+    dims, params = _mvn_problems(B, 4)
+    bt = fac.batch(models.mvn_model(MAXD), params, dims, [0] * B, W).set_partner(partner)
+    lim = np.zeros((B, MAXD, 2), np.float32)
+    lim[:, :, 0], lim[:, :, 1] = -4.0, 4.0
+    bt.init_position(100, lim)
+    bt.burn_in(50, 6, 2.0)
+    xs, lf = bt.state()
+    for b in range(B):
+        ref = O.Stretch(O.MVN_LOGPDF, dims[b], W, params[b], 50 + 2 * b, wgs=128,
+                        params_len=len(params[b]), partner=partner)
+        ref.init(50 + 2 * b)
+        ref.init_position(100 + b, lim[b, :dims[b]])
+        ref.burn_in(6, 2.0)
+        same = np.all(xs[b] == ref.xs.reshape(W, dims[b]), axis=1).mean()
+        assert same > 0.99, (b, same)
+
+
+def test_batch_moments_and_acceptance(fac):
+    # 1-D Gaussians N(mu_b, s_b): posterior means and sds recovered
+    B, W = 64, 512
+    rng = np.random.default_rng(9)
+    mus = rng.uniform(-3, 3, B).astype(np.float32)
+    sds = rng.uniform(0.5, 2, B).astype(np.float32)
+    params = [np.array([m, s], np.float32) for m, s in zip(mus, sds)]
+    bt = fac.batch(models.gaussian_model(), params, [1] * B, [2] * B, W)
+    lim = np.stack([mus - 3 * sds, mus + 3 * sds], 1).reshape(B, 1, 2)
+    bt.init_position(7, lim)
+    bt.burn_in(11, 200, 2.0)
+    res = bt.run(13, 400, 2.0)
+    assert np.all(np.abs(res["mean"][:, 0] - mus) < 0.08 * sds)
+    assert np.all(np.abs(res["sd"][:, 0] / sds - 1) < 0.08)
+    assert np.all((res["acceptance-rate"] > 0.5) & (res["acceptance-rate"] < 0.95))
+
+
+def test_batch_hierarchical_beta_binomial_runs(fac):
+    rng = np.random.default_rng(1004)
+    B, W, GMAX = 32, 256, 50
+    dims, params = [], []
+    for _ in range(B):
+        G = int(rng.integers(5, GMAX + 1))
+        n = np.round(np.exp(rng.uniform(np.log(10), np.log(1e5), G)))
+        th = rng.beta(2, 5, G)
+        y = rng.binomial(n.astype(np.int64), th).astype(np.float64)
+        dims.append(G + 2)
+        params.append(np.concatenate([n, y, [10.0]]).astype(np.float32))
+    bt = fac.batch(models.hbb_model(GMAX), params, dims, [1] * B, W)
+    lim = np.zeros((B, GMAX + 2, 2), np.float32)
+    for b in range(B):
+        G = dims[b] - 2
+        n, y = params[b][:G], params[b][G:2 * G]
+        t = np.log((y + 0.5) / (n - y + 0.5))
+        lim[b, 0] = [0.0, 1.5]
+        lim[b, 1] = [0.5, 2.0]
+        lim[b, 2:dims[b], 0] = t - 0.05
+        lim[b, 2:dims[b], 1] = t + 0.05
+    bt.init_position(3, lim)
+    bt.burn_in(5, 100, 2.0)
+    res = bt.run(6, 100, 2.0)
+    assert np.all(np.isfinite(res["mean"]))
+    assert np.all(res["acceptance-rate"] > 0.02)
+    _, lf = bt.state()
+    assert np.all(np.isfinite(lf))

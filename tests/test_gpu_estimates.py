@@ -25,7 +25,8 @@ def test_dataset_histogram_exact_vs_oracle(fac, m, n):
     h = fac.dataset_engine().histogram(a)
     hr = O.histogram(a)
     assert np.array_equal(h.limits, hr.limits)
-    assert np.array_equal(h.pdf, hr.pdf)
+    # n = 1 gives hi == lo: pdf = count * alpha / 0 (inf and nan), as the reference
+    assert np.array_equal(h.pdf, hr.pdf, equal_nan=True)
     assert np.array_equal(h.bin_ranks, hr.bin_ranks)
 
 
