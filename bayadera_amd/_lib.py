@@ -69,6 +69,10 @@ SIGNATURES = {
     "bh_glm_stretch_factory_create": [_P, _I, _P, _P, _U64, _U32, _F, _PP, _P, _SZ],
     "bh_glm_log_density": [_P, _P, _U64, _I, _P],
     "bh_stretch_set_glm_precision": [_P, _I],
+    "bh_stretch_set_glm_guard": [_P, ctypes.c_double],
+    "bh_stretch_set_kernel_timing": [_P, _I],
+    "bh_stretch_kernel_timing": [_P, _P, _P],
+    "bh_stretch_glm_guard_stats": [_P, _P, _P],
     "bh_stretch_create": [_P, _U32, _U32, _P, _U32, _PP],
     "bh_stretch_destroy": [_P],
     "bh_stretch_set_partner": [_P, _I],

@@ -269,7 +269,14 @@ struct GlmData {
     double prior_prec = 1.0;  // 1 / prior_sd^2
     DevBuf X;                 // bf16, N x d row-major
     DevBuf lin;               // double[d]: X'y
+    DevBuf lin_half;          // double[d]: X'y - (1/2) X'1, for the log2-unit kernels
 };
+
+// The linear coefficients of a precision's log-likelihood: the fp64 kernel
+// sums whole softplus values, the others leave out u/2 per row.
+const double* glm_lin(const GlmData& g, int prec) {
+    return (prec == BH_GLM_FP64 ? g.lin : g.lin_half).as<double>();
+}
 
 struct bh_stretch_factory {
     bh_ctx* ctx = nullptr;
@@ -278,7 +285,7 @@ struct bh_stretch_factory {
     hipFunction_t logfn = nullptr, bare = nullptr, accu = nullptr, resident = nullptr;
     std::unique_ptr<GlmData> glm;
     hipFunction_t glm_propose = nullptr, glm_split = nullptr, glm_logpost = nullptr,
-                  glm_accept = nullptr;
+                  glm_accept = nullptr, glm_screen = nullptr, glm_resolve = nullptr;
 };
 
 struct bh_batch {
@@ -313,7 +320,22 @@ struct bh_stretch {
     // GLM samplers: log posteriors in double, proposal and partial-sum buffers
     int glm_prec = BH_GLM_BF16X3;
     DevBuf lp, gY, gz, gu2, gsplit, gpartial;
+    // guarded accept test of the reduced precisions (guard 0: off)
+    double glm_guard = 0.0;
+    DevBuf gflag, glp, glist, gcount, gtotals, gpart64;
+    // event pairs around the dominant kernel's launches (the stretch move
+    // kernels, or the GLM likelihood contraction), up to the pool's size
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;
+    size_t kused = 0;
 };
+
+constexpr size_t kKernelTimerPairs = 1024;
+
+// Guard fallback: walkers re-evaluated in fp64 per half-step at most, and
+// the row ranges of that evaluation.
+constexpr unsigned kGuardListMax = 1024;
+constexpr unsigned kGuardRanges = 1024;
 
 namespace {
 
@@ -344,6 +366,20 @@ struct Timed {
         return BH_OK;
     }
 };
+
+// Launches a dominant kernel between two events of the sampler's timing pool
+// when kernel timing is on and the pool has room.
+template <class F>
+int ktimed(bh_stretch* s, F&& launch_fn) {
+    const bool rec = s->ktiming && s->kused + 2 <= s->kev.size();
+    if (rec) BH_HIP(hipEventRecord(s->kev[s->kused], s->stream));
+    BH_TRY(launch_fn());
+    if (rec) {
+        BH_HIP(hipEventRecord(s->kev[s->kused + 1], s->stream));
+        s->kused += 2;
+    }
+    return BH_OK;
+}
 
 // block size of the resident kernel: a multiple of WGS dividing K, <= 1024
 unsigned resident_block(const bh_stretch* s) {
@@ -385,8 +421,10 @@ int run_resident(bh_stretch* s, int mode, uint32_t seed_odd, uint32_t seed_even,
                     &params,   &xs,        &lf,      &K,           &a,        &beta,
                     &d_betas,  &step0,     &nsteps,  &partner,     &mode,     &acc,
                     &means,    &sample_out, &sample_floats};
-    return launch(s->f->resident, 1, resident_block(s), (unsigned)resident_lds(s, mode), s->stream,
-                  args);
+    return ktimed(s, [&] {
+        return launch(s->f->resident, 1, resident_block(s), (unsigned)resident_lds(s, mode),
+                      s->stream, args);
+    });
 }
 
 // ---------------------------------------------------------------- GLM
@@ -423,7 +461,7 @@ int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
     double* partial = s->gpartial.as<double>();
     BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, P, ys, n, nranges, rpr,
                                    yscale, partial));
-    const double* lin = g.lin.as<double>();
+    const double* lin = glm_lin(g, prec);
     double scale = prec == BH_GLM_FP64 ? 1.0 : kLn2;
     double pp = g.prior_prec;
     unsigned nr = nranges;
@@ -450,14 +488,20 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     int nsplit = glm_nsplit(prec);
     float yscale = kLog2e;
     const float* Xc = X;
-    void* pa[] = {&seed, &oe, &S, &Xc, &a, &step, &K, &partner, &Y, &z, &u2, &ys, &yscale, &nsplit};
+    const bool guarded = s->glm_guard > 0.0 && prec != BH_GLM_FP64;
+    unsigned* gcount = guarded ? s->gcount.as<unsigned>() : nullptr;
+    void* pa[] = {&seed, &oe, &S, &Xc, &a, &step, &K, &partner, &Y, &z, &u2, &ys, &yscale, &nsplit,
+                  &gcount};
     BH_TRY(launch(f->glm_propose, K / s->wgs, s->wgs, 0, s->stream, pa));
     unsigned nranges = glm_ranges(s->ctx, g, prec, K);
     const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
     double* partial = s->gpartial.as<double>();
-    BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, Y, ys, K, nranges, rpr,
-                                   yscale, partial));
-    const double* lin = g.lin.as<double>();
+    BH_TRY(ktimed(s, [&]() -> int {
+        BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, Y, ys, K, nranges, rpr,
+                                       yscale, partial));
+        return BH_OK;
+    }));
+    const double* lin = glm_lin(g, prec);
     double scale = prec == BH_GLM_FP64 ? 1.0 : kLn2;
     double pp = g.prior_prec;
     const float* Yc = Y;
@@ -466,8 +510,36 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     const double* pc = partial;
     unsigned* acc = accu ? s->accept.as<unsigned>() : nullptr;
     float* means = accu ? s->means_acc.as<float>() : nullptr;
+    unsigned* flag = nullptr;
+    double* glp = nullptr;
+    if (guarded) {
+        // screen, fp64 evaluation of the listed walkers, their decisions
+        flag = s->gflag.as<unsigned>();
+        glp = s->glp.as<double>();
+        unsigned* list = s->glist.as<unsigned>();
+        unsigned lmax = kGuardListMax;
+        double guard = s->glm_guard * std::max(1.0f, beta);
+        const double* lpc = lp;
+        void* sa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &lpc, &beta, &K, &guard,
+                      &flag, &list, &gcount, &lmax};
+        BH_TRY(launch(f->glm_screen, K / s->wgs, s->wgs, 0, s->stream, sa));
+        double* part64 = s->gpart64.as<double>();
+        BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, Y, X, list, gcount, lmax,
+                                        kGuardRanges, part64));
+        const double* p64 = part64;
+        unsigned nr64 = kGuardRanges, stride = 2 * kGuardListMax;
+        const double* lin64 = glm_lin(g, BH_GLM_FP64);
+        const unsigned* listc = list;
+        const unsigned* countc = gcount;
+        unsigned long long* totals = s->gtotals.as<unsigned long long>();
+        void* ra[] = {&Yc, &Xc, &zc, &u2c, &p64, &nr64, &stride, &lin64, &pp, &beta, &listc,
+                      &countc, &lmax, &flag, &glp, &totals};
+        BH_TRY(launch(f->glm_resolve, kGuardListMax / s->wgs, s->wgs, 0, s->stream, ra));
+    }
+    const unsigned* flagc = flag;
+    const double* glpc = glp;
     void* aa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &X, &lp, &beta, &K, &accu,
-                  &acc, &means, &mstep};
+                  &acc, &means, &mstep, &flagc, &glpc};
     return launch(f->glm_accept, K / s->wgs, s->wgs, 0, s->stream, aa);
 }
 
@@ -486,6 +558,13 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
         BH_TRY(s->gY.alloc((size_t)k_half * g.d * sizeof(float)));
         BH_TRY(s->gz.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gu2.alloc((size_t)k_half * sizeof(float)));
+        BH_TRY(s->gflag.alloc((size_t)k_half * sizeof(unsigned)));
+        BH_TRY(s->glp.alloc((size_t)k_half * sizeof(double)));
+        BH_TRY(s->glist.alloc((size_t)kGuardListMax * sizeof(unsigned)));
+        BH_TRY(s->gcount.alloc(sizeof(unsigned)));
+        BH_TRY(s->gtotals.alloc(2 * sizeof(unsigned long long)));
+        BH_TRY(s->gpart64.alloc((size_t)kGuardRanges * 2 * kGuardListMax * sizeof(double)));
+        BH_HIP(hipMemsetAsync(s->gtotals.p, 0, 2 * sizeof(unsigned long long), s->stream));
     }
     return BH_OK;
 }
@@ -512,8 +591,8 @@ int bare_pair(bh_stretch* s, float beta) {
     void* a0[] = {&seed0, &c0, &data_len, &params_len, &params, &S1, &x0, &l0, &a, &beta, &step, &K, &partner};
     void* a1[] = {&seed1, &c1, &data_len, &params_len, &params, &S0, &x1, &l1, &a, &beta, &step, &K, &partner};
     const unsigned grid = s->K / s->wgs;
-    BH_TRY(launch(s->f->bare, grid, s->wgs, 0, s->stream, a0));
-    BH_TRY(launch(s->f->bare, grid, s->wgs, 0, s->stream, a1));
+    BH_TRY(ktimed(s, [&] { return launch(s->f->bare, grid, s->wgs, 0, s->stream, a0); }));
+    BH_TRY(ktimed(s, [&] { return launch(s->f->bare, grid, s->wgs, 0, s->stream, a1); }));
     return BH_OK;
 }
 
@@ -542,8 +621,8 @@ int accu_pair(bh_stretch* s) {
     void* a0[] = {&seed0, &c0, &data_len, &params_len, &params, &S1, &x0, &l0, &acc, &means, &a, &step, &K, &partner};
     void* a1[] = {&seed1, &c1, &data_len, &params_len, &params, &S0, &x1, &l1, &acc, &means, &a, &step, &K, &partner};
     const unsigned grid = s->K / s->wgs;
-    BH_TRY(launch(s->f->accu, grid, s->wgs, 0, s->stream, a0));
-    BH_TRY(launch(s->f->accu, grid, s->wgs, 0, s->stream, a1));
+    BH_TRY(ktimed(s, [&] { return launch(s->f->accu, grid, s->wgs, 0, s->stream, a0); }));
+    BH_TRY(ktimed(s, [&] { return launch(s->f->accu, grid, s->wgs, 0, s->stream, a1); }));
     return BH_OK;
 }
 
@@ -1038,6 +1117,8 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
     BH_TRY(f->mod.fn("bh_glm_split_points", &f->glm_split));
     BH_TRY(f->mod.fn("bh_glm_logpost", &f->glm_logpost));
     BH_TRY(f->mod.fn("bh_glm_accept", &f->glm_accept));
+    BH_TRY(f->mod.fn("bh_glm_screen", &f->glm_screen));
+    BH_TRY(f->mod.fn("bh_glm_resolve", &f->glm_resolve));
     auto g = std::make_unique<GlmData>();
     g->kind = kind;
     g->N = N;
@@ -1054,7 +1135,17 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
     BH_TRY(part.alloc((size_t)chunks * d * sizeof(double)));
     BH_HIP(bh::launch_glm_lin(ctx->stream, g->X.p, dy.as<float>(), N, d, part.as<double>(),
                               chunks, g->lin.as<double>()));
+    BH_TRY(g->lin_half.alloc((size_t)d * sizeof(double)));
+    BH_HIP(bh::launch_glm_lin(ctx->stream, g->X.p, nullptr, N, d, part.as<double>(), chunks,
+                              g->lin_half.as<double>()));
+    std::vector<double> lin(d), colsum(d);
+    BH_HIP(hipMemcpyAsync(lin.data(), g->lin.p, d * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    BH_HIP(hipMemcpyAsync(colsum.data(), g->lin_half.p, d * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
     BH_TRY(sync(ctx->stream));
+    for (uint32_t j = 0; j < d; ++j) colsum[j] = lin[j] - 0.5 * colsum[j];
+    BH_HIP(hipMemcpy(g->lin_half.p, colsum.data(), d * sizeof(double), hipMemcpyHostToDevice));
     f->glm = std::move(g);
     *out = f.release();
     return BH_OK;
@@ -1139,6 +1230,7 @@ int bh_stretch_destroy(bh_stretch* s) {
     }
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    for (hipEvent_t e : s->kev) (void)hipEventDestroy(e);
     delete s;
     return BH_OK;
 }
@@ -1151,6 +1243,34 @@ int bh_stretch_set_timing(bh_stretch* s, int on) {
         BH_HIP(hipEventCreate(&s->ev1));
     }
     s->timing = on != 0;
+    return BH_OK;
+}
+
+int bh_stretch_set_kernel_timing(bh_stretch* s, int on) {
+    BH_ARG(s, "NULL sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    if (on && s->kev.empty()) {
+        s->kev.resize(2 * kKernelTimerPairs, nullptr);
+        for (hipEvent_t& e : s->kev) BH_HIP(hipEventCreate(&e));
+    }
+    BH_HIP(hipStreamSynchronize(s->stream));
+    s->ktiming = on != 0;
+    s->kused = 0;
+    return BH_OK;
+}
+
+int bh_stretch_kernel_timing(bh_stretch* s, double* avg_ms, uint32_t* launches) {
+    BH_ARG(s && avg_ms && launches, "NULL argument");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    BH_HIP(hipStreamSynchronize(s->stream));
+    double total = 0.0;
+    for (size_t i = 0; i + 1 < s->kused; i += 2) {
+        float ms = 0.0f;
+        BH_HIP(hipEventElapsedTime(&ms, s->kev[i], s->kev[i + 1]));
+        total += ms;
+    }
+    *launches = (uint32_t)(s->kused / 2);
+    *avg_ms = s->kused ? total / (double)(s->kused / 2) : 0.0;
     return BH_OK;
 }
 
@@ -1183,6 +1303,26 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
     BH_ARG(s->f->glm, "not a GLM sampler");
     BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
     s->glm_prec = precision;
+    return BH_OK;
+}
+
+int bh_stretch_set_glm_guard(bh_stretch* s, double guard) {
+    BH_ARG(s, "NULL sampler");
+    BH_ARG(s->f->glm, "not a GLM sampler");
+    BH_ARG(guard >= 0.0 && std::isfinite(guard), "guard must be finite and >= 0");
+    s->glm_guard = guard;
+    return BH_OK;
+}
+
+int bh_stretch_glm_guard_stats(bh_stretch* s, uint64_t* listed, uint64_t* overflow) {
+    BH_ARG(s && listed && overflow, "NULL argument");
+    BH_ARG(s->f->glm, "not a GLM sampler");
+    BH_HIP(hipSetDevice(s->ctx->device));
+    unsigned long long t[2];
+    BH_HIP(hipMemcpyAsync(t, s->gtotals.p, sizeof(t), hipMemcpyDeviceToHost, s->stream));
+    BH_TRY(sync(s->stream));
+    *listed = t[0];
+    *overflow = t[1];
     return BH_OK;
 }
 

@@ -1,11 +1,15 @@
 // Logistic-regression log-likelihood of a batch of n parameter vectors,
 //   ll(b) = y.(X b) - sum_i softplus(x_i . b),
 // the dense part of which is the contraction U = X.B (N x n) over the d
-// features.  These kernels produce, for each row range r and point k,
-//   partial[r][k] = sum_{i in r} softplus(x_i . b_k)
-// (in log2 units for the fp32 and MFMA kernels, which contract against
-// log2(e)*b; in natural units for the fp64 kernel).  The linear term y.(X b)
-// = (X'y).b and the prior are added per point by the walker-side kernels.
+// features.  The fp64 kernel produces, for each row range r and point k,
+//   partial[r][k] = sum_{i in r} softplus(x_i . b_k)        (natural units).
+// The fp32 and MFMA kernels contract against log2(e)*b, so u = log2(e) x.b,
+// and use softplus2(u) = max(u,0) + log2(1 + 2^-|u|) with max(u,0) =
+// (u + |u|)/2; the u/2 part sums to (log2 e / 2) (sum_i x_i).b, which is
+// linear in b and added per point, so these kernels produce
+//   partial[r][k] = sum_{i in r} |u_i|/2 + log2(1 + 2^-|u_i|)   (log2 units).
+// The linear terms y.(X b) = (X'y).b and -(1/2)(X'1).b and the prior are
+// added per point by the walker-side kernels.
 //
 // X is bf16, row-major N x D (the data as stored).  U is never materialised:
 // each tile is reduced through the softplus epilogue as it comes out of the
@@ -19,11 +23,18 @@ namespace bh {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// log2(1 + 2^u) = max(u, 0) + log2(1 + 2^-|u|): one v_exp_f32 and one
-// v_log_f32, both on arguments where they are accurate to about an ulp.
-__device__ __forceinline__ float softplus2(float u) {
-    const float e = __builtin_amdgcn_exp2f(-__builtin_fabsf(u));
-    return __builtin_fmaxf(u, 0.0f) + __builtin_amdgcn_logf(1.0f + e);
+// sum over four values of |u|/2 + log2(1 + 2^-|u|): four v_exp_f32 and one
+// v_log_f32 of the product of the four factors 1 + 2^-|u|, which lies in
+// [1, 16] (each factor is in (1, 2]); every transcendental runs where it is
+// accurate to about an ulp.
+__device__ __forceinline__ float softplus2_rest4(float u0, float u1, float u2, float u3) {
+    const float p01 = (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u0))) *
+                      (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u1)));
+    const float p23 = (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u2))) *
+                      (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u3)));
+    const float a = (__builtin_fabsf(u0) + __builtin_fabsf(u1)) +
+                    (__builtin_fabsf(u2) + __builtin_fabsf(u3));
+    return __builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p01 * p23));
 }
 
 // Block index -> (walker group, row range), XCD-aware: blocks are dealt to
@@ -65,22 +76,25 @@ __device__ __forceinline__ void glm_mfma_block(const bf16x8 (&a)[D / 32],
 
 __device__ __forceinline__ void glm_epilogue(const f32x4 (&c)[4], double (&acc)[4]) {
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb) {
-        const float s4 = (softplus2(c[wb][0]) + softplus2(c[wb][1])) +
-                         (softplus2(c[wb][2]) + softplus2(c[wb][3]));
-        acc[wb] += (double)s4;
-    }
+    for (int wb = 0; wb < 4; ++wb)
+        acc[wb] += (double)softplus2_rest4(c[wb][0], c[wb][1], c[wb][2], c[wb][3]);
 }
 
+// rows at or past r1 contribute nothing: u = +inf gives the factor 1 + 0,
+// and its |u|/2 is taken out by zeroing it in the sum
 __device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double (&acc)[4],
                                                     unsigned orow, unsigned r1) {
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
-        float s4 = 0.0f;
+        float p = 1.0f, a = 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (orow + i < r1) s4 += softplus2(c[wb][i]);
-        acc[wb] += (double)s4;
+        for (int i = 0; i < 4; ++i) {
+            if (orow + i < r1) {
+                p *= 1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(c[wb][i]));
+                a += __builtin_fabsf(c[wb][i]);
+            }
+        }
+        acc[wb] += (double)__builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p));
     }
 }
 
@@ -204,25 +218,27 @@ glm_logistic_fp32_kernel(const __bf16* __restrict__ X, unsigned long long N,
             reinterpret_cast<float*>(xs)[e] = row < r1 ? (float)X[row * D + e % D] : 0.0f;
         }
         __syncthreads();
-        const int rows = (int)((r1 - rb) < TR ? (r1 - rb) : TR);
-        float s4 = 0.0f;
-        for (int r = 0; r < rows; ++r) {
-            float u = 0.0f;
+        // rows past r1 are zero in the tile: each adds exactly log2(2) = 1,
+        // taken out below
+        const int pad = (int)((r1 - rb) < TR ? TR - (r1 - rb) : 0);
+        for (int r = 0; r < TR; r += 4) {
+            float u[4];
 #pragma unroll
-            for (int j4 = 0; j4 < D / 4; ++j4) {
-                const float4 x = xs[r * (D / 4) + j4];
-                u = __builtin_fmaf(x.x, y[4 * j4 + 0], u);
-                u = __builtin_fmaf(x.y, y[4 * j4 + 1], u);
-                u = __builtin_fmaf(x.z, y[4 * j4 + 2], u);
-                u = __builtin_fmaf(x.w, y[4 * j4 + 3], u);
+            for (int i = 0; i < 4; ++i) {
+                float t = 0.0f;
+#pragma unroll
+                for (int j4 = 0; j4 < D / 4; ++j4) {
+                    const float4 x = xs[(r + i) * (D / 4) + j4];
+                    t = __builtin_fmaf(x.x, y[4 * j4 + 0], t);
+                    t = __builtin_fmaf(x.y, y[4 * j4 + 1], t);
+                    t = __builtin_fmaf(x.z, y[4 * j4 + 2], t);
+                    t = __builtin_fmaf(x.w, y[4 * j4 + 3], t);
+                }
+                u[i] = t;
             }
-            s4 += softplus2(u);
-            if ((r & 3) == 3) {
-                acc += (double)s4;
-                s4 = 0.0f;
-            }
+            acc += (double)softplus2_rest4(u[0], u[1], u[2], u[3]);
         }
-        acc += (double)s4;
+        acc -= (double)pad;
     }
     partial[(size_t)range * n + k] = acc;
 }
@@ -267,8 +283,89 @@ glm_logistic_fp64_kernel(const __bf16* __restrict__ X, unsigned long long N,
     partial[(size_t)range * n + k] = acc;
 }
 
-// lin[j] = sum_i y_i x_ij in double: one block per chunk of rows, one
-// thread per feature; chunk partials summed in chunk order by lin_final.
+// fp64 softplus sums of listed points (the guarded accept test's fallback):
+// block r owns one row range and runs the listed points eight at a time,
+// their coefficients in LDS, one row per thread per pass with the same
+// arithmetic as the fp64 kernel; per-point sums are reduced over the block.
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
+                     const float* __restrict__ Y, const float* __restrict__ W,
+                     const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+                     unsigned lmax, unsigned long long rows_per_range,
+                     double* __restrict__ partial64) {
+    constexpr int PC = 8;
+    __shared__ double b[PC][D];
+    __shared__ double red[PC][4];
+    const unsigned n = *count;
+    const unsigned c = n < lmax ? n : lmax;
+    if (c == 0) return;
+    const unsigned np = 2 * c;
+    const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_range;
+    unsigned long long r1 = r0 + rows_per_range;
+    if (r1 > N) r1 = N;
+    for (unsigned p0 = 0; p0 < np; p0 += PC) {
+        __syncthreads();
+        for (unsigned e = t; e < PC * D; e += 256) {
+            const unsigned p = p0 + e / D;
+            double v = 0.0;
+            if (p < np) {
+                const unsigned w = list[p < c ? p : p - c];
+                v = (double)(p < c ? Y : W)[(size_t)w * D + e % D];
+            }
+            b[e / D][e % D] = v;
+        }
+        __syncthreads();
+        double acc[PC];
+#pragma unroll
+        for (int q = 0; q < PC; ++q) acc[q] = 0.0;
+        for (unsigned long long row = r0 + t; row < r1; row += 256) {
+            double x[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) x[j] = (double)(float)X[row * D + j];
+#pragma unroll
+            for (int q = 0; q < PC; ++q) {
+                double eta = 0.0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) eta = __builtin_fma(x[j], b[q][j], eta);
+                acc[q] += __builtin_fmax(eta, 0.0) + log1p(exp(-__builtin_fabs(eta)));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+            double v = acc[q];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[q][wave] = v;
+        }
+        __syncthreads();
+        if (t < PC && p0 + t < np)
+            partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
+                (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+    }
+}
+
+hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                                const float* Y, const float* W, const unsigned* list,
+                                const unsigned* count, unsigned lmax, unsigned nranges,
+                                double* partial64) {
+    const unsigned long long rpr = (N + nranges - 1) / nranges;
+    const __bf16* x = static_cast<const __bf16*>(X);
+    if (d == 64)
+        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
+                                                         partial64);
+    else if (d == 32)
+        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
+                                                         partial64);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// lin[j] = sum_i y_i x_ij (or sum_i x_ij when yv is null) in double: one
+// block per chunk of rows, one thread per feature; chunk partials summed in
+// chunk order by lin_final.
 __global__ void __launch_bounds__(256)
 glm_lin_partial_kernel(const __bf16* __restrict__ X, const float* __restrict__ yv,
                        unsigned long long N, unsigned d, unsigned long long rows_per_chunk,
@@ -279,7 +376,12 @@ glm_lin_partial_kernel(const __bf16* __restrict__ X, const float* __restrict__ y
     unsigned long long r1 = r0 + rows_per_chunk;
     if (r1 > N) r1 = N;
     double s = 0.0;
-    for (unsigned long long i = r0; i < r1; ++i) s = __builtin_fma((double)yv[i], (double)(float)X[i * d + j], s);
+    if (yv) {
+        for (unsigned long long i = r0; i < r1; ++i)
+            s = __builtin_fma((double)yv[i], (double)(float)X[i * d + j], s);
+    } else {
+        for (unsigned long long i = r0; i < r1; ++i) s += (double)(float)X[i * d + j];
+    }
     part[(size_t)blockIdx.x * d + j] = s;
 }
 

@@ -20,6 +20,14 @@ hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsi
                                unsigned d, const float* P, const void* ys, unsigned n,
                                unsigned nranges, unsigned long long rows_per_range, float yscale,
                                double* partial);
+// fp64 softplus sums of the points listed by the guarded accept test:
+// columns i < c are Y[list[i]], columns c + i are W[list[i]] (c =
+// min(*count, lmax), read on the device), partial64[r * 2 lmax + col] for
+// r < nranges.  Does nothing when the list is empty.
+hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                                const float* Y, const float* W, const unsigned* list,
+                                const unsigned* count, unsigned lmax, unsigned nranges,
+                                double* partial64);
 hipError_t launch_glm_lin(hipStream_t s, const void* X, const float* yv, unsigned long long N,
                           unsigned d, double* part, unsigned chunks, double* lin);
 

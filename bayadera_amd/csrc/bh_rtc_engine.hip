@@ -570,13 +570,15 @@ __device__ __forceinline__ void bh_glm_split(const float (&y)[DIM], float yscale
 }
 
 // Proposals of one half-step: Y (fp32), z and u2 per walker, and the bf16
-// split of log2(e)*Y for the MFMA kernels.
+// split of log2(e)*Y for the MFMA kernels; clears the guard list's count.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                const float* __restrict__ X, float a, unsigned step, unsigned K, int partner,
                float* __restrict__ Y, float* __restrict__ zbuf, float* __restrict__ u2buf,
-               __bf16* __restrict__ ysplit, float yscale, int nsplit) {
+               __bf16* __restrict__ ysplit, float yscale, int nsplit,
+               unsigned* __restrict__ guard_count) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    if (guard_count && k == 0) *guard_count = 0u;
     float xk[DIM], y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
@@ -635,23 +637,110 @@ bh_glm_logpost(const float* __restrict__ P, unsigned n, const double* __restrict
     lp[k] = bh_glm_logpost(y, k, n, partial, nranges, lin, scale, prior_prec);
 }
 
-// Accept test of one half-step (amd-gcn-mcmc-stretch.cl:66-78): q =
-// z^(DIM-1) exp(beta (lp(Y) - lp(X))), the difference taken in double; with
-// accu != 0 also the accounting of stretch_move_accu.
+// The accept decision of stretch_move (amd-gcn-mcmc-stretch.cl:66-71): q =
+// z^(DIM-1) exp(beta (lp(Y) - lp(X))), u2 <= q, the difference taken in
+// double and rounded to float as the kernels of every precision do.
+__device__ __forceinline__ bool bh_glm_decide(double lpy, double lpx, float z, float u2,
+                                              float beta) {
+    const float d = (float)(lpy - lpx);
+    const float q = __builtin_isfinite(lpy) ? bh_zpow(z) * expf(beta * d) : 0.0f;
+    return u2 <= q;
+}
+
+// Guarded accept test of the reduced-precision likelihoods.  The decision is
+// a monotone step in d = lp(Y) - lp(X); a walker whose log margin
+//   m = log u2 - (DIM-1) log z - beta d
+// is within `guard` of zero could be decided differently by an error of that
+// size in d, so it is listed (up to lmax walkers; the rest count as
+// overflow) for an fp64 evaluation of lp(Y) and lp(X) and decided on those.
+// flag[k]: 0 decided here, 3 listed (bh_glm_resolve turns it into 1 = reject
+// or 2 = accept).
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
+              const float* __restrict__ u2buf, const double* __restrict__ partial,
+              unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
+              const double* __restrict__ lp, float beta, unsigned K, double guard,
+              unsigned* __restrict__ flag, unsigned* __restrict__ list,
+              unsigned* __restrict__ count, unsigned lmax) {
+    const unsigned k = blockIdx.x * WGS + threadIdx.x;
+    float y[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
+    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
+    unsigned f = 0;
+    if (__builtin_isfinite(lpy)) {
+        const double m = log((double)u2buf[k]) - (double)(DIM - 1) * log((double)zbuf[k]) -
+                         (double)beta * (lpy - lp[k]);
+        if (__builtin_fabs(m) <= guard) {
+            const unsigned i = atomicAdd(count, 1u);
+            if (i < lmax) {
+                list[i] = k;
+                f = 3;
+            }
+        }
+    }
+    flag[k] = f;
+}
+
+// Decides the listed walkers on their fp64 log posteriors: partial64 holds,
+// per row range r, column i < c the softplus sum of Y[list[i]] and column
+// c + i that of X[list[i]] (natural units, row stride `stride`).  The
+// accepted walker's fp64 lp(Y) goes to glp.  totals[0] += listed walkers,
+// totals[1] += walkers past lmax.
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
+               const float* __restrict__ zbuf, const float* __restrict__ u2buf,
+               const double* __restrict__ partial64, unsigned nranges64, unsigned stride,
+               const double* __restrict__ lin64, double prior_prec, float beta,
+               const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+               unsigned lmax, unsigned* __restrict__ flag, double* __restrict__ glp,
+               unsigned long long* __restrict__ totals) {
+    const unsigned i = blockIdx.x * WGS + threadIdx.x;
+    const unsigned n = *count;
+    const unsigned c = n < lmax ? n : lmax;
+    if (i == 0) {
+        atomicAdd(&totals[0], (unsigned long long)c);
+        atomicAdd(&totals[1], (unsigned long long)(n - c));
+    }
+    if (i >= c) return;
+    const unsigned k = list[i];
+    float y[DIM], xk[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+        y[j] = Y[(size_t)k * DIM + j];
+        xk[j] = X[(size_t)k * DIM + j];
+    }
+    const double ly = bh_glm_logpost(y, i, stride, partial64, nranges64, lin64, 1.0, prior_prec);
+    const double lx = bh_glm_logpost(xk, c + i, stride, partial64, nranges64, lin64, 1.0,
+                                     prior_prec);
+    flag[k] = bh_glm_decide(ly, lx, zbuf[k], u2buf[k], beta) ? 2u : 1u;
+    glp[k] = ly;
+}
+
+// Accept test of one half-step (amd-gcn-mcmc-stretch.cl:66-78) with accu !=
+// 0 also the accounting of stretch_move_accu.  With flag non-null, walkers
+// resolved in fp64 take that decision and lp(Y).
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_accept(const float* __restrict__ Y, const float* __restrict__ zbuf,
               const float* __restrict__ u2buf, const double* __restrict__ partial,
               unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
               float* __restrict__ X, double* __restrict__ lp, float beta, unsigned K, int accu,
-              unsigned* __restrict__ accept, float* __restrict__ means, unsigned mstep) {
+              unsigned* __restrict__ accept, float* __restrict__ means, unsigned mstep,
+              const unsigned* __restrict__ flag, const double* __restrict__ glp) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float y[DIM], xk[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
-    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
-    const float d = (float)(lpy - lp[k]);
-    const float q = __builtin_isfinite(lpy) ? bh_zpow(zbuf[k]) * expf(beta * d) : 0.0f;
-    const bool accepted = u2buf[k] <= q;
+    const unsigned f = flag ? flag[k] : 0u;
+    double lpy;
+    bool accepted;
+    if (f == 1u || f == 2u) {
+        lpy = glp[k];
+        accepted = f == 2u;
+    } else {
+        lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
+        accepted = bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta);
+    }
     if (accepted) {
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {

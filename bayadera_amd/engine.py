@@ -429,6 +429,29 @@ class HipStretch(_Handle):
         call("bh_stretch_set_glm_precision", self._h, int(precision))
         return self
 
+    def set_glm_guard(self, guard: float):
+        """Guarded accept test of a reduced-precision GLM sampler: walkers
+        within ``guard`` (log units) of their accept threshold are decided
+        on fp64 log posteriors (0 turns it off)."""
+        call("bh_stretch_set_glm_guard", self._h, float(guard))
+        return self
+
+    def set_kernel_timing(self, on: bool = True):
+        """HIP events around the dominant kernel's launches (the first 1024
+        after this call)."""
+        call("bh_stretch_set_kernel_timing", self._h, int(bool(on)))
+        return self
+
+    def kernel_timing(self) -> dict:
+        avg, n = ctypes.c_double(), ctypes.c_uint32()
+        call("bh_stretch_kernel_timing", self._h, ctypes.byref(avg), ctypes.byref(n))
+        return {"avg_ms": avg.value, "launches": n.value}
+
+    def glm_guard_stats(self) -> dict:
+        listed, overflow = ctypes.c_uint64(), ctypes.c_uint64()
+        call("bh_stretch_glm_guard_stats", self._h, ctypes.byref(listed), ctypes.byref(overflow))
+        return {"listed": listed.value, "overflow": overflow.value}
+
     def set_timing(self, on: bool = True):
         call("bh_stretch_set_timing", self._h, int(bool(on)))
         return self

@@ -1,20 +1,38 @@
-"""Benchmark of the stretch-sampler path on one MI355X.
+"""Benchmark of the MI355X Bayadera engine on one GPU.
 
-Workload (one "step" = one whole job, inputs resident in HBM):
-  4-D correlated Gaussian target, mu ~ N(0,1)^4, Sigma = A A' + I with A 4x4
-  i.i.d. N(0,1) (numpy seed 1001); 1024 walkers initialised U(-5,5);
-  burn-in 512 steps + run-sampler 4096 steps at a=2; then sample! 4096 x
-  1024 draws and the dataset engine's mean, sd and 256-bin histogram of
-  them.  ESS_d = Var_d(draws) / sigma_d^2 with sigma from the engine's acor
-  of the run-sampler means; value = min_d ESS_d per second of the whole job.
+Default workload (the metric's comparison, matrix cores vs plain fp32):
+  Bayesian logistic regression, precision form.  X is 1,000,000 x 64 with
+  rows N(0,1)/8, stored bf16 (numpy seed 1003); beta* ~ N(0,1); y ~
+  Bernoulli(sigmoid(X beta*)); prior beta ~ N(0, 5^2).  16384 walkers (8192
+  per half), whole-walker partners, a = 2, 500 burn-in + 2000 run-sampler
+  steps; walkers start uniform in a +-4 sd box around the posterior mode
+  (Newton steps on the host, not timed).  Three runs from the same seeds:
+    bf16x3  likelihood contraction on the matrix cores (split-bf16x3 terms,
+            fp32 accumulate, fp64 row sums) with the guarded accept test,
+    fp32    the plain fp32 VALU likelihood kernel,
+    fp64    the fp64 likelihood kernel (the oracle path on the GPU).
+  ESS_d = Var_d(final ensemble) / sigma_d^2 with sigma_d from the engine's
+  acor of the run-sampler means; value = min_d ESS_d per second of the
+  bf16x3 job (burn-in + run, inputs resident in HBM).  Posterior-mean error
+  = |mean_d - mean_fp64,d| / sigma_fp64,d on these coupled runs.
+  Roofline: the MFMA likelihood kernel, 2 N d K = 1.05e12 algorithmic flops
+  per launch, its launches timed with HIP events on the sampler's stream
+  during the run phase.
 
-Roofline: the plain stretch move (move-bare! kernel) on the same model at
-2^24 walkers (256 MiB of positions, above the 256 MB Infinity Cache), timed
-with HIP events on the sampler's stream; algorithmic bytes per walker-step
-8*DIM + 4 + alpha*(4*DIM + 4).
+--workload c1: the reference's CPU-runnable case.  4-D correlated Gaussian
+  target, mu ~ N(0,1)^4, Sigma = A A' + I with A 4x4 i.i.d. N(0,1) (numpy
+  seed 1001); 1024 walkers initialised U(-5,5); burn-in 512 + run-sampler
+  4096 steps at a=2; then sample! 4096 x 1024 draws and the dataset engine's
+  mean, sd and 256-bin histogram of them.  One "step" is one whole job.
 
-CPU baseline: the same job on the host cores through the CPU restatement of
-the reference kernels (oracle/), OpenMP over walkers.
+Plain stretch move roofline (both workloads): the move-bare! kernel of the
+correlated Gaussian model at 2^24 walkers and DIM 4 and 16 (256 MiB and
+1 GiB of positions, above the 256 MB Infinity Cache), its launches timed
+with HIP events; algorithmic bytes per walker-step 8*DIM + 4 +
+alpha*(4*DIM + 4).
+
+CPU baseline: the CPU restatement of the reference kernels (oracle/) on
+the host cores, OpenMP over walkers, on a bounded sample of the workload.
 """
 from __future__ import annotations
 
@@ -31,32 +49,101 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("ESS/sec & walker-steps/sec, MFMA vs plain FP32 at posterior-mean err "
           "≤0.1 MC-SE")
+A = 2.0
+HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
+
+# correlated Gaussian workload
 DIM = 4
 WALKERS = 1024
 BURN_IN = 512
 RUN_STEPS = 4096
 SAMPLE_COLS = 4096 * 1024
-A = 2.0
-HBM_PEAK_GBS = 8000.0
 ROOF_WALKERS = 1 << 24
 ROOF_STEPS = 20
 
+# logistic-regression workload
+C3_N, C3_D, C3_W = 1_000_000, 64, 16384
+C3_BURN, C3_RUN = 500, 2000
+C3_PRIOR_SD = 5.0
+# guard of the bf16x3 accept test, in log units: walkers whose accept
+# margin is smaller are decided on fp64 log posteriors
+C3_GUARD = 2e-3
 
-def c1_problem():
-    rng = np.random.default_rng(1001)
-    mu = rng.standard_normal(DIM)
-    A_ = rng.standard_normal((DIM, DIM))
-    sigma = A_ @ A_.T + np.eye(DIM)
-    P = np.linalg.inv(sigma)
-    params = np.concatenate([mu, P.reshape(-1)]).astype(np.float32)
-    limits = np.tile(np.array([-5.0, 5.0], np.float32), DIM)
-    return mu, sigma, params, limits
+
+def _threads():
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS"):
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    return threads
 
 
 def ess_from(draws_var, sigma):
     return draws_var.astype(np.float64) / np.square(sigma.astype(np.float64))
 
 
+def mvn_problem(dim, seed):
+    rng = np.random.default_rng(seed)
+    mu = rng.standard_normal(dim)
+    A_ = rng.standard_normal((dim, dim))
+    sigma = A_ @ A_.T + np.eye(dim)
+    P = np.linalg.inv(sigma)
+    params = np.concatenate([mu, P.reshape(-1)]).astype(np.float32)
+    limits = np.tile(np.array([-5.0, 5.0], np.float32), dim)
+    return mu, sigma, params, limits
+
+
+def c1_problem():
+    return mvn_problem(DIM, 1001)
+
+
+# ---------------------------------------------------------------------------
+# plain stretch move roofline
+# ---------------------------------------------------------------------------
+def traffic_of(name):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    calibrated against a copy of known size), committed under profiles/."""
+    tpath = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def move_roofline(fac, dim):
+    """HBM roofline of the move-bare! kernel at 2^24 walkers of a dim-D
+    correlated Gaussian."""
+    from bayadera_amd import _lib, models
+    _, _, params, limits = mvn_problem(dim, 1001 + dim)
+    sf = fac.mcmc_factory(models.mvn_model(dim))
+    s = sf.create_sampler(77, ROOF_WALKERS, params)
+    s.set_exec(_lib.BH_EXEC_STEPWISE).set_partner(_lib.BH_PARTNER_WALKER)
+    s.init(77)
+    s.init_position(78, limits)
+    s.burn_in(4, A)  # warm-up
+    s.set_kernel_timing(True)
+    s.burn_in(ROOF_STEPS, A)
+    kt = s.kernel_timing()
+    s.set_kernel_timing(False)
+    alpha = s.acc_rate(A)
+    avg_ms = kt["avg_ms"]
+    walkers_per_launch = ROOF_WALKERS // 2
+    bytes_per_walker = 8 * dim + 4 + alpha * (4 * dim + 4)
+    achieved = walkers_per_launch * bytes_per_walker / (avg_ms * 1e-3) / 1e9
+    s.release()
+    sf.release()
+    return {"kernel": f"bh_stretch_move_bare (DIM={dim}, K=2^23 walkers per launch)",
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_of(f"stretch_move_bare_dim{dim}_2p24_traffic.json"),
+            "avg_kernel_ms": round(avg_ms, 4), "launches_timed": kt["launches"],
+            "alpha": round(alpha, 4),
+            "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker)}
+
+
+# ---------------------------------------------------------------------------
+# correlated Gaussian job (the reference's CPU-runnable case)
+# ---------------------------------------------------------------------------
 def gpu_job(fac, sampler, sample_out, limits, seed):
     """One whole job on the GPU; returns per-phase timings and results."""
     import torch
@@ -79,45 +166,10 @@ def gpu_job(fac, sampler, sample_out, limits, seed):
             "var": var, "hist": hist}
 
 
-def roofline(fac, model, params):
-    """HBM roofline of the move-bare! kernel at 2^24 walkers."""
-    from bayadera_amd import _lib
-    sf = fac.mcmc_factory(model)
-    s = sf.create_sampler(77, ROOF_WALKERS, params)
-    s.set_exec(_lib.BH_EXEC_STEPWISE).set_partner(_lib.BH_PARTNER_WALKER)
-    s.init(77)
-    s.init_position(78, np.tile(np.array([-5.0, 5.0], np.float32), DIM))
-    s.burn_in(4, A)  # warm-up
-    s.set_timing(True)
-    s.burn_in(ROOF_STEPS, A)
-    ms = s.last_gpu_ms()
-    s.set_timing(False)
-    alpha = s.acc_rate(A)
-    launches = 2 * ROOF_STEPS
-    avg_ms = ms / launches
-    walkers_per_launch = ROOF_WALKERS // 2
-    bytes_per_walker = 8 * DIM + 4 + alpha * (4 * DIM + 4)
-    achieved = walkers_per_launch * bytes_per_walker / (avg_ms * 1e-3) / 1e9
-    s.release()
-    sf.release()
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "stretch_move_bare_2p24_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    return {"kernel": "bh_stretch_move_bare (DIM=4, K=2^23 walkers per launch)",
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "avg_kernel_ms": round(avg_ms, 4), "alpha": round(alpha, 4),
-            "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker)}
-
-
 def cpu_baseline(params, limits, budget_s=25.0):
     """The whole job on the host through the oracle (OpenMP over walkers)."""
     import oracle as O
-    threads = len(os.sched_getaffinity(0))
-    if os.environ.get("OMP_NUM_THREADS"):
-        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    threads = _threads()
     runs = []
     t_start = time.perf_counter()
     while not runs or (time.perf_counter() - t_start < budget_s / 2 and len(runs) < 3):
@@ -144,201 +196,7 @@ def cpu_baseline(params, limits, budget_s=25.0):
                                         / t, 1)}
 
 
-# ---------------------------------------------------------------------------
-# Logistic regression, precision form: X 1,000,000 x 64 (rows N(0,1)/8,
-# stored bf16), beta* ~ N(0,1), y ~ Bernoulli(sigmoid(X beta*)), prior
-# N(0, 5^2), 16384 walkers.  The likelihood of each half-ensemble is one
-# 1M x 64 . 64 x 8192 contraction with a softplus epilogue.
-# ---------------------------------------------------------------------------
-C3_N, C3_D, C3_W = 1_000_000, 64, 16384
-C3_BURN, C3_RUN = 500, 2000
-C3_PRIOR_SD = 5.0
-MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
-
-
-def c3_problem(N=C3_N, d=C3_D, seed=1003):
-    from bayadera_amd import models
-    rng = np.random.default_rng(seed)
-    X = (rng.standard_normal((N, d), dtype=np.float32) / np.float32(np.sqrt(d)))
-    Xb = models.to_bf16(X)
-    Xd = models.from_bf16(Xb).reshape(N, d).astype(np.float64)
-    beta_true = rng.standard_normal(d)
-    eta = Xd @ beta_true
-    y = (rng.random(N) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float32)
-    # a Laplace box for init-position!: Newton steps to the posterior mode,
-    # walkers uniform within 4 sd of it (setup, not timed)
-    b = np.zeros(d)
-    for _ in range(8):
-        p = 1.0 / (1.0 + np.exp(-(Xd @ b)))
-        g = Xd.T @ (y - p) - b / C3_PRIOR_SD ** 2
-        H = (Xd * (p * (1 - p))[:, None]).T @ Xd + np.eye(d) / C3_PRIOR_SD ** 2
-        b = b + np.linalg.solve(H, g)
-    sd = np.sqrt(np.diag(np.linalg.inv(H)))
-    limits = np.stack([b - 4 * sd, b + 4 * sd], 1).astype(np.float32).reshape(-1)
-    return Xb, Xd, y, b, sd, limits
-
-
-def c3_run(sf, prec, limits, burn, run, seed=123):
-    from bayadera_amd import _lib
-    s = sf.create_sampler(seed, C3_W, np.zeros(1, np.float32))
-    s.set_glm_precision(prec).set_partner(_lib.BH_PARTNER_WALKER)
-    s.init(seed)
-    s.init_position(seed + 1, limits)
-    t0 = time.perf_counter()
-    s.burn_in(burn, A)
-    s.set_timing(True)
-    res = s.run_sampler(run, A)
-    run_gpu_ms = s.last_gpu_ms()
-    s.set_timing(False)
-    var = s.variance()
-    t1 = time.perf_counter()
-    s.release()
-    ac = res["autocorrelation"]
-    ess = ess_from(var, ac.sigma)
-    return {"wall_s": t1 - t0, "run_gpu_ms": run_gpu_ms, "ess": ess, "means": res["means"],
-            "grand_mean": res["means"].astype(np.float64).mean(0), "sigma": ac.sigma,
-            "tau": ac.tau, "acc": res["acceptance-rate"]}
-
-
-def c3_kernel_roofline(sf, limits, prec):
-    """Average duration of the likelihood contraction for one half-ensemble
-    (8192 points), from the sampler's stream events over single move-bare!
-    calls minus the walker-side kernels measured the same way at N/16."""
-    from bayadera_amd import _lib
-    s = sf.create_sampler(321, C3_W, np.zeros(1, np.float32))
-    s.set_glm_precision(prec).set_partner(_lib.BH_PARTNER_WALKER)
-    s.init_position(322, limits)
-    s.burn_in(2, A)
-    s.set_timing(True)
-    ms = []
-    for _ in range(10):
-        s.move_bare()
-        ms.append(s.last_gpu_ms())
-    s.set_timing(False)
-    s.release()
-    return float(np.median(ms)) / 2.0  # per half-step (propose + contraction + accept)
-
-
-def c3_cpu_baseline(Xd, y, limits, budget_s=20.0):
-    """Walker-steps/s of the oracle's fp64 GLM sampler on a bounded sample:
-    walkers x one half-step over the full 1M x 64 data, OpenMP over walkers."""
-    import oracle as O
-    threads = len(os.sched_getaffinity(0))
-    if os.environ.get("OMP_NUM_THREADS"):
-        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
-    Xf = np.ascontiguousarray(Xd, np.float32)
-    W = 1024  # the sampler's smallest ensemble; half of it moves per half-step
-    g = O.GlmStretch(Xf, y, C3_PRIOR_SD, W, 5, partner=O.PARTNER_WALKER, threads=threads)
-    # positions only (skip the initial log posteriors of all walkers)
-    g.xs[:] = O.init_walkers(6, limits.reshape(-1, 2), C3_D, W)
-    t0 = time.perf_counter()
-    n_half = 0
-    while n_half == 0 or (time.perf_counter() - t0 < budget_s and n_half < 4):
-        g._half(5, 3333, n_half, True, A, 1.0)
-        n_half += 1
-    t = time.perf_counter() - t0
-    return {"walker_steps_per_s": (W // 2) * n_half / 2 / t, "threads": threads,
-            "sample": f"{n_half} half-step(s) of {W // 2} walkers over the full "
-                      f"{C3_N}x{C3_D} data in {t:.1f} s"}
-
-
-def main_c3(args):
-    import torch
-    from bayadera_amd import _lib, engine, models
-    t_setup = time.perf_counter()
-    Xb, Xd, y, b_map, sd_map, limits = c3_problem()
-    t_setup = time.perf_counter() - t_setup
-    fac = engine.HipBayaderaFactory(0, 256)
-    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=C3_PRIOR_SD))
-    burn, run = args.c3_burn, args.c3_run
-    # warm-up (compiles nothing further; touches every kernel once)
-    c3_run(sf, _lib.BH_GLM_BF16X3, limits, 2, 64)
-    torch.cuda.synchronize()
-    res = {}
-    for name, prec in (("bf16x3", _lib.BH_GLM_BF16X3), ("fp32", _lib.BH_GLM_FP32),
-                       ("bf16", _lib.BH_GLM_BF16)):
-        res[name] = c3_run(sf, prec, limits, burn, run)
-    ref = c3_run(sf, _lib.BH_GLM_FP64, limits, burn, run) if not args.no_fp64 else None
-    out = {}
-    for name, r in res.items():
-        ess = float(r["ess"].min())
-        o = {"ess_per_s": round(ess / r["wall_s"], 2), "ess_min": round(ess, 1),
-             "walker_steps_per_s": round(C3_W * (burn + run) / r["wall_s"], 1),
-             "wall_s": round(r["wall_s"], 3), "acceptance_rate": round(r["acc"], 4),
-             "tau_max": round(float(r["tau"].max()), 2),
-             "ms_per_step_run": round(r["run_gpu_ms"] / run, 3)}
-        if ref is not None:
-            err = np.abs(r["grand_mean"] - ref["grand_mean"]) / ref["sigma"].astype(np.float64)
-            o["mean_err_over_mcse_max"] = round(float(err.max()), 4)
-        out[name] = o
-    half_ms = c3_kernel_roofline(sf, limits, _lib.BH_GLM_BF16X3)
-    flops = 2.0 * C3_N * C3_D * (C3_W // 2)
-    achieved = flops / (half_ms * 1e-3) / 1e12
-    line = {
-        "metric": METRIC, "value": out["bf16x3"]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
-        "steps": burn + run, "warmup": 64, "ms_per_step": round(res["bf16x3"]["wall_s"] /
-                                                                 (burn + run) * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
-        "data": "synthetic (seeded logistic-regression data, X stored bf16)",
-        "config": {"workload": f"logistic regression {C3_N}x{C3_D}, {C3_W} walkers, "
-                               f"{burn} burn-in + {run} run-sampler steps (a=2), prior N(0,5^2)",
-                   "likelihood": "MFMA split-bf16x3, fp32 accumulate, fp64 row sums",
-                   "partner": "walker"},
-        "mfma_vs_fp32_ess_per_s": round(out["bf16x3"]["ess_per_s"] / out["fp32"]["ess_per_s"], 2),
-        "precisions": out,
-        "roofline": {"kernel": "half-step (propose + glm_logistic_mfma<64,3> + accept), 8192 "
-                               "points", "bound": "mfma", "achieved": round(achieved, 1),
-                     "peak": MFMA_BF16_PEAK, "unit": "TFLOP/s",
-                     "frac": round(achieved / MFMA_BF16_PEAK, 4), "traffic": None,
-                     "avg_half_step_ms": round(half_ms, 4),
-                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": 3},
-        "setup_s": round(t_setup, 1),
-    }
-    if not args.no_cpu_baseline:
-        cb = c3_cpu_baseline(Xd, y, limits)
-        ess_per_ws = out["bf16x3"]["ess_min"] / (C3_W * (burn + run))
-        line["cpu_baseline"] = {
-            "value": round(cb["walker_steps_per_s"] * ess_per_ws, 4), "unit": "ESS/s",
-            "cores": cb["threads"], "kind": "port",
-            "sample": cb["sample"] + "; ESS per walker-step taken from the GPU run",
-            "walker_steps_per_s": round(cb["walker_steps_per_s"], 2)}
-    print(json.dumps(line), flush=True)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c1", "c3"], default="c1")
-    ap.add_argument("--c3-burn", type=int, default=C3_BURN)
-    ap.add_argument("--c3-run", type=int, default=C3_RUN)
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--roofline-only", action="store_true",
-                    help="only the move-bare! roofline run, plus a 256 MiB device copy whose "
-                         "FETCH_SIZE/WRITE_SIZE calibrate the counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding across GPUs is designed in DESIGN.md, not built)", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "c3":
-        return main_c3(args)
-    if args.roofline_only:
-        import torch
-        from bayadera_amd import engine, models
-        _, _, params, _ = c1_problem()
-        fac = engine.HipBayaderaFactory(0, 256)
-        r = roofline(fac, models.mvn_model(DIM), params)
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"roofline": r, "calibration_copy_bytes": src.numel() * 4}), flush=True)
-        return
-
+def main_c1(args):
     import torch
 
     from bayadera_amd import _lib, engine, models
@@ -385,10 +243,217 @@ def main():
                      for k in ("t_burn", "t_run", "t_sample")},
     }
     if not args.no_roofline:
-        line["roofline"] = roofline(fac, model, params)
+        line["roofline"] = move_roofline(fac, DIM)
+        line["plain_move_roofline_dim16"] = move_roofline(fac, 16)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(params, limits)
     print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# logistic regression (default)
+# ---------------------------------------------------------------------------
+def c3_problem(N=C3_N, d=C3_D, seed=1003):
+    from bayadera_amd import models
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((N, d), dtype=np.float32) / np.float32(np.sqrt(d)))
+    Xb = models.to_bf16(X)
+    Xd = models.from_bf16(Xb).reshape(N, d).astype(np.float64)
+    beta_true = rng.standard_normal(d)
+    eta = Xd @ beta_true
+    y = (rng.random(N) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float32)
+    # a Laplace box for init-position!: Newton steps to the posterior mode,
+    # walkers uniform within 4 sd of it (setup, not timed)
+    b = np.zeros(d)
+    for _ in range(8):
+        p = 1.0 / (1.0 + np.exp(-(Xd @ b)))
+        g = Xd.T @ (y - p) - b / C3_PRIOR_SD ** 2
+        H = (Xd * (p * (1 - p))[:, None]).T @ Xd + np.eye(d) / C3_PRIOR_SD ** 2
+        b = b + np.linalg.solve(H, g)
+    sd = np.sqrt(np.diag(np.linalg.inv(H)))
+    limits = np.stack([b - 4 * sd, b + 4 * sd], 1).astype(np.float32).reshape(-1)
+    return Xb, Xd, y, b, sd, limits
+
+
+def c3_run(sf, prec, limits, burn, run, guard=0.0, seed=123):
+    """One job: init-position!, burn-in!, run-sampler!, variance; wall time
+    from burn-in! to the variance read back."""
+    from bayadera_amd import _lib
+    s = sf.create_sampler(seed, C3_W, np.zeros(1, np.float32))
+    s.set_glm_precision(prec).set_partner(_lib.BH_PARTNER_WALKER)
+    if guard:
+        s.set_glm_guard(guard)
+    s.init(seed)
+    s.init_position(seed + 1, limits)
+    t0 = time.perf_counter()
+    s.burn_in(burn, A)
+    s.set_timing(True)
+    s.set_kernel_timing(True)
+    res = s.run_sampler(run, A)
+    run_gpu_ms = s.last_gpu_ms()
+    s.set_timing(False)
+    var = s.variance()
+    t1 = time.perf_counter()
+    kt = s.kernel_timing()
+    xs = s.state()[0].copy()
+    gs = s.glm_guard_stats()
+    s.release()
+    ac = res["autocorrelation"]
+    ess = ess_from(var, ac.sigma)
+    return {"wall_s": t1 - t0, "run_gpu_ms": run_gpu_ms, "ess": ess, "means": res["means"],
+            "grand_mean": res["means"].astype(np.float64).mean(0), "sigma": ac.sigma,
+            "sd": np.sqrt(var.astype(np.float64)), "tau": ac.tau,
+            "acc": res["acceptance-rate"], "kernel": kt, "xs": xs, "guard": gs}
+
+
+def c3_cpu_baseline(Xd, y, limits, budget_s=20.0):
+    """Walker-steps/s of the oracle's fp64 GLM sampler on a bounded sample:
+    walkers x one half-step over the full 1M x 64 data, OpenMP over walkers."""
+    import oracle as O
+    threads = _threads()
+    Xf = np.ascontiguousarray(Xd, np.float32)
+    W = 1024  # the sampler's smallest ensemble; half of it moves per half-step
+    g = O.GlmStretch(Xf, y, C3_PRIOR_SD, W, 5, partner=O.PARTNER_WALKER, threads=threads)
+    # positions only (skip the initial log posteriors of all walkers)
+    g.xs[:] = O.init_walkers(6, limits.reshape(-1, 2), C3_D, W)
+    t0 = time.perf_counter()
+    n_half = 0
+    while n_half == 0 or (time.perf_counter() - t0 < budget_s and n_half < 4):
+        g._half(5, 3333, n_half, True, A, 1.0)
+        n_half += 1
+    t = time.perf_counter() - t0
+    return {"walker_steps_per_s": (W // 2) * n_half / 2 / t, "threads": threads,
+            "sample": f"{n_half} half-step(s) of {W // 2} walkers over the full "
+                      f"{C3_N}x{C3_D} data in {t:.1f} s"}
+
+
+def main_c3(args):
+    import torch
+    from bayadera_amd import _lib, engine, models
+    t_setup = time.perf_counter()
+    Xb, Xd, y, b_map, sd_map, limits = c3_problem()
+    t_setup = time.perf_counter() - t_setup
+    fac = engine.HipBayaderaFactory(0, 256)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=C3_PRIOR_SD))
+    burn, run = args.c3_burn, args.c3_run
+    # warm-up: every kernel of the guarded and plain paths once
+    c3_run(sf, _lib.BH_GLM_BF16X3, limits, 2, 64, guard=args.c3_guard)
+    c3_run(sf, _lib.BH_GLM_FP32, limits, 1, 10)
+    torch.cuda.synchronize()
+    legs = [("bf16x3", _lib.BH_GLM_BF16X3, args.c3_guard), ("fp32", _lib.BH_GLM_FP32, 0.0)]
+    if args.c3_bf16:
+        legs.append(("bf16", _lib.BH_GLM_BF16, 0.0))
+    res = {}
+    for name, prec, g in legs:
+        print(f"bench: {name} leg", file=sys.stderr, flush=True)
+        res[name] = c3_run(sf, prec, limits, burn, run, guard=g)
+    ref = None
+    if not args.no_fp64:
+        print("bench: fp64 leg", file=sys.stderr, flush=True)
+        ref = c3_run(sf, _lib.BH_GLM_FP64, limits, burn, run)
+    out = {}
+    for name, prec, g in legs:
+        r = res[name]
+        ess = float(r["ess"].min())
+        o = {"ess_per_s": round(ess / r["wall_s"], 2),
+             "ess_per_s_run_phase": round(ess / (r["run_gpu_ms"] * 1e-3), 2),
+             "ess_min": round(ess, 1),
+             "walker_steps_per_s": round(C3_W * (burn + run) / r["wall_s"], 1),
+             "wall_s": round(r["wall_s"], 3), "acceptance_rate": round(r["acc"], 4),
+             "tau_max": round(float(r["tau"].max()), 2),
+             "ms_per_step_run": round(r["run_gpu_ms"] / run, 3),
+             "likelihood_kernel_ms": round(r["kernel"]["avg_ms"], 4)}
+        if g:
+            o["guard"] = g
+            o["guard_listed_per_half_step"] = round(r["guard"]["listed"] / (2 * (burn + run)), 3)
+            o["guard_overflow"] = r["guard"]["overflow"]
+        if ref is not None:
+            err = np.abs(r["grand_mean"] - ref["grand_mean"]) / ref["sigma"].astype(np.float64)
+            o["mean_err_over_mcse_max"] = round(float(err.max()), 4)
+            o["sd_rel_err_max"] = round(float(np.max(np.abs(r["sd"] / ref["sd"] - 1.0))), 6)
+            o["trajectory_equals_fp64"] = bool(np.array_equal(r["xs"], ref["xs"]))
+        out[name] = o
+    if ref is not None:
+        ess = float(ref["ess"].min())
+        out["fp64"] = {"ess_per_s": round(ess / ref["wall_s"], 2), "ess_min": round(ess, 1),
+                       "wall_s": round(ref["wall_s"], 3),
+                       "ms_per_step_run": round(ref["run_gpu_ms"] / run, 3),
+                       "acceptance_rate": round(ref["acc"], 4),
+                       "likelihood_kernel_ms": round(ref["kernel"]["avg_ms"], 4)}
+    kt = res["bf16x3"]["kernel"]
+    flops = 2.0 * C3_N * C3_D * (C3_W // 2)
+    achieved = flops / (kt["avg_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": METRIC, "value": out["bf16x3"]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
+        "steps": burn + run, "warmup": 66,
+        "ms_per_step": round(res["bf16x3"]["wall_s"] / (burn + run) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "data": "synthetic (seeded logistic-regression data, X stored bf16)",
+        "config": {"workload": f"logistic regression {C3_N}x{C3_D}, {C3_W} walkers, "
+                               f"{burn} burn-in + {run} run-sampler steps (a=2), prior N(0,5^2)",
+                   "likelihood": "MFMA split-bf16x3, fp32 accumulate, fp64 row sums, "
+                                 f"guarded accept test ({args.c3_guard})",
+                   "partner": "walker"},
+        "mfma_vs_fp32_ess_per_s": round(out["bf16x3"]["ess_per_s"] / out["fp32"]["ess_per_s"], 2),
+        "mfma_vs_fp32_walker_steps_per_s": round(out["bf16x3"]["walker_steps_per_s"]
+                                                 / out["fp32"]["walker_steps_per_s"], 2),
+        "precisions": out,
+        "roofline": {"kernel": "glm_logistic_mfma_kernel<64,3> (1M rows x 8192 points)",
+                     "bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK,
+                     "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK, 4),
+                     "traffic": traffic_of("glm_logistic_mfma_1Mx64_traffic.json"),
+                     "avg_kernel_ms": round(kt["avg_ms"], 4), "launches_timed": kt["launches"],
+                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": 3},
+        "setup_s": round(t_setup, 1),
+    }
+    if not args.no_roofline:
+        line["plain_move_roofline"] = [move_roofline(fac, 4), move_roofline(fac, 16)]
+    if not args.no_cpu_baseline:
+        cb = c3_cpu_baseline(Xd, y, limits)
+        ess_per_ws = out["bf16x3"]["ess_min"] / (C3_W * (burn + run))
+        line["cpu_baseline"] = {
+            "value": round(cb["walker_steps_per_s"] * ess_per_ws, 4), "unit": "ESS/s",
+            "cores": cb["threads"], "kind": "port",
+            "sample": cb["sample"] + "; ESS per walker-step taken from the GPU run",
+            "walker_steps_per_s": round(cb["walker_steps_per_s"], 2)}
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="c1: timed jobs")
+    ap.add_argument("--warmup", type=int, default=1, help="c1: untimed jobs")
+    ap.add_argument("--workload", choices=["c1", "c3"], default="c3")
+    ap.add_argument("--c3-burn", type=int, default=C3_BURN)
+    ap.add_argument("--c3-run", type=int, default=C3_RUN)
+    ap.add_argument("--c3-guard", type=float, default=C3_GUARD)
+    ap.add_argument("--c3-bf16", action="store_true", help="also the single-bf16 likelihood")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the move-bare! roofline runs (DIM 4 and 16), plus a 256 MiB "
+                         "device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding across GPUs is designed in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+    if args.roofline_only:
+        import torch
+        from bayadera_amd import engine
+        fac = engine.HipBayaderaFactory(0, 256)
+        r = [move_roofline(fac, 4), move_roofline(fac, 16)]
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline": r, "calibration_copy_bytes": src.numel() * 4}), flush=True)
+        return
+    if args.workload == "c3":
+        return main_c3(args)
+    return main_c1(args)
 
 
 if __name__ == "__main__":

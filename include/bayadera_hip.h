@@ -186,6 +186,18 @@ int bh_stretch_set_partner(bh_stretch* s, int partner_mode);
 int bh_stretch_set_exec(bh_stretch* s, int exec_mode);
 /* GLM samplers: precision of the likelihood contraction (BH_GLM_*). */
 int bh_stretch_set_glm_precision(bh_stretch* s, int precision);
+/* GLM samplers, reduced precisions: guarded accept test (no reference
+ * counterpart; the test it guards is amd-gcn-mcmc-stretch.cl:66-71).  A
+ * walker whose log margin log u2 - (DIM-1) log z - beta (lp(Y) - lp(X)) is
+ * within guard * max(1, beta) of zero is decided on fp64 log posteriors of
+ * Y and X instead; while the reduced-precision error of lp(Y) - lp(X) stays
+ * below the guard, every decision equals the fp64 sampler's.  guard = 0
+ * (the default) turns it off; at most 1024 walkers per half-step are
+ * re-evaluated, the rest count as overflow. */
+int bh_stretch_set_glm_guard(bh_stretch* s, double guard);
+/* Walkers re-evaluated in fp64 and walkers past the limit, summed over all
+ * guarded half-steps of the sampler. */
+int bh_stretch_glm_guard_stats(bh_stretch* s, uint64_t* listed, uint64_t* overflow);
 /* Info.info: {walker-count, iteration-counter} (amd_gcn.clj:321-324) */
 int bh_stretch_info(const bh_stretch* s, uint32_t* walkers, uint64_t* iteration_counter);
 
@@ -229,6 +241,14 @@ int bh_stretch_variance(bh_stretch* s, float* out);
  * counterpart: the reference has no timers (amd_gcn.clj:321-324). */
 int bh_stretch_set_timing(bh_stretch* s, int on);
 int bh_stretch_last_gpu_ms(const bh_stretch* s, double* ms);
+/* Kernel timing (no reference counterpart): with it on, the launches of the
+ * sampler's dominant kernel (the stretch-move kernels, the resident kernel,
+ * or a GLM sampler's likelihood contraction) are bracketed by HIP events on
+ * the sampler's stream, the first 1024 after the call that turned it on;
+ * bh_stretch_kernel_timing waits for the stream and returns their average
+ * duration and count.  Turning it on again restarts the count. */
+int bh_stretch_set_kernel_timing(bh_stretch* s, int on);
+int bh_stretch_kernel_timing(bh_stretch* s, double* avg_ms, uint32_t* launches);
 
 /* ---------------------------------------------------------------------- */
 /* Batched posteriors (no reference counterpart: the reference runs one      */

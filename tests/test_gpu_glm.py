@@ -93,3 +93,71 @@ def test_glm_walker_rule(fac):
     sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y))
     with pytest.raises(_lib.ArgumentError, match="multiple of 512"):
         sf.create_sampler(1, 768, np.zeros(1, np.float32))
+
+
+def _coupled_run(sf, prec, guard, d, W=2048, burn=25, run=25):
+    s = sf.create_sampler(21, W, np.zeros(1, np.float32)).set_glm_precision(prec)
+    s.set_partner(_lib.BH_PARTNER_WALKER)
+    if guard:
+        s.set_glm_guard(guard)
+    s.init(21)
+    s.init_position(22, np.tile(np.array([-0.3, 0.3], np.float32), d))
+    s.burn_in(burn, 2.0)
+    res = s.run_sampler(run, 2.0)
+    xs = s.state()[0].copy()
+    stats = s.glm_guard_stats()
+    s.release()
+    return xs, res, stats
+
+
+@pytest.mark.parametrize("prec", [_lib.BH_GLM_BF16X3, _lib.BH_GLM_FP32])
+def test_glm_guard_makes_the_fp64_decisions(fac, prec):
+    """Same seeds, same initial walkers: with the guarded accept test the
+    reduced-precision sampler's trajectory is the fp64 sampler's, position
+    for position, and its accounting matches."""
+    N, d = 65536, 64
+    Xb, Xd, y = _problem(N, d, 17)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
+    x64, r64, s64 = _coupled_run(sf, _lib.BH_GLM_FP64, 0.0, d)
+    assert s64 == {"listed": 0, "overflow": 0}
+    xg, rg, sg = _coupled_run(sf, prec, 2e-3, d)
+    print("guard stats", sg)
+    assert sg["overflow"] == 0
+    assert np.array_equal(xg, x64)
+    assert rg["acceptance-rate"] == r64["acceptance-rate"]
+    assert np.array_equal(rg["means"], r64["means"])
+
+
+def test_glm_guard_lists_near_threshold_walkers(fac):
+    """A guard wide enough to list every walker whose margin is small still
+    leaves the trajectory on the fp64 one, and the count is visible."""
+    N, d = 4096, 32
+    Xb, Xd, y = _problem(N, d, 19)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
+    x64, r64, _ = _coupled_run(sf, _lib.BH_GLM_FP64, 0.0, d, W=1024)
+    xg, rg, sg = _coupled_run(sf, _lib.BH_GLM_BF16X3, 0.5, d, W=1024)
+    assert sg["listed"] > 0 and sg["overflow"] == 0
+    assert np.array_equal(xg, x64)
+
+
+def test_glm_guard_rejects_bad_values(fac):
+    Xb, Xd, y = _problem(100, 32, 1)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y))
+    s = sf.create_sampler(1, 1024, np.zeros(1, np.float32))
+    with pytest.raises(_lib.ArgumentError):
+        s.set_glm_guard(-1.0)
+    with pytest.raises(_lib.ArgumentError):
+        s.set_glm_guard(float("nan"))
+    s.release()
+
+
+def test_kernel_timing_counts_contractions(fac):
+    Xb, Xd, y = _problem(20000, 64, 23)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y))
+    s = sf.create_sampler(2, 1024, np.zeros(1, np.float32))
+    s.init_position(3, np.tile(np.array([-0.1, 0.1], np.float32), 64))
+    s.set_kernel_timing(True)
+    s.burn_in(5, 2.0)
+    t = s.kernel_timing()
+    assert t["launches"] == 10 and t["avg_ms"] > 0.0
+    s.release()
