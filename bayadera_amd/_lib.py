@@ -100,6 +100,17 @@ SIGNATURES = {
     "bh_batch_init_position": [_P, _U32, _P],
     "bh_batch_burn_in": [_P, _U32, _U64, _F],
     "bh_batch_run": [_P, _U32, _U64, _F, _P, _P, _P],
+    "bh_batch_run_series": [_P, _U32, _U64, _F, _P, _P, _P, _P],
+    "bh_glm_batch_create": [_P, _I, _U32, _P, _P, _P, _P, _P, _F, _U32, _PP],
+    "bh_glm_batch_destroy": [_P],
+    "bh_glm_batch_set_precision": [_P, _I],
+    "bh_glm_batch_init_position": [_P, _U32, _P],
+    "bh_glm_batch_burn_in": [_P, _U32, _U64, _F],
+    "bh_glm_batch_run": [_P, _U32, _U64, _F, _P, _P, _P, _P],
+    "bh_glm_batch_state_get": [_P, _P, _P],
+    "bh_glm_batch_last_gpu_ms": [_P, _P],
+    "bh_glm_batch_log_density": [_P, _I, _P],
+    "bh_fp8_decode": [_P, _P, _P, _U64],
     "bh_batch_state_get": [_P, _P, _P],
     "bh_batch_last_gpu_ms": [_P, _P],
     "bh_stretch_set_timing": [_P, _I],

@@ -17,6 +17,7 @@
 
 #include "../../include/bayadera_hip.h"
 #include "bh_glm.hpp"
+#include "bh_glm_batch.hpp"
 #include "bh_kernels.hpp"
 
 // Sources handed to hipRTC, embedded at build time as raw string literals.
@@ -322,7 +323,7 @@ struct bh_stretch {
     DevBuf lp, gY, gz, gu2, gsplit, gpartial;
     // guarded accept test of the reduced precisions (guard 0: off)
     double glm_guard = 0.0;
-    DevBuf gflag, glp, glist, gcount, gtotals, gpart64;
+    DevBuf gflag, glist, gcount, gtotals, gpart64;
     // event pairs around the dominant kernel's launches (the stretch move
     // kernels, or the GLM likelihood contraction), up to the pool's size
     bool ktiming = false;
@@ -335,7 +336,8 @@ constexpr size_t kKernelTimerPairs = 1024;
 // Guard fallback: walkers re-evaluated in fp64 per half-step at most, and
 // the row ranges of that evaluation.
 constexpr unsigned kGuardListMax = 1024;
-constexpr unsigned kGuardRanges = 1024;
+constexpr unsigned kGuardRanges = 256;
+constexpr unsigned kGuardResolveBlocks = 16;
 
 namespace {
 
@@ -511,11 +513,9 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     unsigned* acc = accu ? s->accept.as<unsigned>() : nullptr;
     float* means = accu ? s->means_acc.as<float>() : nullptr;
     unsigned* flag = nullptr;
-    double* glp = nullptr;
     if (guarded) {
         // screen, fp64 evaluation of the listed walkers, their decisions
         flag = s->gflag.as<unsigned>();
-        glp = s->glp.as<double>();
         unsigned* list = s->glist.as<unsigned>();
         unsigned lmax = kGuardListMax;
         double guard = s->glm_guard * std::max(1.0f, beta);
@@ -533,13 +533,12 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         const unsigned* countc = gcount;
         unsigned long long* totals = s->gtotals.as<unsigned long long>();
         void* ra[] = {&Yc, &Xc, &zc, &u2c, &p64, &nr64, &stride, &lin64, &pp, &beta, &listc,
-                      &countc, &lmax, &flag, &glp, &totals};
-        BH_TRY(launch(f->glm_resolve, kGuardListMax / s->wgs, s->wgs, 0, s->stream, ra));
+                      &countc, &lmax, &flag, &totals};
+        BH_TRY(launch(f->glm_resolve, kGuardResolveBlocks, s->wgs, 0, s->stream, ra));
     }
     const unsigned* flagc = flag;
-    const double* glpc = glp;
     void* aa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &X, &lp, &beta, &K, &accu,
-                  &acc, &means, &mstep, &flagc, &glpc};
+                  &acc, &means, &mstep, &flagc};
     return launch(f->glm_accept, K / s->wgs, s->wgs, 0, s->stream, aa);
 }
 
@@ -559,7 +558,6 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
         BH_TRY(s->gz.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gu2.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gflag.alloc((size_t)k_half * sizeof(unsigned)));
-        BH_TRY(s->glp.alloc((size_t)k_half * sizeof(double)));
         BH_TRY(s->glist.alloc((size_t)kGuardListMax * sizeof(unsigned)));
         BH_TRY(s->gcount.alloc(sizeof(unsigned)));
         BH_TRY(s->gtotals.alloc(2 * sizeof(unsigned long long)));
@@ -1449,7 +1447,8 @@ int bh_batch_init_position(bh_batch* b, uint32_t seed, const float* limits) {
     return sync(b->stream);
 }
 
-static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mode) {
+static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mode,
+                       const unsigned long long* soff = nullptr, float* series = nullptr) {
     BH_HIP(hipSetDevice(b->ctx->device));
     const unsigned* dims = b->dims.as<unsigned>();
     const unsigned long long* poff = b->poff.as<unsigned long long>();
@@ -1467,8 +1466,9 @@ static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mod
     while (done < n) {
         unsigned step0 = b->bare_counter;
         unsigned chunk = (unsigned)std::min<uint64_t>(n - done, 1u << 20);
+        float* ser = series;  // recorded runs are a single launch (n < 2^20)
         void* args[] = {&seed0, &dims, &poff, &dlen, &plen, &params, &xs, &lf, &W, &a,
-                        &step0, &chunk, &partner, &mode, &acc, &sums};
+                        &step0, &chunk, &partner, &mode, &acc, &sums, &soff, &ser};
         BH_HIP(hipModuleLaunchKernel(b->run, b->problems, 1, 1, b->nb, 1, 1,
                                      (unsigned)batch_lds(b), b->stream, args, nullptr));
         b->bare_counter += chunk;
@@ -1488,15 +1488,37 @@ int bh_batch_burn_in(bh_batch* b, uint32_t seed0, uint64_t n, float a) {
     return batch_steps(b, seed0, n, a, 0);
 }
 
-int bh_batch_run(bh_batch* b, uint32_t seed0, uint64_t n, float a, double* acceptance_rate,
-                 double* mean, double* sd) {
+int bh_batch_run_series(bh_batch* b, uint32_t seed0, uint64_t n, float a,
+                        double* acceptance_rate, double* mean, double* sd, float* series) {
     BH_ARG(b, "NULL batch");
     BH_ARG(n > 0, "no steps");
+    BH_ARG(!series || n < (1u << 20), "a recorded run takes fewer than 2^20 steps");
     BH_HIP(hipSetDevice(b->ctx->device));
     BH_HIP(hipMemsetAsync(b->accept.p, 0, (size_t)b->problems * 8, b->stream));
     BH_HIP(hipMemsetAsync(b->sums.p, 0, (size_t)b->problems * b->max_dim * 2 * sizeof(double),
                           b->stream));
-    BH_TRY(batch_steps(b, seed0, n, a, 1));
+    DevOut ser;
+    DevBuf soff;
+    if (series) {
+        // problem p's n x dims[p] block starts at n * (dims[0] + .. + dims[p-1])
+        std::vector<uint32_t> dh(b->problems);
+        BH_HIP(hipMemcpyAsync(dh.data(), b->dims.p, dh.size() * 4, hipMemcpyDeviceToHost,
+                              b->stream));
+        BH_TRY(sync(b->stream));
+        std::vector<unsigned long long> off(b->problems);
+        unsigned long long o = 0;
+        for (uint32_t p = 0; p < b->problems; ++p) {
+            off[p] = o;
+            o += (unsigned long long)dh[p] * n;
+        }
+        BH_TRY(soff.alloc(off.size() * 8));
+        BH_HIP(hipMemcpyAsync(soff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice,
+                              b->stream));
+        BH_TRY(ser.init(series, (size_t)o * sizeof(float)));
+    }
+    BH_TRY(batch_steps(b, seed0, n, a, 1, series ? soff.as<unsigned long long>() : nullptr,
+                       series ? ser.as<float>() : nullptr));
+    if (series) BH_TRY(ser.finish(b->stream));
     std::vector<unsigned long long> acc(b->problems);
     std::vector<double> sums((size_t)b->problems * b->max_dim * 2);
     std::vector<uint32_t> dims(b->problems);
@@ -1519,6 +1541,11 @@ int bh_batch_run(bh_batch* b, uint32_t seed0, uint64_t n, float a, double* accep
     return BH_OK;
 }
 
+int bh_batch_run(bh_batch* b, uint32_t seed0, uint64_t n, float a, double* acceptance_rate,
+                 double* mean, double* sd) {
+    return bh_batch_run_series(b, seed0, n, a, acceptance_rate, mean, sd, nullptr);
+}
+
 int bh_batch_state_get(const bh_batch* b, float* xs, float* logfn) {
     BH_ARG(b, "NULL batch");
     BH_HIP(hipSetDevice(b->ctx->device));
@@ -1532,6 +1559,360 @@ int bh_batch_state_get(const bh_batch* b, float* xs, float* logfn) {
 }
 
 int bh_batch_last_gpu_ms(const bh_batch* b, double* ms) {
+    BH_ARG(b && ms, "NULL argument");
+    *ms = b->last_ms;
+    return BH_OK;
+}
+
+// ---------------------------------------------------------------- GLM batch
+}  // extern "C"
+
+struct bh_glm_batch {
+    bh_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t P = 0, W = 0, K = 0, U = 0, n32 = 0, n64 = 0;
+    int prec = BH_GLM_BF16X3;
+    double prior_prec = 1.0;
+    std::vector<uint32_t> dims_h;
+    DevBuf dims, scales, tscale, X8, xoff, units32, units64, ubeg, lin, lin_half;
+    DevBuf xs, lp, Y, z, u2, ysc, ysplit, partial, acc_cnt, msum, msum2, acc1, acc2;
+    uint32_t bare_counter = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+// e4m3 (OCP: bias 7, no infinities, S.1111.111 is NaN) to float
+float fp8_e4m3_value(uint8_t c) {
+    const int s = c >> 7, e = (c >> 3) & 15, m = c & 7;
+    float v;
+    if (e == 15 && m == 7) v = NAN;
+    else if (e == 0) v = std::ldexp((float)m, -9);
+    else v = std::ldexp(1.0f + (float)m / 8.0f, e - 7);
+    return s ? -v : v;
+}
+
+// log posteriors of the walkers of every problem (n = W points each)
+int gb_contract(bh_glm_batch* b, const float* Ypts, unsigned n) {
+    const unsigned char* x8 = b->X8.as<unsigned char>();
+    const unsigned long long* xoff = b->xoff.as<unsigned long long>();
+    double* partial = b->partial.as<double>();
+    BH_HIP(bh::launch_gb_contract(b->prec, 32, b->stream, x8, xoff, b->units32.p, b->n32, b->P,
+                                  b->scales.as<float>(), Ypts, b->ysc.as<float>(), b->ysplit.p, n,
+                                  partial));
+    BH_HIP(bh::launch_gb_contract(b->prec, 64, b->stream, x8, xoff, b->units64.p, b->n64, b->P,
+                                  b->scales.as<float>(), Ypts, b->ysc.as<float>(), b->ysplit.p, n,
+                                  partial));
+    return BH_OK;
+}
+
+const double* gb_lin(const bh_glm_batch* b) {
+    return (b->prec == BH_GLM_FP64 ? b->lin : b->lin_half).as<double>();
+}
+
+double gb_scale(const bh_glm_batch* b) { return b->prec == BH_GLM_FP64 ? 1.0 : kLn2; }
+
+int gb_logpost_all(bh_glm_batch* b) {
+    const int ns = glm_nsplit(b->prec);
+    BH_HIP(bh::launch_gb_points(b->stream, b->xs.as<float>(), b->dims.as<unsigned>(),
+                                b->tscale.as<float>(), b->P, b->W, b->ysc.as<float>(),
+                                b->ysplit.p, ns));
+    BH_TRY(gb_contract(b, b->xs.as<float>(), b->W));
+    BH_HIP(bh::launch_gb_logpost(b->stream, b->xs.as<float>(), b->dims.as<unsigned>(),
+                                 b->ubeg.as<unsigned>(), b->partial.as<double>(), b->P, b->W,
+                                 gb_lin(b), gb_scale(b), b->prior_prec, b->lp.as<double>()));
+    return BH_OK;
+}
+
+// n steps (both halves each) with seeds seed0 + 2p / + 1 and constants
+// 3333/4444; with accu the accounting and, if series, the per-step means.
+int gb_steps(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a, int accu,
+             const unsigned long long* soff, float* series) {
+    const int ns = glm_nsplit(b->prec);
+    BH_HIP(hipEventRecord(b->ev0, b->stream));
+    for (uint64_t st = 0; st < n; ++st) {
+        const unsigned step = b->bare_counter;
+        for (int half = 0; half < 2; ++half) {
+            BH_HIP(bh::launch_gb_propose(b->stream, seed0, half, half ? 4444u : 3333u, step, a,
+                                         b->dims.as<unsigned>(), b->tscale.as<float>(),
+                                         b->xs.as<float>(), b->P, b->K, b->Y.as<float>(),
+                                         b->z.as<float>(), b->u2.as<float>(), b->ysc.as<float>(),
+                                         b->ysplit.p, ns));
+            BH_TRY(gb_contract(b, b->Y.as<float>(), b->K));
+            BH_HIP(bh::launch_gb_accept(b->stream, half, b->dims.as<unsigned>(),
+                                        b->ubeg.as<unsigned>(), b->Y.as<float>(),
+                                        b->z.as<float>(), b->u2.as<float>(),
+                                        b->partial.as<double>(), gb_lin(b), gb_scale(b),
+                                        b->prior_prec, b->xs.as<float>(), b->lp.as<double>(),
+                                        1.0f, b->P, b->K, accu,
+                                        b->acc_cnt.as<unsigned long long>(), b->msum.as<float>(),
+                                        b->msum2.as<float>()));
+        }
+        if (accu)
+            BH_HIP(bh::launch_gb_step(b->stream, b->dims.as<unsigned>(), b->msum.as<float>(),
+                                      b->msum2.as<float>(), b->P, b->W, b->acc1.as<double>(),
+                                      b->acc2.as<double>(), soff, series, (unsigned)st));
+        b->bare_counter += 1;
+    }
+    BH_HIP(hipEventRecord(b->ev1, b->stream));
+    BH_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    BH_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    return BH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n) {
+    BH_ARG(ctx && in && out, "NULL argument");
+    BH_ARG(n > 0 && n % 4 == 0, "n must be a positive multiple of 4");
+    BH_HIP(hipSetDevice(ctx->device));
+    DevIn din;
+    BH_TRY(din.init(in, n, ctx->stream));
+    DevOut dout;
+    BH_TRY(dout.init(out, n * sizeof(float)));
+    BH_HIP(bh::launch_gb_fp8_decode(ctx->stream, din.dev, dout.as<float>(), n));
+    BH_TRY(dout.finish(ctx->stream));
+    return sync(ctx->stream);
+}
+
+int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t* rows,
+                        const uint32_t* dims, const void* X_fp8, const float* col_scales,
+                        const float* y, float prior_sd, uint32_t walkers, bh_glm_batch** out) {
+    BH_ARG(ctx && rows && dims && X_fp8 && col_scales && y && out, "NULL argument");
+    BH_ARG(kind == BH_GLM_LOGISTIC, "unknown GLM kind %d", kind);
+    BH_ARG(problems >= 1 && problems <= 65535, "bad problem count %u", problems);
+    BH_ARG(prior_sd > 0.0f, "prior sd must be positive");
+    if (!(walkers >= 512 && walkers % 512 == 0 && walkers <= 65536))
+        return fail(BH_ERR_RANGE, "batched GLM samplers need a multiple of 512 walkers, got %u",
+                    walkers);
+    BH_HIP(hipSetDevice(ctx->device));
+    auto b = std::make_unique<bh_glm_batch>();
+    b->ctx = ctx;
+    b->P = problems;
+    b->W = walkers;
+    b->K = walkers / 2;
+    b->prior_prec = 1.0 / ((double)prior_sd * (double)prior_sd);
+    b->dims_h.assign(dims, dims + problems);
+    float dec[256];
+    for (int c = 0; c < 256; ++c) dec[c] = fp8_e4m3_value((uint8_t)c);
+    // padded data, scales, linear terms and units
+    const uint8_t* xin = static_cast<const uint8_t*>(X_fp8);
+    std::vector<unsigned long long> xoff(problems);
+    unsigned long long total = 0, xin_off = 0, yoff = 0, soff = 0;
+    for (uint32_t p = 0; p < problems; ++p) {
+        BH_ARG(dims[p] >= 1 && dims[p] <= 64, "problem %u: dimension %u outside [1, 64]", p, dims[p]);
+        BH_ARG(rows[p] >= 1 && rows[p] < (1ull << 31), "problem %u: bad row count", p);
+        const unsigned D = dims[p] <= 32 ? 32 : 64;
+        xoff[p] = total;
+        total += (rows[p] * D + 255) / 256 * 256;
+    }
+    std::vector<uint8_t> xpad(total, 0);
+    std::vector<float> sc((size_t)problems * 64, 0.0f), ts((size_t)problems * 64, 0.0f);
+    std::vector<double> lin((size_t)problems * 64, 0.0), linh((size_t)problems * 64, 0.0);
+    std::vector<uint32_t> u32, u64, ub(problems + 1);
+    uint32_t uid = 0;
+    for (uint32_t p = 0; p < problems; ++p) {
+        const unsigned d = dims[p], D = d <= 32 ? 32 : 64;
+        const unsigned long long N = rows[p];
+        for (unsigned j = 0; j < d; ++j) {
+            const float s = col_scales[soff + j];
+            BH_ARG(std::isfinite(s), "problem %u: column scale %u not finite", p, j);
+            sc[(size_t)p * 64 + j] = s;
+            ts[(size_t)p * 64 + j] = (float)((double)s * 1.4426950408889634);
+        }
+        std::vector<double> l(d, 0.0), cs(d, 0.0);
+        for (unsigned long long i = 0; i < N; ++i) {
+            const uint8_t* r = xin + xin_off + i * d;
+            std::memcpy(&xpad[xoff[p] + i * D], r, d);
+            const double yi = y[yoff + i];
+            for (unsigned j = 0; j < d; ++j) {
+                const float q = dec[r[j]];
+                BH_ARG(std::isfinite(q), "problem %u: fp8 NaN in row %llu", p, i);
+                const double x = (double)sc[(size_t)p * 64 + j] * (double)q;
+                l[j] += yi * x;
+                cs[j] += x;
+            }
+        }
+        for (unsigned j = 0; j < d; ++j) {
+            lin[(size_t)p * 64 + j] = l[j];
+            linh[(size_t)p * 64 + j] = l[j] - 0.5 * cs[j];
+        }
+        ub[p] = uid;
+        for (unsigned long long r0 = 0; r0 < N; r0 += bh::kGbUnitRows) {
+            const unsigned long long r1 = std::min<unsigned long long>(N, r0 + bh::kGbUnitRows);
+            auto& dst = D == 32 ? u32 : u64;
+            dst.insert(dst.end(), {p, (uint32_t)r0, (uint32_t)r1, uid++});
+        }
+        xin_off += N * d;
+        yoff += N;
+        soff += d;
+    }
+    ub[problems] = uid;
+    b->U = uid;
+    b->n32 = (uint32_t)(u32.size() / 4);
+    b->n64 = (uint32_t)(u64.size() / 4);
+    const uint32_t P = problems, W = walkers, K = walkers / 2;
+    BH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BH_HIP(hipEventCreate(&b->ev0));
+    BH_HIP(hipEventCreate(&b->ev1));
+    auto up = [&](DevBuf& d, const void* h, size_t bytes) -> int {
+        BH_TRY(d.alloc(std::max<size_t>(bytes, 16)));
+        if (bytes) BH_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, b->stream));
+        return BH_OK;
+    };
+    BH_TRY(up(b->dims, dims, (size_t)P * 4));
+    BH_TRY(up(b->scales, sc.data(), sc.size() * 4));
+    BH_TRY(up(b->tscale, ts.data(), ts.size() * 4));
+    BH_TRY(up(b->X8, xpad.data(), xpad.size()));
+    BH_TRY(up(b->xoff, xoff.data(), xoff.size() * 8));
+    BH_TRY(up(b->units32, u32.data(), u32.size() * 4));
+    BH_TRY(up(b->units64, u64.data(), u64.size() * 4));
+    BH_TRY(up(b->ubeg, ub.data(), ub.size() * 4));
+    BH_TRY(up(b->lin, lin.data(), lin.size() * 8));
+    BH_TRY(up(b->lin_half, linh.data(), linh.size() * 8));
+    BH_TRY(b->xs.alloc((size_t)P * W * 64 * 4));
+    BH_TRY(b->lp.alloc((size_t)P * W * 8));
+    BH_TRY(b->Y.alloc((size_t)P * K * 64 * 4));
+    BH_TRY(b->z.alloc((size_t)P * K * 4));
+    BH_TRY(b->u2.alloc((size_t)P * K * 4));
+    BH_TRY(b->ysc.alloc((size_t)P * W * 64 * 4));
+    BH_TRY(b->ysplit.alloc((size_t)3 * P * W * 64 * 2));
+    BH_TRY(b->partial.alloc((size_t)std::max(uid, 1u) * W * 8));
+    BH_TRY(b->acc_cnt.alloc((size_t)P * 8));
+    BH_TRY(b->msum.alloc((size_t)P * 8 * 64 * 4));
+    BH_TRY(b->msum2.alloc((size_t)P * 8 * 64 * 4));
+    BH_TRY(b->acc1.alloc((size_t)P * 64 * 8));
+    BH_TRY(b->acc2.alloc((size_t)P * 64 * 8));
+    BH_HIP(hipMemsetAsync(b->xs.p, 0, (size_t)P * W * 64 * 4, b->stream));
+    BH_TRY(sync(b->stream));
+    *out = b.release();
+    return BH_OK;
+}
+
+int bh_glm_batch_destroy(bh_glm_batch* b) {
+    if (!b) return BH_OK;
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipStreamDestroy(b->stream);
+    }
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+    return BH_OK;
+}
+
+int bh_glm_batch_set_precision(bh_glm_batch* b, int precision) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    b->prec = precision;
+    return BH_OK;
+}
+
+int bh_glm_batch_init_position(bh_glm_batch* b, uint32_t seed, const float* limits) {
+    BH_ARG(b && limits, "NULL argument");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    for (uint32_t p = 0; p < b->P; ++p)
+        BH_ARG((uint64_t)b->dims_h[p] * b->W % 4 == 0, "problem %u: dim*walkers not a multiple of 4", p);
+    DevIn dl;
+    BH_TRY(dl.init(limits, (size_t)b->P * 64 * 2 * sizeof(float), b->stream));
+    BH_HIP(hipMemsetAsync(b->xs.p, 0, (size_t)b->P * b->W * 64 * 4, b->stream));
+    BH_HIP(bh::launch_gb_init(b->stream, seed, b->dims.as<unsigned>(), dl.as<float>(),
+                              b->xs.as<float>(), b->P, b->W));
+    BH_TRY(gb_logpost_all(b));
+    b->bare_counter = 0;
+    return sync(b->stream);
+}
+
+int bh_glm_batch_burn_in(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a) {
+    BH_ARG(b, "NULL batch");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    return gb_steps(b, seed0, n, a, 0, nullptr, nullptr);
+}
+
+int bh_glm_batch_run(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a,
+                     double* acceptance_rate, double* mean, double* sd, float* series) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(n > 0 && n < (1u << 30), "bad step count");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    const uint32_t P = b->P;
+    BH_HIP(hipMemsetAsync(b->acc_cnt.p, 0, (size_t)P * 8, b->stream));
+    BH_HIP(hipMemsetAsync(b->acc1.p, 0, (size_t)P * 64 * 8, b->stream));
+    BH_HIP(hipMemsetAsync(b->acc2.p, 0, (size_t)P * 64 * 8, b->stream));
+    DevOut ser;
+    DevBuf soff;
+    if (series) {
+        std::vector<unsigned long long> off(P);
+        unsigned long long o = 0;
+        for (uint32_t p = 0; p < P; ++p) {
+            off[p] = o;
+            o += (unsigned long long)b->dims_h[p] * n;
+        }
+        BH_TRY(soff.alloc(off.size() * 8));
+        BH_HIP(hipMemcpyAsync(soff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice,
+                              b->stream));
+        BH_TRY(ser.init(series, (size_t)o * sizeof(float)));
+    }
+    BH_TRY(gb_steps(b, seed0, n, a, 1, series ? soff.as<unsigned long long>() : nullptr,
+                    series ? ser.as<float>() : nullptr));
+    if (series) BH_TRY(ser.finish(b->stream));
+    std::vector<unsigned long long> acc(P);
+    std::vector<double> s1((size_t)P * 64), s2((size_t)P * 64);
+    BH_HIP(hipMemcpyAsync(acc.data(), b->acc_cnt.p, acc.size() * 8, hipMemcpyDeviceToHost,
+                          b->stream));
+    BH_HIP(hipMemcpyAsync(s1.data(), b->acc1.p, s1.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    BH_HIP(hipMemcpyAsync(s2.data(), b->acc2.p, s2.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    BH_TRY(sync(b->stream));
+    const double cnt = (double)b->W * (double)n;
+    for (uint32_t p = 0; p < P; ++p) {
+        if (acceptance_rate) acceptance_rate[p] = (double)acc[p] / cnt;
+        for (uint32_t i = 0; i < 64; ++i) {
+            const size_t o = (size_t)p * 64 + i;
+            const bool valid = i < b->dims_h[p];
+            const double m = valid ? s1[o] / cnt : 0.0;
+            const double v = valid ? std::max(0.0, s2[o] / cnt - m * m) : 0.0;
+            if (mean) mean[o] = m;
+            if (sd) sd[o] = std::sqrt(v);
+        }
+    }
+    return BH_OK;
+}
+
+int bh_glm_batch_state_get(const bh_glm_batch* b, float* xs, double* lp) {
+    BH_ARG(b, "NULL batch");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    if (xs)
+        BH_HIP(hipMemcpyAsync(xs, b->xs.p, (size_t)b->P * b->W * 64 * 4, hipMemcpyDefault,
+                              b->stream));
+    if (lp)
+        BH_HIP(hipMemcpyAsync(lp, b->lp.p, (size_t)b->P * b->W * 8, hipMemcpyDefault, b->stream));
+    return sync(b->stream);
+}
+
+int bh_glm_batch_log_density(bh_glm_batch* b, int precision, double* lp) {
+    BH_ARG(b && lp, "NULL argument");
+    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    BH_HIP(hipSetDevice(b->ctx->device));
+    DevBuf keep;
+    BH_TRY(keep.alloc((size_t)b->P * b->W * 8));
+    BH_HIP(hipMemcpyAsync(keep.p, b->lp.p, (size_t)b->P * b->W * 8, hipMemcpyDeviceToDevice,
+                          b->stream));
+    const int prec = b->prec;
+    b->prec = precision;
+    const int rc = gb_logpost_all(b);
+    b->prec = prec;
+    BH_TRY(rc);
+    BH_HIP(hipMemcpyAsync(lp, b->lp.p, (size_t)b->P * b->W * 8, hipMemcpyDefault, b->stream));
+    BH_HIP(hipMemcpyAsync(b->lp.p, keep.p, (size_t)b->P * b->W * 8, hipMemcpyDeviceToDevice,
+                          b->stream));
+    return sync(b->stream);
+}
+
+int bh_glm_batch_last_gpu_ms(const bh_glm_batch* b, double* ms) {
     BH_ARG(b && ms, "NULL argument");
     *ms = b->last_ms;
     return BH_OK;

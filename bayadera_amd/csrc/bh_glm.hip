@@ -17,25 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include "bh_glm.hpp"
+#include "bh_glm_core.hpp"
 
 namespace bh {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// sum over four values of |u|/2 + log2(1 + 2^-|u|): four v_exp_f32 and one
-// v_log_f32 of the product of the four factors 1 + 2^-|u|, which lies in
-// [1, 16] (each factor is in (1, 2]); every transcendental runs where it is
-// accurate to about an ulp.
-__device__ __forceinline__ float softplus2_rest4(float u0, float u1, float u2, float u3) {
-    const float p01 = (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u0))) *
-                      (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u1)));
-    const float p23 = (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u2))) *
-                      (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u3)));
-    const float a = (__builtin_fabsf(u0) + __builtin_fabsf(u1)) +
-                    (__builtin_fabsf(u2) + __builtin_fabsf(u3));
-    return __builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p01 * p23));
-}
 
 // Block index -> (walker group, row range), XCD-aware: blocks are dealt to
 // the 8 XCDs round-robin, so the consecutive blocks one XCD receives take all
@@ -48,56 +32,41 @@ __device__ __forceinline__ void glm_tile(unsigned groups, unsigned& group, unsig
     range = (idx / groups) * 8 + (b & 7);
 }
 
+// A fragments from bf16 X (row-major N x D): lane row lr, features
+// kc*32 + lg*8 .. +7.
+template <int D>
+struct Bf16Rows {
+    const __bf16* X;
+    unsigned lr, lg;
+    __device__ void full(unsigned rb, bf16x8 (&a)[D / 32]) const {
+        const __bf16* p = X + (size_t)(rb + lr) * D + lg * 8;
+#pragma unroll
+        for (int kc = 0; kc < D / 32; ++kc) a[kc] = *reinterpret_cast<const bf16x8*>(p + kc * 32);
+    }
+    __device__ void masked(unsigned rb, unsigned r1, bf16x8 (&a)[D / 32]) const {
+        const unsigned row = rb + lr;
+#pragma unroll
+        for (int kc = 0; kc < D / 32; ++kc) {
+            if (row < r1) {
+                a[kc] = *reinterpret_cast<const bf16x8*>(X + (size_t)row * D + kc * 32 + lg * 8);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[kc][j] = (__bf16)0.0f;
+            }
+        }
+    }
+};
+
 // MFMA kernel: v_mfma_f32_16x16x32_bf16.  A block of 4 waves owns 256
 // points, 64 per wave as four 16-point column blocks whose B fragments (all
 // D features, NS split terms) stay in registers for the whole row range.
 // Lane layout (16x16x32, checked on the device): A lane l holds row l%16,
 // features (l/16)*8..+7 of a 32-feature chunk; B lane l holds point l%16,
 // the same features; the accumulator lane l holds rows (l/16)*4..+3 of
-// point l%16.  The loop is software-pipelined: the matrix-core products of
-// row block i+1 are issued while the softplus epilogue of block i runs on
-// the VALU, and the scheduler is asked to interleave them.  Split terms are
-// accumulated smallest first.  Rows are masked only in the last block.
-template <int D, int NS>
-__device__ __forceinline__ void glm_mfma_block(const bf16x8 (&a)[D / 32],
-                                               const bf16x8 (&bfrag)[4][D / 32][NS],
-                                               f32x4 (&c)[4]) {
-#pragma unroll
-    for (int wb = 0; wb < 4; ++wb) {
-        c[wb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = NS - 1; s >= 0; --s)
-#pragma unroll
-            for (int kc = 0; kc < D / 32; ++kc)
-                c[wb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], c[wb], 0,
-                                                                0, 0);
-    }
-}
-
-__device__ __forceinline__ void glm_epilogue(const f32x4 (&c)[4], double (&acc)[4]) {
-#pragma unroll
-    for (int wb = 0; wb < 4; ++wb)
-        acc[wb] += (double)softplus2_rest4(c[wb][0], c[wb][1], c[wb][2], c[wb][3]);
-}
-
-// rows at or past r1 contribute nothing: u = +inf gives the factor 1 + 0,
-// and its |u|/2 is taken out by zeroing it in the sum
-__device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double (&acc)[4],
-                                                    unsigned orow, unsigned r1) {
-#pragma unroll
-    for (int wb = 0; wb < 4; ++wb) {
-        float p = 1.0f, a = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (orow + i < r1) {
-                p *= 1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(c[wb][i]));
-                a += __builtin_fabsf(c[wb][i]);
-            }
-        }
-        acc[wb] += (double)__builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p));
-    }
-}
-
+// point l%16.  The row loop (glm_mfma_rows) is software-pipelined: the
+// matrix-core products of row block i+1 are issued with the softplus
+// epilogue of block i placed between them.  Split terms are accumulated
+// smallest first.  Rows are masked only in the last block.
 template <int D, int NS>
 __global__ void __launch_bounds__(256)
 glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned N,
@@ -125,61 +94,7 @@ glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned N,
     const unsigned r0 = range * rows_per_range;
     const unsigned r1 = (r0 >= N) ? r0 : ((N - r0 < rows_per_range) ? N : r0 + rows_per_range);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (r0 >= r1) {
-        if (lg == 0)
-#pragma unroll
-            for (int wb = 0; wb < 4; ++wb) partial[(size_t)range * n + w0 + wb * 16 + lr] = 0.0;
-        return;
-    }
-    // full 16-row blocks [r0, rfull), then one masked block
-    const unsigned rfull = r0 + (r1 - r0) / 16 * 16;
-    const __bf16* xl = X + (size_t)lr * D + lg * 8;
-    auto load_full = [&](unsigned rb, bf16x8 (&a)[KC]) {
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
-            a[kc] = *reinterpret_cast<const bf16x8*>(xl + (size_t)rb * D + kc * 32);
-    };
-    f32x4 c[4];
-    if (rfull > r0) {
-        const unsigned rlast = rfull - 16;
-        bf16x8 a[KC], an[KC];
-        load_full(r0, a);
-        load_full(r0 + 16 <= rlast ? r0 + 16 : rlast, an);
-        glm_mfma_block<D, NS>(a, bfrag, c);
-        for (unsigned rb = r0 + 16; rb < rfull; rb += 16) {
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc) a[kc] = an[kc];
-            // prefetch the next block (clamped: the last iteration reloads
-            // the last block, which is in range)
-            load_full(rb + 16 <= rlast ? rb + 16 : rlast, an);
-            f32x4 cn[4];
-            glm_mfma_block<D, NS>(a, bfrag, cn);
-            glm_epilogue(c, acc);
-#pragma unroll
-            for (int i = 0; i < 4 * KC * NS; ++i) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
-            }
-#pragma unroll
-            for (int wb = 0; wb < 4; ++wb) c[wb] = cn[wb];
-        }
-        glm_epilogue(c, acc);
-    }
-    if (rfull < r1) {
-        bf16x8 a[KC];
-        const unsigned row = rfull + lr;
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-            if (row < r1) {
-                a[kc] = *reinterpret_cast<const bf16x8*>(X + (size_t)row * D + kc * 32 + lg * 8);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) a[kc][j] = (__bf16)0.0f;
-            }
-        }
-        glm_mfma_block<D, NS>(a, bfrag, c);
-        glm_epilogue_masked(c, acc, rfull + lg * 4, r1);
-    }
+    if (r0 < r1) glm_mfma_rows<D, NS>(Bf16Rows<D>{X, lr, lg}, bfrag, r0, r1, lg, acc);
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
         double v = acc[wb];
@@ -283,10 +198,11 @@ glm_logistic_fp64_kernel(const __bf16* __restrict__ X, unsigned long long N,
     partial[(size_t)range * n + k] = acc;
 }
 
-// fp64 softplus sums of listed points (the guarded accept test's fallback):
-// block r owns one row range and runs the listed points eight at a time,
-// their coefficients in LDS, one row per thread per pass with the same
-// arithmetic as the fp64 kernel; per-point sums are reduced over the block.
+// fp64 softplus sums of listed points (the guarded accept test's fallback).
+// Block (r, s) owns row range r and runs points p = s, s + gridDim.y, ...
+// of the 2c listed ones (c = min(*count, lmax)); its 256 threads take the
+// range's rows one each, 256 at a time, with the point's coefficients read
+// from LDS as broadcasts and the same arithmetic as the fp64 kernel.
 template <int D>
 __global__ void __launch_bounds__(256)
 glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
@@ -294,55 +210,41 @@ glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
                      const unsigned* __restrict__ list, const unsigned* __restrict__ count,
                      unsigned lmax, unsigned long long rows_per_range,
                      double* __restrict__ partial64) {
-    constexpr int PC = 8;
-    __shared__ double b[PC][D];
-    __shared__ double red[PC][4];
+    __shared__ double b[D];
+    __shared__ double red[4];
     const unsigned n = *count;
     const unsigned c = n < lmax ? n : lmax;
-    if (c == 0) return;
     const unsigned np = 2 * c;
     const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_range;
     unsigned long long r1 = r0 + rows_per_range;
     if (r1 > N) r1 = N;
-    for (unsigned p0 = 0; p0 < np; p0 += PC) {
+    for (unsigned p = blockIdx.y; p < np; p += gridDim.y) {
         __syncthreads();
-        for (unsigned e = t; e < PC * D; e += 256) {
-            const unsigned p = p0 + e / D;
-            double v = 0.0;
-            if (p < np) {
-                const unsigned w = list[p < c ? p : p - c];
-                v = (double)(p < c ? Y : W)[(size_t)w * D + e % D];
-            }
-            b[e / D][e % D] = v;
+        if (t < D) {
+            const unsigned w = list[p < c ? p : p - c];
+            b[t] = (double)(p < c ? Y : W)[(size_t)w * D + t];
         }
         __syncthreads();
-        double acc[PC];
-#pragma unroll
-        for (int q = 0; q < PC; ++q) acc[q] = 0.0;
+        double acc = 0.0;
         for (unsigned long long row = r0 + t; row < r1; row += 256) {
-            double x[D];
+            const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
+            double eta = 0.0;
 #pragma unroll
-            for (int j = 0; j < D; ++j) x[j] = (double)(float)X[row * D + j];
+            for (int v = 0; v < D / 8; ++v) {
+                const bf16x8 xv = xr[v];
 #pragma unroll
-            for (int q = 0; q < PC; ++q) {
-                double eta = 0.0;
-#pragma unroll
-                for (int j = 0; j < D; ++j) eta = __builtin_fma(x[j], b[q][j], eta);
-                acc[q] += __builtin_fmax(eta, 0.0) + log1p(exp(-__builtin_fabs(eta)));
+                for (int e = 0; e < 8; ++e)
+                    eta = __builtin_fma((double)(float)xv[e], b[v * 8 + e], eta);
             }
+            acc += __builtin_fmax(eta, 0.0) + log1p(exp(-__builtin_fabs(eta)));
         }
 #pragma unroll
-        for (int q = 0; q < PC; ++q) {
-            double v = acc[q];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) red[q][wave] = v;
-        }
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) red[wave] = acc;
         __syncthreads();
-        if (t < PC && p0 + t < np)
-            partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
-                (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+        if (t == 0)
+            partial64[(size_t)blockIdx.x * 2 * lmax + p] = (red[0] + red[1]) + (red[2] + red[3]);
     }
 }
 
@@ -352,12 +254,13 @@ hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long
                                 double* partial64) {
     const unsigned long long rpr = (N + nranges - 1) / nranges;
     const __bf16* x = static_cast<const __bf16*>(X);
+    const dim3 grid(nranges, 8);
     if (d == 64)
-        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
-                                                         partial64);
+        glm_fp64_list_kernel<64><<<grid, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
+                                                      partial64);
     else if (d == 32)
-        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
-                                                         partial64);
+        glm_fp64_list_kernel<32><<<grid, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
+                                                      partial64);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -1,0 +1,183 @@
+// Device building blocks of the GLM likelihood kernels, shared by the
+// single-posterior kernels (bh_glm.hip) and the batched ones
+// (bh_glm_batch.hip): the softplus epilogue, the v_mfma_f32_16x16x32_bf16
+// products of one 16-row block, and the software-pipelined loop over the
+// row blocks of a range with the epilogue spread between the products.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace bh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// sum over four values of |u|/2 + log2(1 + 2^-|u|): four v_exp_f32 and one
+// v_log_f32 of the product of the four factors 1 + 2^-|u|, which lies in
+// [1, 16] (each factor is in (1, 2]); every transcendental runs where it is
+// accurate to about an ulp.
+__device__ __forceinline__ float softplus2_rest4(float u0, float u1, float u2, float u3) {
+    const float p01 = (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u0))) *
+                      (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u1)));
+    const float p23 = (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u2))) *
+                      (1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(u3)));
+    const float a = (__builtin_fabsf(u0) + __builtin_fabsf(u1)) +
+                    (__builtin_fabsf(u2) + __builtin_fabsf(u3));
+    return __builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p01 * p23));
+}
+
+template <int D, int NS>
+__device__ __forceinline__ void glm_mfma_block(const bf16x8 (&a)[D / 32],
+                                               const bf16x8 (&bfrag)[4][D / 32][NS],
+                                               f32x4 (&c)[4]) {
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb) {
+        c[wb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = NS - 1; s >= 0; --s)
+#pragma unroll
+            for (int kc = 0; kc < D / 32; ++kc)
+                c[wb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], c[wb], 0,
+                                                                0, 0);
+    }
+}
+
+__device__ __forceinline__ void glm_epilogue(const f32x4 (&c)[4], double (&acc)[4]) {
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb)
+        acc[wb] += (double)softplus2_rest4(c[wb][0], c[wb][1], c[wb][2], c[wb][3]);
+}
+
+// rows at or past r1 contribute nothing: u = +inf gives the factor 1 + 0,
+// and its |u|/2 is taken out by zeroing it in the sum
+__device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double (&acc)[4],
+                                                    unsigned orow, unsigned r1) {
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb) {
+        float p = 1.0f, a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (orow + i < r1) {
+                p *= 1.0f + __builtin_amdgcn_exp2f(-__builtin_fabsf(c[wb][i]));
+                a += __builtin_fabsf(c[wb][i]);
+            }
+        }
+        acc[wb] += (double)__builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p));
+    }
+}
+
+// One piece of the epilogue of four values (one column block): six pieces
+// per block, each a few VALU instructions, so that they can be placed
+// between matrix-core products.  State carried between pieces in e/q.
+__device__ __forceinline__ void glm_epilogue_piece(int ph, const f32x4& c, float (&e)[4],
+                                                   float& q, double& acc) {
+    switch (ph) {
+    case 0:
+        e[0] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[0]));
+        e[1] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[1]));
+        break;
+    case 1:
+        e[2] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[2]));
+        e[3] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[3]));
+        break;
+    case 2:
+        e[0] = (1.0f + e[0]) * (1.0f + e[1]);
+        e[2] = (1.0f + e[2]) * (1.0f + e[3]);
+        break;
+    case 3:
+        q = __builtin_amdgcn_logf(e[0] * e[2]);
+        break;
+    case 4:
+        e[1] = (__builtin_fabsf(c[0]) + __builtin_fabsf(c[1])) +
+               (__builtin_fabsf(c[2]) + __builtin_fabsf(c[3]));
+        q = __builtin_fmaf(0.5f, e[1], q);
+        break;
+    default:
+        acc += (double)q;
+        break;
+    }
+}
+
+// The products of one 16-row block into cn (zeroed by the first product of
+// each chain) with the epilogue of the previous block's c spread between
+// them: products of the four column blocks alternate, so consecutive
+// products are independent, and scheduling barriers keep each epilogue
+// piece between two products, where it runs on the VALU while the matrix
+// core works.  The per-column-block product order is that of
+// glm_mfma_block (split terms smallest first, chunks in order).
+template <int D, int NS>
+__device__ __forceinline__ void glm_mfma_overlap(const bf16x8 (&a)[D / 32],
+                                                 const bf16x8 (&bfrag)[4][D / 32][NS],
+                                                 f32x4 (&cn)[4], const f32x4 (&c)[4],
+                                                 double (&acc)[4]) {
+    constexpr int KC = D / 32;
+    constexpr int NM = 4 * KC * NS;
+    constexpr int NP = 24;  // epilogue pieces: 6 per column block
+    float e[4][4];
+    float q[4];
+    int m = 0;
+#pragma unroll
+    for (int s = NS - 1; s >= 0; --s)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int wb = 0; wb < 4; ++wb) {
+                const f32x4 acc0 = (s == NS - 1 && kc == 0) ? f32x4{0.0f, 0.0f, 0.0f, 0.0f}
+                                                            : cn[wb];
+                cn[wb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], acc0, 0,
+                                                                 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int pc = m * NP / NM; pc < (m + 1) * NP / NM; ++pc)
+                    glm_epilogue_piece(pc % 6, c[pc / 6], e[pc / 6], q[pc / 6], acc[pc / 6]);
+                __builtin_amdgcn_sched_barrier(0);
+                ++m;
+            }
+}
+
+// The row blocks [r0, r1) of one range for the 64 points whose B fragments
+// the wave holds: full 16-row blocks through the software pipeline, then one
+// masked block.  `ld` loads the A fragments of the block starting at a row
+// (ld.full(rb, a): all 16 rows valid; ld.masked(rb, r1, a): rows >= r1 read
+// as zeros).  acc[wb] accumulates this lane's four rows of column block wb.
+template <int D, int NS, class Loader>
+__device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
+                                              const bf16x8 (&bfrag)[4][D / 32][NS], unsigned r0,
+                                              unsigned r1, unsigned lg, double (&acc)[4]) {
+    constexpr int KC = D / 32;
+    const unsigned rfull = r0 + (r1 - r0) / 16 * 16;
+    f32x4 c0[4], c1[4];
+    if (rfull > r0) {
+        // software pipeline over 16-row blocks, unrolled by two so the
+        // accumulator sets c0/c1 alternate without copies: the products of
+        // block i+1 run with the epilogue of block i.  Row blocks past the
+        // last are clamped to it (loaded, never used).
+        const unsigned rlast = rfull - 16;
+        auto clamp = [&](unsigned rb) { return rb <= rlast ? rb : rlast; };
+        bf16x8 a0[KC], a1[KC];
+        ld.full(r0, a0);
+        ld.full(clamp(r0 + 16), a1);
+        glm_mfma_block<D, NS>(a0, bfrag, c0);
+        unsigned rb = r0 + 16;
+        for (; rb + 16 < rfull; rb += 32) {
+            ld.full(clamp(rb + 16), a0);
+            glm_mfma_overlap<D, NS>(a1, bfrag, c1, c0, acc);
+            ld.full(clamp(rb + 32), a1);
+            glm_mfma_overlap<D, NS>(a0, bfrag, c0, c1, acc);
+        }
+        if (rb < rfull) {
+            glm_mfma_overlap<D, NS>(a1, bfrag, c1, c0, acc);
+            glm_epilogue(c1, acc);
+        } else {
+            glm_epilogue(c0, acc);
+        }
+    }
+    if (rfull < r1) {
+        bf16x8 a[KC];
+        ld.masked(rfull, r1, a);
+        f32x4 c[4];
+        glm_mfma_block<D, NS>(a, bfrag, c);
+        glm_epilogue_masked(c, acc, rfull + lg * 4, r1);
+    }
+}
+
+}  // namespace bh

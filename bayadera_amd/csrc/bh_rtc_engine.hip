@@ -381,7 +381,9 @@ bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsig
 // problem's own dimension; a proposal is built in the thread's LDS slot,
 // where the model reads it.  Mode 1 also accumulates per problem, in double
 // and in a fixed order, the sums over steps of the ensemble sums of x and
-// x^2 (each step's ensemble sums are float tree sums over the walkers).
+// x^2 (each step's ensemble sums are float tree sums over the walkers); with
+// series non-null it also writes each step's ensemble mean (that float sum
+// over W), problem b's n x dims[b] block starting at series + soff[b].
 // LDS: W*dim + W + nb*dim floats + 2*DIM doubles + 8*nb floats.
 extern "C" __global__ void __launch_bounds__(1024)
 bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
@@ -389,7 +391,8 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
              const unsigned* __restrict__ plen, const float* __restrict__ params,
              float* __restrict__ xs, float* __restrict__ logfn, unsigned W, float a,
              unsigned step0, unsigned nsteps, int partner, int mode,
-             unsigned long long* __restrict__ accept, double* __restrict__ sums) {
+             unsigned long long* __restrict__ accept, double* __restrict__ sums,
+             const unsigned long long* __restrict__ soff, float* __restrict__ series) {
     extern __shared__ double lds_d[];
     const unsigned b = blockIdx.x;
     const unsigned dim = dims[b];
@@ -482,6 +485,8 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
                 if (t < 4 && d0 + t < dim) {
                     acc1[d0 + t] += (double)red[t * nb];
                     acc2[d0 + t] += (double)red[(4 + t) * nb];
+                    if (series)
+                        series[soff[b] + (size_t)st * dim + d0 + t] = red[t * nb] / (float)W;
                 }
                 __syncthreads();
             }
@@ -624,6 +629,32 @@ __device__ __forceinline__ double bh_glm_logpost(const float (&y)[DIM], unsigned
     return l - scale * s - 0.5 * prior_prec * q;
 }
 
+// The same from the softplus sum s of point y.
+__device__ __forceinline__ double bh_glm_logpost_sum(const float (&y)[DIM], double s,
+                                                     const double* __restrict__ lin, double scale,
+                                                     double prior_prec) {
+    double l = 0.0, q = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        l = __builtin_fma(lin[i], (double)y[i], l);
+        q = __builtin_fma((double)y[i], (double)y[i], q);
+    }
+    return l - scale * s - 0.5 * prior_prec * q;
+}
+
+// Sum over r < nranges of partial[r * stride + col] by one wave: lane-strided
+// partial sums, then a butterfly (a fixed order).
+__device__ __forceinline__ double bh_wave_range_sum(const double* __restrict__ partial,
+                                                    unsigned nranges, unsigned stride,
+                                                    unsigned col) {
+    const unsigned lane = threadIdx.x & 63u;
+    double s = 0.0;
+    for (unsigned r = lane; r < nranges; r += 64) s += partial[(size_t)r * stride + col];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
 // Initial log posteriors of n points.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_logpost(const float* __restrict__ P, unsigned n, const double* __restrict__ partial,
@@ -682,65 +713,65 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
     flag[k] = f;
 }
 
-// Decides the listed walkers on their fp64 log posteriors: partial64 holds,
-// per row range r, column i < c the softplus sum of Y[list[i]] and column
-// c + i that of X[list[i]] (natural units, row stride `stride`).  The
-// accepted walker's fp64 lp(Y) goes to glp.  totals[0] += listed walkers,
-// totals[1] += walkers past lmax.
+// Decides the listed walkers on their fp64 log posteriors, one wave per
+// walker: partial64 holds, per row range r (row stride `stride`), column
+// i < c the softplus sum of Y[list[i]] and column c + i that of X[list[i]]
+// (natural units).
+// totals[0] += listed walkers, totals[1] += walkers past lmax.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
                const float* __restrict__ zbuf, const float* __restrict__ u2buf,
                const double* __restrict__ partial64, unsigned nranges64, unsigned stride,
                const double* __restrict__ lin64, double prior_prec, float beta,
                const unsigned* __restrict__ list, const unsigned* __restrict__ count,
-               unsigned lmax, unsigned* __restrict__ flag, double* __restrict__ glp,
+               unsigned lmax, unsigned* __restrict__ flag,
                unsigned long long* __restrict__ totals) {
-    const unsigned i = blockIdx.x * WGS + threadIdx.x;
     const unsigned n = *count;
     const unsigned c = n < lmax ? n : lmax;
-    if (i == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&totals[0], (unsigned long long)c);
         atomicAdd(&totals[1], (unsigned long long)(n - c));
     }
-    if (i >= c) return;
-    const unsigned k = list[i];
-    float y[DIM], xk[DIM];
+    const unsigned waves = WGS / 64;
+    for (unsigned i = blockIdx.x * waves + threadIdx.x / 64; i < c; i += gridDim.x * waves) {
+        const double sy = bh_wave_range_sum(partial64, nranges64, stride, i);
+        const double sx = bh_wave_range_sum(partial64, nranges64, stride, c + i);
+        if ((threadIdx.x & 63u) == 0) {
+            const unsigned k = list[i];
+            float y[DIM], xk[DIM];
 #pragma unroll
-    for (int j = 0; j < DIM; ++j) {
-        y[j] = Y[(size_t)k * DIM + j];
-        xk[j] = X[(size_t)k * DIM + j];
+            for (int j = 0; j < DIM; ++j) {
+                y[j] = Y[(size_t)k * DIM + j];
+                xk[j] = X[(size_t)k * DIM + j];
+            }
+            const double ly = bh_glm_logpost_sum(y, sy, lin64, 1.0, prior_prec);
+            const double lx = bh_glm_logpost_sum(xk, sx, lin64, 1.0, prior_prec);
+            flag[k] = bh_glm_decide(ly, lx, zbuf[k], u2buf[k], beta) ? 2u : 1u;
+        }
     }
-    const double ly = bh_glm_logpost(y, i, stride, partial64, nranges64, lin64, 1.0, prior_prec);
-    const double lx = bh_glm_logpost(xk, c + i, stride, partial64, nranges64, lin64, 1.0,
-                                     prior_prec);
-    flag[k] = bh_glm_decide(ly, lx, zbuf[k], u2buf[k], beta) ? 2u : 1u;
-    glp[k] = ly;
 }
 
 // Accept test of one half-step (amd-gcn-mcmc-stretch.cl:66-78) with accu !=
 // 0 also the accounting of stretch_move_accu.  With flag non-null, walkers
-// resolved in fp64 take that decision and lp(Y).
+// resolved in fp64 take that decision.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_accept(const float* __restrict__ Y, const float* __restrict__ zbuf,
               const float* __restrict__ u2buf, const double* __restrict__ partial,
               unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
               float* __restrict__ X, double* __restrict__ lp, float beta, unsigned K, int accu,
               unsigned* __restrict__ accept, float* __restrict__ means, unsigned mstep,
-              const unsigned* __restrict__ flag, const double* __restrict__ glp) {
+              const unsigned* __restrict__ flag) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float y[DIM], xk[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
+    // the stored lp stays in this sampler's precision in every case, so that
+    // its common offset from the fp64 values cancels in later differences
+    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
     const unsigned f = flag ? flag[k] : 0u;
-    double lpy;
-    bool accepted;
-    if (f == 1u || f == 2u) {
-        lpy = glp[k];
-        accepted = f == 2u;
-    } else {
-        lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
-        accepted = bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta);
-    }
+    const bool accepted = (f == 1u || f == 2u)
+                              ? f == 2u
+                              : bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta);
     if (accepted) {
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {

@@ -142,6 +142,16 @@ class HipBayaderaFactory(_Handle):
     def batch(self, model: DistributionModel, params, dims, hyper_len, walkers: int) -> "HipBatch":
         return HipBatch(self, model, params, dims, hyper_len, walkers)
 
+    def glm_batch(self, model, walkers: int) -> "HipGlmBatch":
+        return HipGlmBatch(self, model, walkers)
+
+    def fp8_decode(self, codes) -> np.ndarray:
+        """The device's e4m3 decoding of a byte array (length a multiple of 4)."""
+        c = np.ascontiguousarray(codes, np.uint8).reshape(-1)
+        out = np.zeros(c.size, np.float32)
+        call("bh_fp8_decode", self.handle, c.ctypes.data, out.ctypes.data, c.size)
+        return out
+
     def mcmc_factory(self, model) -> "HipStretchFactory":
         if isinstance(model, GlmModel):
             return HipGlmStretchFactory(self, model)
@@ -338,6 +348,10 @@ class HipGlmStretchFactory(HipStretchFactory):
         return out
 
 
+def _shape_size(x) -> int:
+    return int(x.numel()) if hasattr(x, "numel") else int(np.asarray(x).size)
+
+
 class HipBatch(_Handle):
     """Many independent posteriors of one model in one launch (no reference
     counterpart).  ``params``: one array per problem ([data, hyperparameters]
@@ -375,13 +389,31 @@ class HipBatch(_Handle):
         call("bh_batch_burn_in", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a))
         return self
 
-    def run(self, seed0: int, n: int, a: float) -> dict:
+    def run(self, seed0: int, n: int, a: float, series=None) -> dict:
+        """n accounted steps.  ``series``: optional (device or host) float
+        array of n * sum(dims) values that receives each problem's
+        per-step ensemble means, problem b's (n, dims[b]) block after those
+        of problems 0 .. b-1 (see ``series_blocks``)."""
         acc = np.zeros(self.problems, np.float64)
         mean = np.zeros((self.problems, self.max_dim), np.float64)
         sd = np.zeros((self.problems, self.max_dim), np.float64)
-        call("bh_batch_run", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a),
-             acc.ctypes.data, mean.ctypes.data, sd.ctypes.data)
+        if series is None:
+            call("bh_batch_run", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a),
+                 acc.ctypes.data, mean.ctypes.data, sd.ctypes.data)
+        else:
+            total = int(n) * int(self.dims.astype(np.int64).sum())
+            if _shape_size(series) < total:
+                raise ValueError(f"series holds fewer than {total} values")
+            ser = _Arr(series)
+            call("bh_batch_run_series", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a),
+                 acc.ctypes.data, mean.ctypes.data, sd.ctypes.data, ser.ptr)
         return {"acceptance-rate": acc, "mean": mean, "sd": sd}
+
+    def series_blocks(self, series, n: int):
+        """Problem b's (n, dims[b]) view of a recorded series."""
+        offs = np.concatenate([[0], np.cumsum(self.dims.astype(np.int64) * int(n))])
+        return [series[offs[b]:offs[b + 1]].reshape(int(n), int(d))
+                for b, d in enumerate(self.dims)]
 
     def state(self):
         xs = np.zeros((self.problems, self.walkers * self.max_dim), np.float32)
@@ -394,6 +426,84 @@ class HipBatch(_Handle):
     def last_gpu_ms(self) -> float:
         ms = ctypes.c_double()
         call("bh_batch_last_gpu_ms", self._h, ctypes.byref(ms))
+        return ms.value
+
+
+class HipGlmBatch(_Handle):
+    """Many independent logistic-regression posteriors (GlmBatchModel) in
+    lockstep, their likelihoods as one grouped contraction over fp8 data (no
+    reference counterpart)."""
+    _destroy = "bh_glm_batch_destroy"
+
+    def __init__(self, factory: HipBayaderaFactory, model, walkers: int):
+        super().__init__()
+        self.factory, self.dist_model = factory, model
+        self.problems = model.problems
+        self.walkers = int(walkers)
+        self.dims = model.dims
+        rows = model.rows
+        X = np.concatenate([np.ascontiguousarray(x, np.uint8).reshape(-1) for x in model.X])
+        sc = np.concatenate([np.asarray(s, np.float32) for s in model.scales])
+        y = np.concatenate(model.y).astype(np.float32)
+        kind = {"logistic": _lib.BH_GLM_LOGISTIC}[model.kind]
+        call("bh_glm_batch_create", factory.handle, kind, self.problems, rows.ctypes.data,
+             self.dims.ctypes.data, X.ctypes.data, sc.ctypes.data, y.ctypes.data,
+             float(model.prior_sd), self.walkers, ctypes.byref(self._h))
+
+    def set_precision(self, precision: int):
+        call("bh_glm_batch_set_precision", self._h, int(precision))
+        return self
+
+    def init_position(self, seed: int, limits):
+        """limits: (problems, 64, 2) or (problems, d, 2) with d <= 64."""
+        lim = np.zeros((self.problems, 64, 2), np.float32)
+        la = np.asarray(limits, np.float32)
+        lim[:, :la.shape[1]] = la
+        call("bh_glm_batch_init_position", self._h, int(seed) & 0xFFFFFFFF, lim.ctypes.data)
+        return self
+
+    def burn_in(self, seed0: int, n: int, a: float):
+        call("bh_glm_batch_burn_in", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a))
+        return self
+
+    def run(self, seed0: int, n: int, a: float, series=None) -> dict:
+        acc = np.zeros(self.problems, np.float64)
+        mean = np.zeros((self.problems, 64), np.float64)
+        sd = np.zeros((self.problems, 64), np.float64)
+        ptr = None
+        if series is not None:
+            total = int(n) * int(self.dims.astype(np.int64).sum())
+            if _shape_size(series) < total:
+                raise ValueError(f"series holds fewer than {total} values")
+            ser = _Arr(series)
+            ptr = ser.ptr
+        call("bh_glm_batch_run", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a),
+             acc.ctypes.data, mean.ctypes.data, sd.ctypes.data, ptr)
+        return {"acceptance-rate": acc, "mean": mean, "sd": sd}
+
+    def series_blocks(self, series, n: int):
+        offs = np.concatenate([[0], np.cumsum(self.dims.astype(np.int64) * int(n))])
+        return [series[offs[b]:offs[b + 1]].reshape(int(n), int(d))
+                for b, d in enumerate(self.dims)]
+
+    def state(self):
+        """(positions per problem (walkers, d_p), log posteriors (problems,
+        walkers) in double)."""
+        xs = np.zeros((self.problems, self.walkers, 64), np.float32)
+        lp = np.zeros((self.problems, self.walkers), np.float64)
+        call("bh_glm_batch_state_get", self._h, xs.ctypes.data, lp.ctypes.data)
+        return [xs[b, :, :d] for b, d in enumerate(self.dims)], lp
+
+    def log_density(self, precision: int) -> np.ndarray:
+        """Log posteriors of the current positions in `precision`
+        (problems, walkers)."""
+        lp = np.zeros((self.problems, self.walkers), np.float64)
+        call("bh_glm_batch_log_density", self._h, int(precision), lp.ctypes.data)
+        return lp
+
+    def last_gpu_ms(self) -> float:
+        ms = ctypes.c_double()
+        call("bh_glm_batch_last_gpu_ms", self._h, ctypes.byref(ms))
         return ms.value
 
 

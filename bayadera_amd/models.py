@@ -577,6 +577,97 @@ def from_bf16(b) -> np.ndarray:
     return (np.ascontiguousarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
+def _fp8_e4m3_table() -> np.ndarray:
+    """The 256 e4m3 values (OCP: bias 7, subnormals, no infinities,
+    S.1111.111 is NaN)."""
+    c = np.arange(256)
+    s = np.where(c >= 128, -1.0, 1.0)
+    e = (c >> 3) & 15
+    m = c & 7
+    v = np.where(e == 0, m * 2.0 ** -9, (1.0 + m / 8.0) * 2.0 ** (e - 7.0))
+    v = np.where((e == 15) & (m == 7), np.nan, v)
+    return (s * v).astype(np.float32)
+
+
+FP8_E4M3_VALUES = _fp8_e4m3_table()
+FP8_E4M3_MAX = 448.0
+
+
+def to_fp8_e4m3(a) -> np.ndarray:
+    """float -> e4m3 codes (uint8), round to nearest with ties to the even
+    code, saturating at +-448.  With a = |x| in binade 2^e (e clamped to
+    [-6, 8]) the e4m3 grid step is 2^(e-3), so the rounded magnitude is
+    rint(a / 2^(e-3)) 2^(e-3) (rint: ties to even, which is the even code)."""
+    x = np.asarray(a, np.float64)
+    mag = np.minimum(np.abs(x), FP8_E4M3_MAX)
+    _, E = np.frexp(mag)                      # mag = f 2^E, f in [0.5, 1)
+    e = np.clip(E - 1, -6, 8)
+    step = np.ldexp(1.0, e - 3)
+    v = np.rint(mag / step) * step            # exact: division by a power of two
+    _, E2 = np.frexp(v)
+    e2 = np.clip(E2 - 1, -6, 8)
+    normal = v >= 2.0 ** -6
+    mant = np.where(normal, np.rint(np.ldexp(v, -e2) * 8.0) - 8.0, np.rint(v * 2.0 ** 9))
+    code = np.where(normal, ((e2 + 7) << 3) + mant, mant).astype(np.uint8)
+    return np.where(x < 0, code | 0x80, code).astype(np.uint8)
+
+
+def from_fp8_e4m3(codes) -> np.ndarray:
+    return FP8_E4M3_VALUES[np.asarray(codes, np.uint8)]
+
+
+@dataclass
+class GlmBatchModel:
+    """Many independent logistic-regression posteriors (the batched GLM
+    runner): per problem X_p quantised to fp8 e4m3 codes (N_p x d_p, uint8)
+    with one fp32 scale per column, labels y_p in {0, 1}, prior
+    beta ~ N(0, prior_sd^2 I)."""
+    X: list
+    scales: list
+    y: list
+    prior_sd: float
+    kind: str = "logistic"
+
+    @property
+    def problems(self) -> int:
+        return len(self.X)
+
+    @property
+    def dims(self) -> np.ndarray:
+        return np.array([x.shape[1] for x in self.X], np.uint32)
+
+    @property
+    def rows(self) -> np.ndarray:
+        return np.array([x.shape[0] for x in self.X], np.uint64)
+
+    def dequantized(self, p: int) -> np.ndarray:
+        """x_ij = scale_j * fp8(q_ij) in float64, the data the posterior of
+        problem p is defined on."""
+        return (from_fp8_e4m3(self.X[p]).astype(np.float64)
+                * np.asarray(self.scales[p], np.float32).astype(np.float64)[None, :])
+
+
+def quantize_fp8_columns(X) -> tuple[np.ndarray, np.ndarray]:
+    """Per-column fp32 scales max|x_j| / 448 (1 for an all-zero column) and
+    the e4m3 codes of x / scale."""
+    X = np.asarray(X, np.float64)
+    m = np.abs(X).max(axis=0)
+    sc = np.where(m > 0, m / FP8_E4M3_MAX, 1.0).astype(np.float32)
+    return to_fp8_e4m3(X / sc.astype(np.float64)[None, :]), sc
+
+
+def glm_logistic_batch_model(Xs, ys, prior_sd: float = 5.0) -> GlmBatchModel:
+    """From float data (each X_p N_p x d_p): per-column fp8 quantisation."""
+    q, sc = zip(*(quantize_fp8_columns(x) for x in Xs))
+    yv = [np.ascontiguousarray(y, np.float32).reshape(-1) for y in ys]
+    for x, y in zip(q, yv):
+        if x.shape[0] != y.size:
+            raise ValueError("y must have one label per row of X")
+        if not 1 <= x.shape[1] <= 64:
+            raise ValueError("batched GLM problems have 1 to 64 coefficients")
+    return GlmBatchModel(list(q), list(sc), yv, float(prior_sd))
+
+
 def hbb_model(max_groups: int = 50) -> DistributionModel:
     """Hierarchical beta-binomial with up to max_groups groups (dimension
     max_groups + 2; each batched problem has its own G)."""

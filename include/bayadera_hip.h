@@ -276,11 +276,55 @@ int bh_batch_burn_in(bh_batch* b, uint32_t seed0, uint64_t n, float a);
  * (problems x max_dim; NULL to skip). */
 int bh_batch_run(bh_batch* b, uint32_t seed0, uint64_t n, float a, double* acceptance_rate,
                  double* mean, double* sd);
+/* bh_batch_run that also records each step's ensemble mean of every problem
+ * (the series acor takes, amd_gcn.clj:437-439 for one sampler): problem p's
+ * n x dims[p] block (row-major, step by step) starts at
+ * series + n * (dims[0] + .. + dims[p-1]).  series is a device or host
+ * pointer; n < 2^20. */
+int bh_batch_run_series(bh_batch* b, uint32_t seed0, uint64_t n, float a,
+                        double* acceptance_rate, double* mean, double* sd, float* series);
 /* positions (problems x walkers x max_dim slots, each problem packed by its
  * own dim) and logfn values (problems x walkers) */
 int bh_batch_state_get(const bh_batch* b, float* xs, float* logfn);
 /* GPU time of the last burn-in or run call (HIP events on the batch's stream) */
 int bh_batch_last_gpu_ms(const bh_batch* b, double* ms);
+
+/* ---- Batched GLM posteriors (no reference counterpart: many independent
+ * logistic regressions, each with its own rows N_p and dimension d_p <= 64,
+ * advanced in lockstep; the likelihood of every problem is one grouped
+ * contraction).  X is fp8 e4m3 (OCP: bias 7, max 448, S.1111.111 NaN) with
+ * one fp32 scale per column: x_ij = scale_j * fp8(q_ij).  Arguments are
+ * concatenated over problems: X_fp8 the N_p x d_p row-major bytes, col_scales
+ * the d_p scales, y the N_p labels (0/1).  Prior beta ~ N(0, prior_sd^2).
+ * Each problem runs the single GLM sampler's moves (move-bare! arithmetic,
+ * amd-gcn-mcmc-stretch.cl:31-81) with whole-walker partners and seeds
+ * seed0 + 2p / seed0 + 2p + 1 for its two halves.  walkers: a multiple of
+ * 512.  Precision as bh_stretch_set_glm_precision (BH_GLM_*). */
+typedef struct bh_glm_batch bh_glm_batch;
+int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t* rows,
+                        const uint32_t* dims, const void* X_fp8, const float* col_scales,
+                        const float* y, float prior_sd, uint32_t walkers, bh_glm_batch** out);
+int bh_glm_batch_destroy(bh_glm_batch* b);
+int bh_glm_batch_set_precision(bh_glm_batch* b, int precision);
+/* limits: problems x 64 x 2 (lo, hi per coordinate; rows past d_p unused);
+ * problem p's walkers from init_walkers' stream with seed + p
+ * (amd-gcn-mcmc-stretch.cl:138-159), then their log posteriors. */
+int bh_glm_batch_init_position(bh_glm_batch* b, uint32_t seed, const float* limits);
+int bh_glm_batch_burn_in(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a);
+/* n accounted steps: per problem the acceptance rate and, over the steps and
+ * walkers, the mean and sd of every coordinate (problems x 64), and when
+ * series is not NULL each step's ensemble means, problem p's n x d_p block
+ * at series + n * (d_0 + .. + d_{p-1}). */
+int bh_glm_batch_run(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a,
+                     double* acceptance_rate, double* mean, double* sd, float* series);
+/* positions (problems x walkers x 64, padded) and log posteriors (double) */
+int bh_glm_batch_state_get(const bh_glm_batch* b, float* xs, double* lp);
+/* Log posteriors of the current positions in the given precision
+ * (problems x walkers, double); the sampler's own values are kept. */
+int bh_glm_batch_log_density(bh_glm_batch* b, int precision, double* lp);
+int bh_glm_batch_last_gpu_ms(const bh_glm_batch* b, double* ms);
+/* The device's fp8 e4m3 decoding of n bytes (n a multiple of 4). */
+int bh_fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n);
 
 /* Sampler state for resume: positions (DIM x W), logfn (W), and the host
  * counters {bare seed, move seed, bare counter, move counter, iteration

@@ -95,7 +95,7 @@ def test_glm_walker_rule(fac):
         sf.create_sampler(1, 768, np.zeros(1, np.float32))
 
 
-def _coupled_run(sf, prec, guard, d, W=2048, burn=25, run=25):
+def _coupled_run(sf, prec, guard, d, W=2048, burn=25, run=60):
     s = sf.create_sampler(21, W, np.zeros(1, np.float32)).set_glm_precision(prec)
     s.set_partner(_lib.BH_PARTNER_WALKER)
     if guard:
@@ -115,12 +115,15 @@ def test_glm_guard_makes_the_fp64_decisions(fac, prec):
     """Same seeds, same initial walkers: with the guarded accept test the
     reduced-precision sampler's trajectory is the fp64 sampler's, position
     for position, and its accounting matches."""
-    N, d = 65536, 64
+    # at this size the reduced precisions' log posteriors sit ~0.005 above
+    # the fp64 ones (a common offset that cancels only between values of the
+    # same precision) with ~5e-4 of scatter
+    N, d = 262144, 64
     Xb, Xd, y = _problem(N, d, 17)
     sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
     x64, r64, s64 = _coupled_run(sf, _lib.BH_GLM_FP64, 0.0, d)
     assert s64 == {"listed": 0, "overflow": 0}
-    xg, rg, sg = _coupled_run(sf, prec, 2e-3, d)
+    xg, rg, sg = _coupled_run(sf, prec, 5e-3, d)
     print("guard stats", sg)
     assert sg["overflow"] == 0
     assert np.array_equal(xg, x64)
