@@ -26,6 +26,7 @@ BH_EXEC_AUTO = 0
 BH_EXEC_STEPWISE = 1
 BH_EXEC_RESIDENT = 2
 BH_GLM_LOGISTIC = 1
+BH_GLM_GAUSSIAN = 2
 BH_GLM_FP64 = 0
 BH_GLM_FP32 = 1
 BH_GLM_BF16 = 2
@@ -67,6 +68,7 @@ SIGNATURES = {
     "bh_stretch_factory_create": [_P, _S, _S, _U32, _U32, _PP, _P, _SZ],
     "bh_stretch_factory_destroy": [_P],
     "bh_glm_stretch_factory_create": [_P, _I, _P, _P, _U64, _U32, _F, _PP, _P, _SZ],
+    "bh_glm_gaussian_factory_create": [_P, _P, _P, _U64, _U32, _F, _F, _PP, _P, _SZ],
     "bh_glm_log_density": [_P, _P, _U64, _I, _P],
     "bh_stretch_set_glm_precision": [_P, _I],
     "bh_stretch_set_glm_guard": [_P, ctypes.c_double],

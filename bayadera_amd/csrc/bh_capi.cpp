@@ -267,16 +267,35 @@ struct GlmData {
     int kind = 0;
     unsigned long long N = 0;
     uint32_t d = 0;
-    double prior_prec = 1.0;  // 1 / prior_sd^2
-    DevBuf X;                 // bf16, N x d row-major
-    DevBuf lin;               // double[d]: X'y
-    DevBuf lin_half;          // double[d]: X'y - (1/2) X'1, for the log2-unit kernels
+    uint32_t D = 0;           // linear regression: padded column count of Xf
+    double prior_prec = 1.0;  // logistic: 1 / prior_sd^2
+    DevBuf X;                 // logistic: bf16, N x d row-major
+    DevBuf lin;               // logistic: double[d]: X'y; linear regression: the
+                              // four constants of its log posterior
+    DevBuf lin_half;          // logistic: double[d]: X'y - (1/2) X'1 (log2-unit kernels)
+    DevBuf Xf, y;             // linear regression: fp32 N x D and N
 };
 
-// The linear coefficients of a precision's log-likelihood: the fp64 kernel
-// sums whole softplus values, the others leave out u/2 per row.
+// The linear coefficients of a precision's log-likelihood: the logistic
+// fp64 kernel sums whole softplus values, the others leave out u/2 per row.
 const double* glm_lin(const GlmData& g, int prec) {
+    if (g.kind == BH_GLM_GAUSSIAN) return g.lin.as<double>();
     return (prec == BH_GLM_FP64 ? g.lin : g.lin_half).as<double>();
+}
+
+// The likelihood contraction of n points P (row stride = the walker
+// dimension) into partial[nranges][n].
+int glm_contract(const GlmData& g, int prec, hipStream_t st, const float* P, unsigned pstride,
+                 const void* ys, unsigned n, unsigned nranges, unsigned long long rpr,
+                 float yscale, double* partial) {
+    if (g.kind == BH_GLM_GAUSSIAN) {
+        BH_HIP(bh::launch_glm_gauss(prec, st, g.Xf.as<float>(), g.y.as<float>(), g.N, g.D, P,
+                                    pstride, g.d, n, nranges, rpr, partial));
+    } else {
+        BH_HIP(bh::launch_glm_logistic(prec, st, g.X.p, g.N, g.d, P, ys, n, nranges, rpr, yscale,
+                                       partial));
+    }
+    return BH_OK;
 }
 
 struct bh_stretch_factory {
@@ -461,8 +480,7 @@ int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
         BH_TRY(launch(f->glm_split, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a));
     }
     double* partial = s->gpartial.as<double>();
-    BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, P, ys, n, nranges, rpr,
-                                   yscale, partial));
+    BH_TRY(glm_contract(g, prec, s->stream, P, s->dim, ys, n, nranges, rpr, yscale, partial));
     const double* lin = glm_lin(g, prec);
     double scale = prec == BH_GLM_FP64 ? 1.0 : kLn2;
     double pp = g.prior_prec;
@@ -499,8 +517,8 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
     double* partial = s->gpartial.as<double>();
     BH_TRY(ktimed(s, [&]() -> int {
-        BH_HIP(bh::launch_glm_logistic(prec, s->stream, g.X.p, g.N, g.d, Y, ys, K, nranges, rpr,
-                                       yscale, partial));
+        BH_TRY(glm_contract(g, prec, s->stream, Y, s->dim, ys, K, nranges, rpr, yscale,
+                            partial));
         return BH_OK;
     }));
     const double* lin = glm_lin(g, prec);
@@ -1149,11 +1167,66 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
     return BH_OK;
 }
 
+int bh_glm_gaussian_factory_create(bh_ctx* ctx, const float* X, const float* y, uint64_t N,
+                                   uint32_t d, float beta_sd, float log_sigma_sd,
+                                   bh_stretch_factory** out, char* log, size_t log_size) {
+    BH_ARG(ctx && X && y && out, "NULL argument");
+    BH_ARG(d >= 1 && d <= 64, "regression dimension %u outside [1, 64]", d);
+    BH_ARG(N > 0 && N < (1ull << 31), "bad row count");
+    BH_ARG(beta_sd > 0.0f && log_sigma_sd > 0.0f, "prior sds must be positive");
+    BH_HIP(hipSetDevice(ctx->device));
+    std::vector<char> code;
+    BH_TRY(rtc_compile(ctx->arch, "",
+                       {"-DWGS=" + std::to_string(ctx->wgs), "-DDIM=" + std::to_string(d + 1),
+                        "-DBH_GLM=1", "-DBH_GLM_KIND=2"},
+                       code, log, log_size));
+    auto f = std::make_unique<bh_stretch_factory>();
+    f->ctx = ctx;
+    f->dim = d + 1;
+    f->params_len = 0;
+    BH_TRY(f->mod.load(code));
+    BH_TRY(f->mod.fn("bh_glm_propose", &f->glm_propose));
+    BH_TRY(f->mod.fn("bh_glm_split_points", &f->glm_split));
+    BH_TRY(f->mod.fn("bh_glm_logpost", &f->glm_logpost));
+    BH_TRY(f->mod.fn("bh_glm_accept", &f->glm_accept));
+    BH_TRY(f->mod.fn("bh_glm_screen", &f->glm_screen));
+    BH_TRY(f->mod.fn("bh_glm_resolve", &f->glm_resolve));
+    auto g = std::make_unique<GlmData>();
+    g->kind = BH_GLM_GAUSSIAN;
+    g->N = N;
+    g->d = d;
+    g->D = d <= 16 ? 16 : (d <= 32 ? 32 : 64);
+    std::vector<float> xp((size_t)N * g->D, 0.0f);
+    const float* xh = X;
+    std::vector<float> xcopy;
+    if (is_device_ptr(X)) {
+        xcopy.resize((size_t)N * d);
+        BH_HIP(hipMemcpy(xcopy.data(), X, xcopy.size() * 4, hipMemcpyDeviceToHost));
+        xh = xcopy.data();
+    }
+    for (uint64_t i = 0; i < N; ++i)
+        std::memcpy(&xp[(size_t)i * g->D], xh + (size_t)i * d, (size_t)d * 4);
+    BH_TRY(g->Xf.alloc(xp.size() * 4));
+    BH_HIP(hipMemcpy(g->Xf.p, xp.data(), xp.size() * 4, hipMemcpyHostToDevice));
+    BH_TRY(g->y.alloc((size_t)N * 4));
+    BH_HIP(hipMemcpy(g->y.p, y, (size_t)N * 4, hipMemcpyDefault));
+    const double c[4] = {(double)N, 0.5 * (double)N * std::log(2.0 * M_PI),
+                         1.0 / ((double)beta_sd * beta_sd),
+                         1.0 / ((double)log_sigma_sd * log_sigma_sd)};
+    BH_TRY(g->lin.alloc(sizeof(c)));
+    BH_HIP(hipMemcpy(g->lin.p, c, sizeof(c), hipMemcpyHostToDevice));
+    f->glm = std::move(g);
+    *out = f.release();
+    return BH_OK;
+}
+
 int bh_glm_log_density(bh_stretch_factory* f, const float* beta, uint64_t n, int precision,
                        double* out) {
     BH_ARG(f && beta && out, "NULL argument");
     BH_ARG(f->glm, "not a GLM factory");
     BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    BH_ARG(f->glm->kind != BH_GLM_GAUSSIAN || precision <= BH_GLM_FP32,
+           "linear-regression models run in fp64 or fp32");
     BH_ARG(n > 0 && n < (1ull << 26), "bad point count");
     bh_ctx* c = f->ctx;
     BH_HIP(hipSetDevice(c->device));
@@ -1202,6 +1275,7 @@ int bh_stretch_create(bh_stretch_factory* f, uint32_t seed, uint32_t walkers,
     s->data_len = params_count > f->params_len ? params_count - f->params_len : 0u;
     BH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     if (f->glm) {
+        if (f->glm->kind == BH_GLM_GAUSSIAN) s->glm_prec = BH_GLM_FP32;
         BH_TRY(glm_alloc(s.get(), walkers, walkers / 2));
         BH_TRY(s->lp.alloc((size_t)walkers * sizeof(double)));
         BH_HIP(hipMemsetAsync(s->lp.p, 0, (size_t)walkers * sizeof(double), s->stream));
@@ -1300,6 +1374,8 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
     BH_ARG(s, "NULL sampler");
     BH_ARG(s->f->glm, "not a GLM sampler");
     BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    BH_ARG(s->f->glm->kind != BH_GLM_GAUSSIAN || precision <= BH_GLM_FP32,
+           "linear-regression samplers run in fp64 or fp32");
     s->glm_prec = precision;
     return BH_OK;
 }
@@ -1307,6 +1383,7 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
 int bh_stretch_set_glm_guard(bh_stretch* s, double guard) {
     BH_ARG(s, "NULL sampler");
     BH_ARG(s->f->glm, "not a GLM sampler");
+    BH_ARG(s->f->glm->kind == BH_GLM_LOGISTIC, "the guarded accept test is for logistic GLMs");
     BH_ARG(guard >= 0.0 && std::isfinite(guard), "guard must be finite and >= 0");
     s->glm_guard = guard;
     return BH_OK;
@@ -1784,8 +1861,8 @@ int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t
     BH_TRY(b->ysplit.alloc((size_t)3 * P * W * 64 * 2));
     BH_TRY(b->partial.alloc((size_t)std::max(uid, 1u) * W * 8));
     BH_TRY(b->acc_cnt.alloc((size_t)P * 8));
-    BH_TRY(b->msum.alloc((size_t)P * 8 * 64 * 4));
-    BH_TRY(b->msum2.alloc((size_t)P * 8 * 64 * 4));
+    BH_TRY(b->msum.alloc((size_t)P * (W / 256) * 64 * 4));
+    BH_TRY(b->msum2.alloc((size_t)P * (W / 256) * 64 * 4));
     BH_TRY(b->acc1.alloc((size_t)P * 64 * 8));
     BH_TRY(b->acc2.alloc((size_t)P * 64 * 8));
     BH_HIP(hipMemsetAsync(b->xs.p, 0, (size_t)P * W * 64 * 4, b->stream));

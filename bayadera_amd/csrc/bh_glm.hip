@@ -198,6 +198,107 @@ glm_logistic_fp64_kernel(const __bf16* __restrict__ X, unsigned long long N,
     partial[(size_t)range * n + k] = acc;
 }
 
+// Linear regression (Gaussian likelihood with unknown sigma): for each row
+// range r and point k (coefficients b_k = the first d of its DIM = d + 1
+// coordinates, row stride pstride),
+//   partial[r][k] = sum_{i in r} (y_i - x_i . b_k)^2,
+// X fp32 N x D (columns past d zero), y fp32.  The plain fp32 kernel: one
+// point per thread, tiles of 32 rows of X and y in LDS read as broadcasts,
+// fp32 residuals, squares summed four rows at a time into a double.
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_gauss_fp32_kernel(const float* __restrict__ X, const float* __restrict__ yv,
+                      unsigned long long N, const float* __restrict__ P, unsigned pstride,
+                      unsigned d, unsigned n, unsigned groups, unsigned nranges,
+                      unsigned long long rows_per_range, double* __restrict__ partial) {
+    constexpr int TR = 32;
+    __shared__ float4 xs[TR * D / 4];
+    __shared__ float ys[TR];
+    unsigned group, range;
+    glm_tile(groups, group, range);
+    if (range >= nranges) return;
+    const unsigned k = group * 256u + threadIdx.x;
+    float b[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) b[j] = (unsigned)j < d ? P[(size_t)k * pstride + j] : 0.0f;
+    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
+    unsigned long long r1 = r0 + rows_per_range;
+    if (r1 > N) r1 = N;
+    double acc = 0.0;
+    for (unsigned long long rb = r0; rb < r1; rb += TR) {
+        __syncthreads();
+        for (unsigned e = threadIdx.x; e < TR * D / 4; e += 256) {
+            const unsigned long long row = rb + (4 * e) / D;
+            xs[e] = row < r1 ? *reinterpret_cast<const float4*>(X + rb * D + 4 * e)
+                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (threadIdx.x < TR) ys[threadIdx.x] = rb + threadIdx.x < r1 ? yv[rb + threadIdx.x] : 0.0f;
+        __syncthreads();
+        // rows past r1 are zero with y = 0: residual 0
+        for (int r = 0; r < TR; r += 4) {
+            float s4 = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float t = 0.0f;
+#pragma unroll
+                for (int j4 = 0; j4 < D / 4; ++j4) {
+                    const float4 x = xs[(r + i) * (D / 4) + j4];
+                    t = __builtin_fmaf(x.x, b[4 * j4 + 0], t);
+                    t = __builtin_fmaf(x.y, b[4 * j4 + 1], t);
+                    t = __builtin_fmaf(x.z, b[4 * j4 + 2], t);
+                    t = __builtin_fmaf(x.w, b[4 * j4 + 3], t);
+                }
+                const float res = ys[r + i] - t;
+                s4 = __builtin_fmaf(res, res, s4);
+            }
+            acc += (double)s4;
+        }
+    }
+    partial[(size_t)range * n + k] = acc;
+}
+
+// fp64 reference of the same sums: exact products of fp32 data and
+// coefficients, double residuals and squares.
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_gauss_fp64_kernel(const float* __restrict__ X, const float* __restrict__ yv,
+                      unsigned long long N, const float* __restrict__ P, unsigned pstride,
+                      unsigned d, unsigned n, unsigned groups, unsigned nranges,
+                      unsigned long long rows_per_range, double* __restrict__ partial) {
+    constexpr int TR = 16;
+    __shared__ double xs[TR * D];
+    __shared__ double ys[TR];
+    unsigned group, range;
+    glm_tile(groups, group, range);
+    if (range >= nranges) return;
+    const unsigned k = group * 256u + threadIdx.x;
+    double b[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) b[j] = (unsigned)j < d ? (double)P[(size_t)k * pstride + j] : 0.0;
+    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
+    unsigned long long r1 = r0 + rows_per_range;
+    if (r1 > N) r1 = N;
+    double acc = 0.0;
+    for (unsigned long long rb = r0; rb < r1; rb += TR) {
+        __syncthreads();
+        for (unsigned e = threadIdx.x; e < TR * D; e += 256) {
+            const unsigned long long row = rb + e / D;
+            xs[e] = row < r1 ? (double)X[rb * D + e] : 0.0;
+        }
+        if (threadIdx.x < TR) ys[threadIdx.x] = rb + threadIdx.x < r1 ? (double)yv[rb + threadIdx.x] : 0.0;
+        __syncthreads();
+        const int rows = (int)((r1 - rb) < TR ? (r1 - rb) : TR);
+        for (int r = 0; r < rows; ++r) {
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) t = __builtin_fma(xs[r * D + j], b[j], t);
+            const double res = ys[r] - t;
+            acc = __builtin_fma(res, res, acc);
+        }
+    }
+    partial[(size_t)range * n + k] = acc;
+}
+
 // fp64 softplus sums of listed points (the guarded accept test's fallback).
 // Block (r, s) owns row range r and runs points p = s, s + gridDim.y, ...
 // of the 2c listed ones (c = min(*count, lmax)); its 256 threads take the
@@ -350,6 +451,34 @@ hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsi
     if (d == 64) return launch_d<64>(precision, s, x, N, P, y, n, nranges, rows_per_range, yscale, partial);
     if (d == 32) return launch_d<32>(precision, s, x, N, P, y, n, nranges, rows_per_range, yscale, partial);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_glm_gauss(int precision, hipStream_t s, const float* X, const float* yv,
+                            unsigned long long N, unsigned D, const float* P, unsigned pstride,
+                            unsigned d, unsigned n, unsigned nranges,
+                            unsigned long long rows_per_range, double* partial) {
+    const unsigned groups = n / 256;
+    const unsigned blocks = groups * nranges;
+#define BH_GAUSS(DD)                                                                           \
+    if (precision == kGlmFp32)                                                                 \
+        glm_gauss_fp32_kernel<DD><<<blocks, 256, 0, s>>>(X, yv, N, P, pstride, d, n, groups,    \
+                                                         nranges, rows_per_range, partial);    \
+    else if (precision == kGlmFp64)                                                            \
+        glm_gauss_fp64_kernel<DD><<<blocks, 256, 0, s>>>(X, yv, N, P, pstride, d, n, groups,    \
+                                                         nranges, rows_per_range, partial);    \
+    else                                                                                       \
+        return hipErrorInvalidValue;
+    if (D == 16) {
+        BH_GAUSS(16)
+    } else if (D == 32) {
+        BH_GAUSS(32)
+    } else if (D == 64) {
+        BH_GAUSS(64)
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef BH_GAUSS
+    return hipGetLastError();
 }
 
 hipError_t launch_glm_lin(hipStream_t s, const void* X, const float* yv, unsigned long long N,

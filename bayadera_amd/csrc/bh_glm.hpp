@@ -28,6 +28,13 @@ hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long
                                 const float* Y, const float* W, const unsigned* list,
                                 const unsigned* count, unsigned lmax, unsigned nranges,
                                 double* partial64);
+// Linear regression sums of squared residuals (fp32 or fp64 kernels): X fp32
+// N x D (D = 16, 32 or 64, columns past d zero), points P with row stride
+// pstride whose first d coordinates are the coefficients.
+hipError_t launch_glm_gauss(int precision, hipStream_t s, const float* X, const float* yv,
+                            unsigned long long N, unsigned D, const float* P, unsigned pstride,
+                            unsigned d, unsigned n, unsigned nranges,
+                            unsigned long long rows_per_range, double* partial);
 hipError_t launch_glm_lin(hipStream_t s, const void* X, const float* yv, unsigned long long N,
                           unsigned d, double* part, unsigned chunks, double* lin);
 

@@ -357,8 +357,8 @@ gb_logpost_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dim
 // 66-78 with z^(d_p - 1) by repeated multiplication), and with accu != 0
 // the accounting: accepted walkers per problem (acc_cnt), and per 256-walker
 // block the sums of x and x^2 of every coordinate over the block's walkers
-// after the move (msum/msum2[p][slot][64], slot = half * 4 + block; a fixed
-// shuffle-and-LDS tree).
+// after the move (msum/msum2[p][slot][64], slot = half * K/256 + block of the
+// W/256 slots; a fixed shuffle-and-LDS tree).
 __global__ void __launch_bounds__(256)
 gb_accept_kernel(int half, const unsigned* __restrict__ dims, const unsigned* __restrict__ ubeg,
                  const float* __restrict__ Y, const float* __restrict__ zbuf,
@@ -407,18 +407,20 @@ gb_accept_kernel(int half, const unsigned* __restrict__ dims, const unsigned* __
     __syncthreads();
     if (threadIdx.x == 0)
         atomicAdd(&acc_cnt[p], (unsigned long long)((cnt[0] + cnt[1]) + (cnt[2] + cnt[3])));
-    const unsigned slot = (half ? 4u : 0u) + blockIdx.x;
+    const unsigned nslots = 2 * gridDim.x;
+    const unsigned slot = (half ? gridDim.x : 0u) + blockIdx.x;
     if (threadIdx.x < d) {
         const unsigned i = threadIdx.x;
-        msum[((size_t)p * 8 + slot) * kGbD + i] = (red[0][0][i] + red[0][1][i]) + (red[0][2][i] + red[0][3][i]);
-        msum2[((size_t)p * 8 + slot) * kGbD + i] = (red[1][0][i] + red[1][1][i]) + (red[1][2][i] + red[1][3][i]);
+        const size_t o = ((size_t)p * nslots + slot) * kGbD + i;
+        msum[o] = (red[0][0][i] + red[0][1][i]) + (red[0][2][i] + red[0][3][i]);
+        msum2[o] = (red[1][0][i] + red[1][1][i]) + (red[1][2][i] + red[1][3][i]);
     }
 }
 
 // End of an accounted step: per problem and coordinate, the step's
-// ensemble sums (slots in a fixed order) into the double accumulators, and
-// the step's ensemble mean into the recorded series (problem p's n x d_p
-// block at series + soff[p]) when there is one.
+// ensemble sums (its W/256 slots, in slot order) into the double
+// accumulators, and the step's ensemble mean into the recorded series
+// (problem p's n x d_p block at series + soff[p]) when there is one.
 __global__ void __launch_bounds__(64)
 gb_step_kernel(const unsigned* __restrict__ dims, const float* __restrict__ msum,
                const float* __restrict__ msum2, unsigned W, double* __restrict__ acc1,
@@ -427,12 +429,14 @@ gb_step_kernel(const unsigned* __restrict__ dims, const float* __restrict__ msum
     const unsigned p = blockIdx.x;
     const unsigned i = threadIdx.x;
     if (i >= dims[p]) return;
-    const float* m1 = msum + (size_t)p * 8 * kGbD + i;
-    const float* m2 = msum2 + (size_t)p * 8 * kGbD + i;
-    const float s1 = ((m1[0] + m1[kGbD]) + (m1[2 * kGbD] + m1[3 * kGbD])) +
-                     ((m1[4 * kGbD] + m1[5 * kGbD]) + (m1[6 * kGbD] + m1[7 * kGbD]));
-    const float s2 = ((m2[0] + m2[kGbD]) + (m2[2 * kGbD] + m2[3 * kGbD])) +
-                     ((m2[4 * kGbD] + m2[5 * kGbD]) + (m2[6 * kGbD] + m2[7 * kGbD]));
+    const unsigned nslots = W / 256;
+    const float* m1 = msum + (size_t)p * nslots * kGbD + i;
+    const float* m2 = msum2 + (size_t)p * nslots * kGbD + i;
+    float s1 = 0.0f, s2 = 0.0f;
+    for (unsigned sl = 0; sl < nslots; ++sl) {
+        s1 += m1[(size_t)sl * kGbD];
+        s2 += m2[(size_t)sl * kGbD];
+    }
     acc1[(size_t)p * kGbD + i] += (double)s1;
     acc2[(size_t)p * kGbD + i] += (double)s2;
     if (series) series[soff[p] + (size_t)st * dims[p] + i] = s1 / (float)W;

@@ -65,11 +65,21 @@ __device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double 
     }
 }
 
-// One piece of the epilogue of four values (one column block): six pieces
-// per block, each a few VALU instructions, so that they can be placed
-// between matrix-core products.  State carried between pieces in e/q.
-__device__ __forceinline__ void glm_epilogue_piece(int ph, const f32x4& c, float (&e)[4],
-                                                   float& q, double& acc) {
+// The epilogue of two consecutive 16-row blocks, eight values per lane and
+// column block, with one v_log_f32 for the eight: the product of the eight
+// factors 1 + 2^-|u| lies in [1, 256].  It is cut into pieces of a few VALU
+// instructions so that they can be placed between matrix-core products:
+// five per column block for the first block of the pair (exponentials, the
+// product and |u| sum carried in st), seven for the second (the same, then
+// the logarithm and the double accumulation).
+struct GlmEpiState {
+    float p[4];  // product of the first block's four factors, per column block
+    float a[4];  // sum of its four |u|
+};
+
+template <int H>
+__device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& c, float (&e)[4],
+                                                   GlmEpiState& st, double& acc) {
     switch (ph) {
     case 0:
         e[0] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[0]));
@@ -84,36 +94,41 @@ __device__ __forceinline__ void glm_epilogue_piece(int ph, const f32x4& c, float
         e[2] = (1.0f + e[2]) * (1.0f + e[3]);
         break;
     case 3:
-        q = __builtin_amdgcn_logf(e[0] * e[2]);
+        if (H == 0) st.p[wb] = e[0] * e[2];
+        else st.p[wb] = st.p[wb] * (e[0] * e[2]);
         break;
     case 4:
         e[1] = (__builtin_fabsf(c[0]) + __builtin_fabsf(c[1])) +
                (__builtin_fabsf(c[2]) + __builtin_fabsf(c[3]));
-        q = __builtin_fmaf(0.5f, e[1], q);
+        if (H == 0) st.a[wb] = e[1];
+        else st.a[wb] = st.a[wb] + e[1];
+        break;
+    case 5:
+        e[3] = __builtin_fmaf(0.5f, st.a[wb], __builtin_amdgcn_logf(st.p[wb]));
         break;
     default:
-        acc += (double)q;
+        acc += (double)e[3];
         break;
     }
 }
 
 // The products of one 16-row block into cn (zeroed by the first product of
-// each chain) with the epilogue of the previous block's c spread between
-// them: products of the four column blocks alternate, so consecutive
-// products are independent, and scheduling barriers keep each epilogue
-// piece between two products, where it runs on the VALU while the matrix
-// core works.  The per-column-block product order is that of
+// each chain) with half H of the pair epilogue of the previous block's c
+// spread between them: products of the four column blocks alternate, so
+// consecutive products are independent, and scheduling barriers keep each
+// epilogue piece between two products, where it runs on the VALU while the
+// matrix core works.  The per-column-block product order is that of
 // glm_mfma_block (split terms smallest first, chunks in order).
-template <int D, int NS>
+template <int D, int NS, int H>
 __device__ __forceinline__ void glm_mfma_overlap(const bf16x8 (&a)[D / 32],
                                                  const bf16x8 (&bfrag)[4][D / 32][NS],
                                                  f32x4 (&cn)[4], const f32x4 (&c)[4],
-                                                 double (&acc)[4]) {
+                                                 double (&acc)[4], GlmEpiState& st) {
     constexpr int KC = D / 32;
     constexpr int NM = 4 * KC * NS;
-    constexpr int NP = 24;  // epilogue pieces: 6 per column block
+    constexpr int PW = H == 0 ? 5 : 7;  // pieces per column block
+    constexpr int NP = 4 * PW;
     float e[4][4];
-    float q[4];
     int m = 0;
 #pragma unroll
     for (int s = NS - 1; s >= 0; --s)
@@ -128,10 +143,21 @@ __device__ __forceinline__ void glm_mfma_overlap(const bf16x8 (&a)[D / 32],
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int pc = m * NP / NM; pc < (m + 1) * NP / NM; ++pc)
-                    glm_epilogue_piece(pc % 6, c[pc / 6], e[pc / 6], q[pc / 6], acc[pc / 6]);
+                    glm_epilogue_piece<H>(pc % PW, pc / PW, c[pc / PW], e[pc / PW], st,
+                                          acc[pc / PW]);
                 __builtin_amdgcn_sched_barrier(0);
                 ++m;
             }
+}
+
+// Half 1 of the pair epilogue without products (the last pair of a range).
+__device__ __forceinline__ void glm_epilogue_second(const f32x4 (&c)[4], double (&acc)[4],
+                                                    GlmEpiState& st) {
+    float e[4][4];
+#pragma unroll
+    for (int wb = 0; wb < 4; ++wb)
+#pragma unroll
+        for (int ph = 0; ph < 7; ++ph) glm_epilogue_piece<1>(ph, wb, c[wb], e[wb], st, acc[wb]);
 }
 
 // The row blocks [r0, r1) of one range for the 64 points whose B fragments
@@ -149,24 +175,28 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
     if (rfull > r0) {
         // software pipeline over 16-row blocks, unrolled by two so the
         // accumulator sets c0/c1 alternate without copies: the products of
-        // block i+1 run with the epilogue of block i.  Row blocks past the
-        // last are clamped to it (loaded, never used).
+        // block i+1 run with the epilogue of block i, whose logarithm is
+        // shared with block i-1 or i+1.  Row blocks past the last are
+        // clamped to it (loaded, never used).
         const unsigned rlast = rfull - 16;
         auto clamp = [&](unsigned rb) { return rb <= rlast ? rb : rlast; };
         bf16x8 a0[KC], a1[KC];
         ld.full(r0, a0);
         ld.full(clamp(r0 + 16), a1);
         glm_mfma_block<D, NS>(a0, bfrag, c0);
+        GlmEpiState st;
         unsigned rb = r0 + 16;
+        // pairs of blocks (rb - 16, rb): the first's epilogue runs with the
+        // products of rb, the second's with those of rb + 16
         for (; rb + 16 < rfull; rb += 32) {
             ld.full(clamp(rb + 16), a0);
-            glm_mfma_overlap<D, NS>(a1, bfrag, c1, c0, acc);
+            glm_mfma_overlap<D, NS, 0>(a1, bfrag, c1, c0, acc, st);
             ld.full(clamp(rb + 32), a1);
-            glm_mfma_overlap<D, NS>(a0, bfrag, c0, c1, acc);
+            glm_mfma_overlap<D, NS, 1>(a0, bfrag, c0, c1, acc, st);
         }
         if (rb < rfull) {
-            glm_mfma_overlap<D, NS>(a1, bfrag, c1, c0, acc);
-            glm_epilogue(c1, acc);
+            glm_mfma_overlap<D, NS, 0>(a1, bfrag, c1, c0, acc, st);
+            glm_epilogue_second(c1, acc, st);
         } else {
             glm_epilogue(c0, acc);
         }

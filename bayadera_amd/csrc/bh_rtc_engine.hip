@@ -553,6 +553,9 @@ bh_batch_logfn(const unsigned* __restrict__ dims, const unsigned long long* __re
 #endif  // BH_BATCH
 
 #ifdef BH_GLM
+#ifndef BH_GLM_KIND
+#define BH_GLM_KIND 1
+#endif
 // Generalised linear model posteriors evaluated for a whole batch of walkers
 // at once (the likelihood is a dense contraction X.Y over data rows, run by
 // the ahead-of-time kernels of bh_glm.hip).  These kernels are the
@@ -611,35 +614,45 @@ bh_glm_split_points(const float* __restrict__ P, unsigned n, __bf16* __restrict_
     bh_glm_split(y, yscale, nsplit, ysplit, k, n);
 }
 
-// log posterior of point k from the per-range row sums of the likelihood
-// kernel: lin.Y - scale * sum_r partial[r][k] - |Y|^2 / (2 s^2), in double
-// (the sum over ranges runs in range order).
+// Log posterior of point y from the sum s over data rows of the likelihood
+// kernel's per-row terms, in double.
+// BH_GLM_KIND 1, logistic: lin.y - scale * s - |y|^2 / (2 s0^2), with s the
+//   softplus sum and lin the linear coefficients of the precision.
+// BH_GLM_KIND 2, linear regression, y = [beta (DIM-1), log sigma], s the sum
+//   of squared residuals, lin = {N, N log(2 pi) / 2, 1 / s_beta^2,
+//   1 / s_logsigma^2}: -N log sigma - N log(2 pi)/2 - s / (2 sigma^2)
+//   - |beta|^2 / (2 s_beta^2) - (log sigma)^2 / (2 s_logsigma^2).
+__device__ __forceinline__ double bh_glm_logpost_sum(const float (&y)[DIM], double s,
+                                                     const double* __restrict__ lin, double scale,
+                                                     double prior_prec) {
+#if BH_GLM_KIND == 2
+    (void)scale;
+    (void)prior_prec;
+    const double ls = (double)y[DIM - 1];
+    double q = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM - 1; ++i) q = __builtin_fma((double)y[i], (double)y[i], q);
+    return -lin[0] * ls - lin[1] - 0.5 * exp(-2.0 * ls) * s - 0.5 * (lin[2] * q + lin[3] * ls * ls);
+#else
+    double l = 0.0, q = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        l = __builtin_fma(lin[i], (double)y[i], l);
+        q = __builtin_fma((double)y[i], (double)y[i], q);
+    }
+    return l - scale * s - 0.5 * prior_prec * q;
+#endif
+}
+
+// The same from the per-range row sums of the likelihood kernel (the sum
+// over ranges runs in range order).
 __device__ __forceinline__ double bh_glm_logpost(const float (&y)[DIM], unsigned k, unsigned n,
                                                  const double* __restrict__ partial,
                                                  unsigned nranges, const double* __restrict__ lin,
                                                  double scale, double prior_prec) {
     double s = 0.0;
     for (unsigned r = 0; r < nranges; ++r) s += partial[(size_t)r * n + k];
-    double l = 0.0, q = 0.0;
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-        l = __builtin_fma(lin[i], (double)y[i], l);
-        q = __builtin_fma((double)y[i], (double)y[i], q);
-    }
-    return l - scale * s - 0.5 * prior_prec * q;
-}
-
-// The same from the softplus sum s of point y.
-__device__ __forceinline__ double bh_glm_logpost_sum(const float (&y)[DIM], double s,
-                                                     const double* __restrict__ lin, double scale,
-                                                     double prior_prec) {
-    double l = 0.0, q = 0.0;
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-        l = __builtin_fma(lin[i], (double)y[i], l);
-        q = __builtin_fma((double)y[i], (double)y[i], q);
-    }
-    return l - scale * s - 0.5 * prior_prec * q;
+    return bh_glm_logpost_sum(y, s, lin, scale, prior_prec);
 }
 
 // Sum over r < nranges of partial[r * stride + col] by one wave: lane-strided

@@ -331,11 +331,17 @@ class HipGlmStretchFactory(HipStretchFactory):
     def __init__(self, factory: HipBayaderaFactory, model: GlmModel):
         _Handle.__init__(self)
         self.factory, self.dist_model = factory, model
-        kind = {"logistic": _lib.BH_GLM_LOGISTIC}[model.kind]
-        X = _Arr(model.X, np.uint16) if not hasattr(model.X, "data_ptr") else _Arr(model.X)
-        y = _Arr(model.y)
         n, d = model.X.shape
         log = _build_log()
+        y = _Arr(model.y)
+        if model.kind == "gaussian":
+            X = _Arr(model.X)
+            call("bh_glm_gaussian_factory_create", factory.handle, X.ptr, y.ptr, int(n), int(d),
+                 float(model.prior_sd), float(model.log_sigma_sd), ctypes.byref(self._h), log,
+                 len(log))
+            return
+        kind = {"logistic": _lib.BH_GLM_LOGISTIC}[model.kind]
+        X = _Arr(model.X, np.uint16) if not hasattr(model.X, "data_ptr") else _Arr(model.X)
         call("bh_glm_stretch_factory_create", factory.handle, kind, X.ptr, y.ptr, int(n), int(d),
              float(model.prior_sd), ctypes.byref(self._h), log, len(log))
 

@@ -543,10 +543,12 @@ class GlmModel:
     y: np.ndarray
     prior_sd: float
     name: str = "glm"
+    log_sigma_sd: float = 1.0
 
     @property
     def dimension(self) -> int:
-        return int(self.X.shape[1])
+        # linear regression carries log sigma after the coefficients
+        return int(self.X.shape[1]) + (1 if self.kind == "gaussian" else 0)
 
     @property
     def params_size(self) -> int:
@@ -564,6 +566,21 @@ def glm_logistic_model(X_bf16, y, prior_sd: float = 5.0) -> GlmModel:
     if yv.size != X.shape[0]:
         raise ValueError("y must have one label per row of X")
     return GlmModel("logistic", X, yv, float(prior_sd))
+
+
+def glm_gaussian_model(X, y, beta_sd: float = 10.0, log_sigma_sd: float = 1.0) -> GlmModel:
+    """Linear regression y_i ~ N(x_i . beta, sigma^2) declared by form:
+    walkers [beta (d), log sigma], priors beta ~ N(0, beta_sd^2 I) and
+    log sigma ~ N(0, log_sigma_sd^2); X fp32 (N, d).  The same posterior as
+    the LINREG_SOURCE user model, evaluated as one data-parallel pass per
+    half-ensemble."""
+    Xf = np.ascontiguousarray(X, np.float32)
+    if Xf.ndim != 2 or not 1 <= Xf.shape[1] <= 64:
+        raise ValueError("X must be an (N, d) float array with 1 <= d <= 64")
+    yv = np.ascontiguousarray(y, np.float32).reshape(-1)
+    if yv.size != Xf.shape[0]:
+        raise ValueError("y must have one value per row of X")
+    return GlmModel("gaussian", Xf, yv, float(beta_sd), log_sigma_sd=float(log_sigma_sd))
 
 
 def to_bf16(a) -> np.ndarray:

@@ -297,23 +297,13 @@ def c2_loglik64(rows, xs):
     return ll + lp
 
 
-def main_c2(args):
-    import torch
-    from bayadera_amd import _lib, engine, models
-    rows, params, bh, limits = c2_problem()
-    fac = engine.HipBayaderaFactory(0, 256)
-    lik = models.likelihood_model(models.LINREG_SOURCE, name="linreg")
-    prior = models.distribution_model(models.LINREG_SOURCE, name="linreg_prior",
-                                      dimension=C2_D + 1, params_size=2)
-    post = models.posterior_model(prior, "linreg_post", lik)
-    sf = fac.mcmc_factory(post)
-    burn, run = args.c2_burn, args.c2_run
+def c2_job(sf, params, limits, burn, run, prec=None):
+    from bayadera_amd import _lib
     s = sf.create_sampler(123, C2_W, params).set_partner(_lib.BH_PARTNER_WALKER)
+    if prec is not None:
+        s.set_glm_precision(prec)
     s.init(123)
     s.init_position(124, limits)
-    s.burn_in(2, A)  # warm-up
-    s.init_position(124, limits)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.burn_in(burn, A)
     s.set_timing(True)
@@ -325,6 +315,20 @@ def main_c2(args):
     kt = s.kernel_timing()
     xs, lpv, _ = s.state()
     s.release()
+    return res, var, wall, run_ms, kt, xs, lpv
+
+
+def main_c2(args):
+    from bayadera_amd import _lib, engine, models
+    rows, params, bh, limits = c2_problem()
+    fac = engine.HipBayaderaFactory(0, 256)
+    X, y = rows[:, :C2_D], rows[:, C2_D]
+    # the posterior declared by form: one data-parallel fp32 pass per half
+    sf = fac.mcmc_factory(models.glm_gaussian_model(X, y, 10.0, 1.0))
+    burn, run = args.c2_burn, args.c2_run
+    print("bench: linear-regression GLM run", file=sys.stderr, flush=True)
+    c2_job(sf, params, limits, 2, 50)  # warm-up
+    res, var, wall, run_ms, kt, xs, lpv = c2_job(sf, params, limits, burn, run, _lib.BH_GLM_FP32)
     ac = res["autocorrelation"]
     ess = ess_from(var, ac.sigma)
     ess_min = float(ess.min())
@@ -333,28 +337,44 @@ def main_c2(args):
     lp_err = np.abs(lpv[:2048].astype(np.float64) - lp64) / np.abs(lp64)
     gm = res["means"].astype(np.float64).mean(0)
     err = np.abs(gm[:C2_D] - bh) / ac.sigma[:C2_D].astype(np.float64)
-    flops_ws = C2_N * (2 * C2_D + 4)
+    flops_ws = C2_N * (2 * C2_D + 3)
     achieved = (C2_W // 2) * flops_ws / (kt["avg_ms"] * 1e-3) / 1e12
     line = {
         "metric": METRIC, "value": round(ess_min / wall, 2), "unit": "ESS/s", "n_gpus": 1,
-        "steps": burn + run, "warmup": 2, "ms_per_step": round(wall / (burn + run) * 1e3, 3),
+        "steps": burn + run, "warmup": 52, "ms_per_step": round(wall / (burn + run) * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded linear-regression data)",
         "config": {"workload": f"linear regression {C2_N}x{C2_D} (DIM 17), {C2_W} walkers, "
-                               f"{burn} burn-in + {run} run-sampler steps (a=2), fp32 model "
-                               "source through hipRTC", "partner": "walker"},
+                               f"{burn} burn-in + {run} run-sampler steps (a=2), fp32",
+                   "likelihood": "declared linear-regression GLM, plain fp32 VALU pass "
+                                 "(fp32 residuals, fp64 row sums)", "partner": "walker"},
         "walker_steps_per_s": round(C2_W * (burn + run) / wall, 1),
         "ess_per_s_run_phase": round(ess_min / (run_ms * 1e-3), 2),
-        "ess_min": round(ess_min, 1), "tau_max": round(float(ac.tau.max()), 2),
+        "ess_min": round(ess_min, 1), "ess_nan_dims": int(np.sum(~np.isfinite(ess))),
+        "tau_max": round(float(np.nanmax(ac.tau)), 2),
         "acceptance_rate": round(float(res["acceptance-rate"]), 4),
         "lp_rel_err_vs_fp64_max": float(lp_err.max()),
         "beta_mean_err_over_mcse_max_vs_least_squares": round(float(err.max()), 3),
-        "roofline": {"kernel": "bh_stretch_move_accu (4096 walkers x 100,000 rows per launch)",
+        "roofline": {"kernel": "glm_gauss_fp32_kernel<16> (100,000 rows x 4096 points)",
                      "bound": "valu", "achieved": round(achieved, 2), "peak": 157.3,
                      "unit": "TFLOP/s", "frac": round(achieved / 157.3, 4), "traffic": None,
                      "avg_kernel_ms": round(kt["avg_ms"], 4), "launches_timed": kt["launches"],
                      "algorithmic_flops_per_walker_step": flops_ws},
     }
+    if args.c2_user_steps:
+        # the reference's plugin path: the LINREG_SOURCE user model through
+        # hipRTC, one thread per walker looping over the rows (timed on a
+        # short run)
+        lik = models.likelihood_model(models.LINREG_SOURCE, name="linreg")
+        prior = models.distribution_model(models.LINREG_SOURCE, name="linreg_prior",
+                                          dimension=C2_D + 1, params_size=2)
+        usf = fac.mcmc_factory(models.posterior_model(prior, "linreg_post", lik))
+        n = args.c2_user_steps
+        print("bench: user-model run", file=sys.stderr, flush=True)
+        _, _, wall_u, run_u, kt_u, _, _ = c2_job(usf, params, limits, 0, n)
+        line["user_model_path"] = {"steps": n, "ms_per_step": round(run_u / n, 3),
+                                   "walker_steps_per_s": round(C2_W * n / (run_u * 1e-3), 1),
+                                   "kernel_avg_ms": round(kt_u["avg_ms"], 4)}
     print(json.dumps(line), flush=True)
 
 
@@ -415,6 +435,7 @@ def main_c4(args):
     blocks = bt.series_blocks(series, run)
     ess = np.zeros(len(dims))
     tau_max = np.zeros(len(dims))
+    print("bench: acor of every problem", file=sys.stderr, flush=True)
     for b, blk in enumerate(blocks):
         ac = acor.acor(blk)
         var = res["sd"][b, :dims[b]] ** 2
@@ -470,6 +491,8 @@ def c5_problems(P=C5_P, seed=1005):
     m = models.glm_logistic_batch_model(Xs, ys, prior_sd=C3_PRIOR_SD)
     lim = np.zeros((P, 64, 2), np.float32)
     for p in range(P):
+        if p % 128 == 0:
+            print(f"bench: start boxes {p}/{P}", file=sys.stderr, flush=True)
         # float32 Newton steps (setup only; the box need not be exact)
         Xd, y = m.dequantized(p).astype(np.float32), m.y[p]
         d = Xd.shape[1]
@@ -509,6 +532,7 @@ def main_c5(args):
     bt, _, _, _, _ = c5_run(fac, m, lim, _lib.BH_GLM_BF16X3, 2, 50)  # warm-up
     bt.release()
     torch.cuda.synchronize()
+    print("bench: bf16x3 batched run", file=sys.stderr, flush=True)
     bt, res, wall, burn_ms, run_ms = c5_run(fac, m, lim, _lib.BH_GLM_BF16X3, burn, run, series)
     # after the timed region: ESS per problem from the engine's acor of its
     # recorded means; the precision bias of the bf16x3 chain's stationary
@@ -737,6 +761,9 @@ def main_c3(args):
         o = {"ess_per_s": round(ess / r["wall_s"], 2),
              "ess_nan_dims": nan_dims,
              "tau_min": round(float(np.nanmin(r["tau"])), 2),
+             "ess_finite_min": round(float(np.nanmin(np.where(np.isfinite(r["ess"]), r["ess"],
+                                                               np.nan))), 1),
+             "sigma_nan_dims": int(np.sum(~np.isfinite(r["sigma"]))),
              "ess_per_s_run_phase": round(ess / (r["run_gpu_ms"] * 1e-3), 2),
              "ess_min": round(ess, 1),
              "walker_steps_per_s": round(C3_W * (burn + run) / r["wall_s"], 1),
@@ -809,6 +836,8 @@ def main():
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c3")
     ap.add_argument("--c2-burn", type=int, default=C2_BURN)
     ap.add_argument("--c2-run", type=int, default=C2_RUN)
+    ap.add_argument("--c2-user-steps", type=int, default=50,
+                    help="steps of the user-model comparison run (0 to skip)")
     ap.add_argument("--c5-burn", type=int, default=C5_BURN)
     ap.add_argument("--c5-run", type=int, default=C5_RUN)
     ap.add_argument("--c5-fp32", action="store_true", help="also the plain fp32 batched run")

@@ -51,6 +51,7 @@ extern "C" {
  * loop over data rows serially inside each walker's LOGFN, e.g.
  * distributions/gaussian.cl:29-42).  Precision of the contraction X.B: */
 #define BH_GLM_LOGISTIC 1   /* y_i ~ Bernoulli(sigmoid(x_i . beta)), beta ~ N(0, s^2 I) */
+#define BH_GLM_GAUSSIAN 2   /* y_i ~ N(x_i . beta, sigma^2), beta ~ N(0, s_b^2 I), log sigma ~ N(0, s_s^2) */
 #define BH_GLM_FP64 0       /* fp64 VALU: the accuracy reference */
 #define BH_GLM_FP32 1       /* plain fp32 VALU contraction and softplus */
 #define BH_GLM_BF16 2       /* MFMA, bf16 x bf16 -> fp32, one product */
@@ -173,6 +174,15 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
                                   uint64_t N, uint32_t d, float prior_sd,
                                   bh_stretch_factory** out, char* log, size_t log_size);
 /* Log posteriors (likelihood + prior, FP64) of n points beta (d x n). */
+/* Linear regression declared by form (no reference counterpart; the
+ * reference runs it as a user model whose LOGFN loops over the rows,
+ * posterior.cl:1-6): walkers x = [beta (d), log sigma], DIM = d + 1 <= 65;
+ * X fp32 N x d row-major, y fp32 N.  The likelihood of a half-ensemble is
+ * one data-parallel pass computing sum_i (y_i - x_i . beta)^2 per walker;
+ * precisions BH_GLM_FP32 (the default) and BH_GLM_FP64. */
+int bh_glm_gaussian_factory_create(bh_ctx* ctx, const float* X, const float* y, uint64_t N,
+                                   uint32_t d, float beta_sd, float log_sigma_sd,
+                                   bh_stretch_factory** out, char* log, size_t log_size);
 int bh_glm_log_density(bh_stretch_factory* f, const float* beta, uint64_t n, int precision,
                        double* out);
 

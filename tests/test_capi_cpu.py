@@ -106,6 +106,7 @@ def test_bad_wgs_rejected():
 @pytest.mark.parametrize("defines,source", [
     (["-DWGS=256", "-DDIM=64", "-DBH_GLM=1"], ""),
     (["-DWGS=256", "-DDIM=32", "-DBH_GLM=1"], ""),
+    (["-DWGS=256", "-DDIM=17", "-DBH_GLM=1", "-DBH_GLM_KIND=2"], ""),
     (["-DWGS=256", "-DDIM=52", "-DBH_BATCH=1", "-DLOGFN=hbb_logpdf"], models.HBB_SOURCE),
     (["-DWGS=256", "-DDIM=6", "-DBH_BATCH=1", "-DLOGFN=mvn_logpdf"], models.MVN_SOURCE),
 ])

@@ -164,3 +164,69 @@ def test_kernel_timing_counts_contractions(fac):
     t = s.kernel_timing()
     assert t["launches"] == 10 and t["avg_ms"] > 0.0
     s.release()
+
+
+def _linreg(N, d, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((N, d)).astype(np.float32)
+    beta = rng.standard_normal(d)
+    y = (X.astype(np.float64) @ beta + 0.5 * rng.standard_normal(N)).astype(np.float32)
+    pts = np.concatenate([beta + 0.01 * rng.standard_normal((300, d)),
+                          np.log(0.5) + 0.05 * rng.standard_normal((300, 1))], 1)
+    return X, y, pts.astype(np.float32)
+
+
+def _linreg_lp64(X, y, pts, sb=10.0, ss=1.0):
+    """The linear-regression log posterior in float64 (numpy)."""
+    Xd, yd = X.astype(np.float64), y.astype(np.float64)
+    b, ls = pts[:, :-1].astype(np.float64), pts[:, -1].astype(np.float64)
+    r = yd[None, :] - b @ Xd.T
+    N = X.shape[0]
+    ll = -N * ls - 0.5 * N * np.log(2 * np.pi) - 0.5 * np.exp(-2 * ls) * np.sum(r * r, 1)
+    return ll - 0.5 * (np.sum(b * b, 1) / sb ** 2 + ls ** 2 / ss ** 2)
+
+
+@pytest.mark.parametrize("N,d", [(5000, 16), (777, 5), (20011, 33)])
+def test_glm_gaussian_log_density_vs_fp64_and_user_model(fac, N, d):
+    X, y, pts = _linreg(N, d, 29 + d)
+    sf = fac.mcmc_factory(models.glm_gaussian_model(X, y, 10.0, 1.0))
+    ref = _linreg_lp64(X, y, pts)
+    scale = np.abs(ref) + 1.0
+    f64 = sf.log_density(pts, _lib.BH_GLM_FP64)
+    f32 = sf.log_density(pts, _lib.BH_GLM_FP32)
+    assert np.max(np.abs(f64 - ref) / scale) < 1e-12
+    assert np.max(np.abs(f32 - ref) / scale) < 2e-6
+    with pytest.raises(_lib.ArgumentError):
+        sf.log_density(pts, _lib.BH_GLM_BF16X3)
+    # the same posterior through the reference's plugin path (user C99
+    # source compiled by hipRTC, one walker per thread looping over rows)
+    lik = models.likelihood_model(models.LINREG_SOURCE, name="linreg")
+    prior = models.distribution_model(models.LINREG_SOURCE, name="linreg_prior",
+                                      dimension=d + 1, params_size=2)
+    post = models.posterior_model(prior, "linreg_post", lik)
+    de = fac.distribution_engine(post)
+    params = np.concatenate([np.concatenate([X, y[:, None]], 1).reshape(-1),
+                             np.array([10.0, 1.0], np.float32)]).astype(np.float32)
+    user = de.log_density(params, pts[:64]).astype(np.float64)
+    assert np.max(np.abs(user - ref[:64]) / scale[:64]) < 2e-5
+    de.release()
+
+
+def test_glm_gaussian_sampler_moves_and_rejects_guard(fac):
+    X, y, pts = _linreg(4000, 16, 41)
+    sf = fac.mcmc_factory(models.glm_gaussian_model(X, y))
+    s = sf.create_sampler(3, 1024, np.zeros(1, np.float32)).set_partner(_lib.BH_PARTNER_WALKER)
+    with pytest.raises(_lib.ArgumentError):
+        s.set_glm_guard(1e-2)
+    with pytest.raises(_lib.ArgumentError):
+        s.set_glm_precision(_lib.BH_GLM_BF16X3)
+    lo = pts.min(0) - 0.01
+    hi = pts.max(0) + 0.01
+    s.init_position(4, np.stack([lo, hi], 1).reshape(-1))
+    s.burn_in(50, 2.0)
+    res = s.run_sampler(100, 2.0)
+    assert 0.05 < res["acceptance-rate"] < 0.9
+    xs, lp, _ = s.state()
+    ref = _linreg_lp64(X, y, xs[:64])
+    assert np.allclose(lp[:64], ref, rtol=2e-6)
+    s.release()
