@@ -102,6 +102,7 @@ SIGNATURES = {
     "bh_batch_init_position": [_P, _U32, _P],
     "bh_batch_burn_in": [_P, _U32, _U64, _F],
     "bh_batch_run": [_P, _U32, _U64, _F, _P, _P, _P],
+    "bh_batch_autotune": [_P, _U32, _P, _P, _P],
     "bh_batch_run_series": [_P, _U32, _U64, _F, _P, _P, _P, _P],
     "bh_glm_batch_create": [_P, _I, _U32, _P, _P, _P, _P, _P, _F, _U32, _PP],
     "bh_glm_batch_destroy": [_P],

@@ -314,6 +314,14 @@ struct bh_batch {
     Module mod;
     hipFunction_t run = nullptr, init = nullptr, logfn = nullptr;
     uint32_t max_dim = 0, problems = 0, W = 0, nb = 0, dim_hi = 0;
+    // launch classes: problems order[begin .. begin + count) with dimensions
+    // up to dim_hi, each launched with the LDS its largest problem needs
+    struct Class {
+        uint32_t begin, count, dim_hi;
+    };
+    std::vector<Class> classes;
+    std::vector<uint32_t> dims_h;
+    DevBuf order;
     uint32_t bare_counter = 0;
     int partner = BH_PARTNER_FLAT;
     DevBuf dims, poff, dlen, plen, params, xs, logfn_buf, accept, sums;
@@ -1402,9 +1410,32 @@ int bh_stretch_glm_guard_stats(bh_stretch* s, uint64_t* listed, uint64_t* overfl
 }
 
 // ------------------------------------------------------------------ batch
-static size_t batch_lds(const bh_batch* b) {
-    return (size_t)2 * b->max_dim * sizeof(double) + (size_t)8 * b->nb * sizeof(float) +
-           ((size_t)b->W * b->dim_hi + b->W + (size_t)b->nb * b->dim_hi) * sizeof(float);
+static size_t batch_lds(const bh_batch* b, uint32_t dim_hi, uint32_t nb) {
+    return (size_t)2 * b->max_dim * sizeof(double) + (size_t)8 * nb * sizeof(float) +
+           ((size_t)b->W * dim_hi + b->W + (size_t)nb * dim_hi) * sizeof(float);
+}
+
+static size_t batch_lds(const bh_batch* b) { return batch_lds(b, b->dim_hi, b->nb); }
+
+// Launch classes from dimension bounds (ascending; the last >= every dim):
+// problems sorted by dimension, one class per bound that holds any.
+static int batch_set_classes(bh_batch* b, const std::vector<uint32_t>& bounds) {
+    std::vector<uint32_t> ord(b->problems);
+    for (uint32_t i = 0; i < b->problems; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](uint32_t x, uint32_t y) { return b->dims_h[x] < b->dims_h[y]; });
+    b->classes.clear();
+    uint32_t i = 0;
+    for (uint32_t bound : bounds) {
+        const uint32_t begin = i;
+        uint32_t hi = 0;
+        while (i < b->problems && b->dims_h[ord[i]] <= bound) hi = std::max(hi, b->dims_h[ord[i++]]);
+        if (i > begin) b->classes.push_back({begin, i - begin, hi});
+    }
+    if (i != b->problems) return fail(BH_ERR_ARG, "class bounds do not cover every dimension");
+    BH_HIP(hipMemcpyAsync(b->order.p, ord.data(), ord.size() * 4, hipMemcpyHostToDevice,
+                          b->stream));
+    return sync(b->stream);
 }
 
 int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uint32_t max_dim,
@@ -1466,6 +1497,7 @@ int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uin
     BH_TRY(b->logfn_buf.alloc((size_t)problems * walkers * sizeof(float)));
     BH_TRY(b->accept.alloc((size_t)problems * 8));
     BH_TRY(b->sums.alloc((size_t)problems * max_dim * 2 * sizeof(double)));
+    BH_TRY(b->order.alloc((size_t)problems * 4));
     BH_HIP(hipMemcpyAsync(b->dims.p, dims, (size_t)problems * 4, hipMemcpyDefault, b->stream));
     BH_HIP(hipMemcpyAsync(b->dlen.p, dl.data(), (size_t)problems * 4, hipMemcpyDefault, b->stream));
     BH_HIP(hipMemcpyAsync(b->plen.p, pl.data(), (size_t)problems * 4, hipMemcpyDefault, b->stream));
@@ -1476,7 +1508,8 @@ int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uin
     BH_HIP(hipMemsetAsync(b->xs.p, 0, (size_t)problems * walkers * max_dim * sizeof(float),
                           b->stream));
     BH_HIP(hipMemsetAsync(b->logfn_buf.p, 0, (size_t)problems * walkers * sizeof(float), b->stream));
-    BH_TRY(sync(b->stream));
+    b->dims_h.assign(dims, dims + problems);
+    BH_TRY(batch_set_classes(b.get(), {b->dim_hi}));
     *out = b.release();
     return BH_OK;
 }
@@ -1544,10 +1577,16 @@ static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mod
         unsigned step0 = b->bare_counter;
         unsigned chunk = (unsigned)std::min<uint64_t>(n - done, 1u << 20);
         float* ser = series;  // recorded runs are a single launch (n < 2^20)
-        void* args[] = {&seed0, &dims, &poff, &dlen, &plen, &params, &xs, &lf, &W, &a,
-                        &step0, &chunk, &partner, &mode, &acc, &sums, &soff, &ser};
-        BH_HIP(hipModuleLaunchKernel(b->run, b->problems, 1, 1, b->nb, 1, 1,
-                                     (unsigned)batch_lds(b), b->stream, args, nullptr));
+        const unsigned* ord = b->order.as<unsigned>();
+        for (const auto& cl : b->classes) {
+            unsigned base = cl.begin;
+            void* args[] = {&seed0, &dims, &poff, &dlen, &plen, &params, &xs, &lf, &W, &a,
+                            &step0, &chunk, &partner, &mode, &acc, &sums, &soff, &ser, &ord,
+                            &base};
+            BH_HIP(hipModuleLaunchKernel(b->run, cl.count, 1, 1, b->nb, 1, 1,
+                                         (unsigned)batch_lds(b, cl.dim_hi, b->nb), b->stream,
+                                         args, nullptr));
+        }
         b->bare_counter += chunk;
         done += chunk;
     }
@@ -1556,6 +1595,69 @@ static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mod
     float ms = 0.0f;
     BH_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
+    return BH_OK;
+}
+
+int bh_batch_autotune(bh_batch* b, uint32_t steps, uint32_t* block_out, uint32_t* classes_out,
+                      double* ms_out) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(steps >= 1 && steps <= 10000, "bad step count");
+    BH_HIP(hipSetDevice(b->ctx->device));
+    int lds_max = 0;
+    BH_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                 b->ctx->device));
+    // the state every candidate starts from, restored afterwards
+    const size_t xbytes = (size_t)b->problems * b->W * b->max_dim * sizeof(float);
+    const size_t lbytes = (size_t)b->problems * b->W * sizeof(float);
+    DevBuf xs0, lf0;
+    BH_TRY(xs0.alloc(xbytes));
+    BH_TRY(lf0.alloc(lbytes));
+    BH_HIP(hipMemcpyAsync(xs0.p, b->xs.p, xbytes, hipMemcpyDeviceToDevice, b->stream));
+    BH_HIP(hipMemcpyAsync(lf0.p, b->logfn_buf.p, lbytes, hipMemcpyDeviceToDevice, b->stream));
+    const uint32_t counter0 = b->bare_counter;
+    std::vector<uint32_t> sorted(b->dims_h);
+    std::sort(sorted.begin(), sorted.end());
+    std::vector<std::vector<uint32_t>> bound_sets;
+    for (uint32_t nc = 1; nc <= 4; ++nc) {
+        std::vector<uint32_t> bs;
+        for (uint32_t c = 1; c < nc; ++c) bs.push_back(sorted[(size_t)sorted.size() * c / nc]);
+        bs.push_back(sorted.back());
+        bs.erase(std::unique(bs.begin(), bs.end()), bs.end());
+        bound_sets.push_back(bs);
+    }
+    const uint32_t K = b->W / 2;
+    double best = -1.0;
+    uint32_t best_nb = b->nb;
+    std::vector<uint32_t> best_bounds = {b->dim_hi};
+    for (uint32_t nb = 64; nb <= 1024 && nb <= std::max(64u, K); nb *= 2) {
+        for (const auto& bs : bound_sets) {
+            b->nb = nb;
+            BH_TRY(batch_set_classes(b, bs));
+            bool fits = true;
+            for (const auto& cl : b->classes)
+                fits = fits && batch_lds(b, cl.dim_hi, nb) <= (size_t)lds_max;
+            if (!fits) continue;
+            BH_HIP(hipMemcpyAsync(b->xs.p, xs0.p, xbytes, hipMemcpyDeviceToDevice, b->stream));
+            BH_HIP(hipMemcpyAsync(b->logfn_buf.p, lf0.p, lbytes, hipMemcpyDeviceToDevice,
+                                  b->stream));
+            b->bare_counter = counter0;
+            BH_TRY(batch_steps(b, 0x5eedu, steps, 2.0f, 0));
+            if (best < 0.0 || b->last_ms < best) {
+                best = b->last_ms;
+                best_nb = nb;
+                best_bounds = bs;
+            }
+        }
+    }
+    b->nb = best_nb;
+    BH_TRY(batch_set_classes(b, best_bounds));
+    BH_HIP(hipMemcpyAsync(b->xs.p, xs0.p, xbytes, hipMemcpyDeviceToDevice, b->stream));
+    BH_HIP(hipMemcpyAsync(b->logfn_buf.p, lf0.p, lbytes, hipMemcpyDeviceToDevice, b->stream));
+    b->bare_counter = counter0;
+    BH_TRY(sync(b->stream));
+    if (block_out) *block_out = b->nb;
+    if (classes_out) *classes_out = (uint32_t)b->classes.size();
+    if (ms_out) *ms_out = best;
     return BH_OK;
 }
 

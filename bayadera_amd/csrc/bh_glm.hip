@@ -300,10 +300,12 @@ glm_gauss_fp64_kernel(const float* __restrict__ X, const float* __restrict__ yv,
 }
 
 // fp64 softplus sums of listed points (the guarded accept test's fallback).
-// Block (r, s) owns row range r and runs points p = s, s + gridDim.y, ...
-// of the 2c listed ones (c = min(*count, lmax)); its 256 threads take the
-// range's rows one each, 256 at a time, with the point's coefficients read
-// from LDS as broadcasts and the same arithmetic as the fp64 kernel.
+// Block r owns row range r; the 2c listed points (c = min(*count, lmax))
+// run eight at a time, their coefficients in LDS (double).  Each thread
+// takes rows of the range one at a time, converts the row to double once,
+// and evaluates the eight points on it: eight independent fma chains, then
+// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)) in double as the fp64
+// kernel does.  Per-point sums are reduced over the block.
 template <int D>
 __global__ void __launch_bounds__(256)
 glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
@@ -311,41 +313,65 @@ glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
                      const unsigned* __restrict__ list, const unsigned* __restrict__ count,
                      unsigned lmax, unsigned long long rows_per_range,
                      double* __restrict__ partial64) {
-    __shared__ double b[D];
-    __shared__ double red[4];
+    constexpr int PC = 8;
+    __shared__ double b[PC][D];
+    __shared__ double red[PC][4];
     const unsigned n = *count;
     const unsigned c = n < lmax ? n : lmax;
+    if (c == 0) return;
     const unsigned np = 2 * c;
     const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_range;
     unsigned long long r1 = r0 + rows_per_range;
     if (r1 > N) r1 = N;
-    for (unsigned p = blockIdx.y; p < np; p += gridDim.y) {
+    for (unsigned p0 = 0; p0 < np; p0 += PC) {
         __syncthreads();
-        if (t < D) {
-            const unsigned w = list[p < c ? p : p - c];
-            b[t] = (double)(p < c ? Y : W)[(size_t)w * D + t];
+        for (unsigned e = t; e < PC * D; e += 256) {
+            const unsigned p = p0 + e / D;
+            double v = 0.0;
+            if (p < np) {
+                const unsigned w = list[p < c ? p : p - c];
+                v = (double)(p < c ? Y : W)[(size_t)w * D + e % D];
+            }
+            b[e / D][e % D] = v;
         }
         __syncthreads();
-        double acc = 0.0;
+        double acc[PC];
+#pragma unroll
+        for (int q = 0; q < PC; ++q) acc[q] = 0.0;
         for (unsigned long long row = r0 + t; row < r1; row += 256) {
+            // the coefficients are read from LDS row by row (a compiler
+            // barrier keeps them from being hoisted into registers)
+            asm volatile("" ::: "memory");
             const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
-            double eta = 0.0;
+            double eta[PC];
+#pragma unroll
+            for (int q = 0; q < PC; ++q) eta[q] = 0.0;
 #pragma unroll
             for (int v = 0; v < D / 8; ++v) {
                 const bf16x8 xv = xr[v];
 #pragma unroll
-                for (int e = 0; e < 8; ++e)
-                    eta = __builtin_fma((double)(float)xv[e], b[v * 8 + e], eta);
+                for (int e = 0; e < 8; ++e) {
+                    const double x = (double)(float)xv[e];
+#pragma unroll
+                    for (int q = 0; q < PC; ++q) eta[q] = __builtin_fma(x, b[q][v * 8 + e], eta[q]);
+                }
             }
-            acc += __builtin_fmax(eta, 0.0) + log1p(exp(-__builtin_fabs(eta)));
+#pragma unroll
+            for (int q = 0; q < PC; ++q)
+                acc[q] += __builtin_fmax(eta[q], 0.0) + log1p(exp(-__builtin_fabs(eta[q])));
         }
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (lane == 0) red[wave] = acc;
+        for (int q = 0; q < PC; ++q) {
+            double v = acc[q];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[q][wave] = v;
+        }
         __syncthreads();
-        if (t == 0)
-            partial64[(size_t)blockIdx.x * 2 * lmax + p] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (t < PC && p0 + t < np)
+            partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
+                (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
     }
 }
 
@@ -355,13 +381,12 @@ hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long
                                 double* partial64) {
     const unsigned long long rpr = (N + nranges - 1) / nranges;
     const __bf16* x = static_cast<const __bf16*>(X);
-    const dim3 grid(nranges, 8);
     if (d == 64)
-        glm_fp64_list_kernel<64><<<grid, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
-                                                      partial64);
+        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
+                                                         partial64);
     else if (d == 32)
-        glm_fp64_list_kernel<32><<<grid, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
-                                                      partial64);
+        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
+                                                         partial64);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

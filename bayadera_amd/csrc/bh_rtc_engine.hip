@@ -384,6 +384,8 @@ bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsig
 // x^2 (each step's ensemble sums are float tree sums over the walkers); with
 // series non-null it also writes each step's ensemble mean (that float sum
 // over W), problem b's n x dims[b] block starting at series + soff[b].
+// Work-group i of a launch runs problem order[base + i] (problems are
+// launched in classes of similar dimension, each with the LDS it needs).
 // LDS: W*dim + W + nb*dim floats + 2*DIM doubles + 8*nb floats.
 extern "C" __global__ void __launch_bounds__(1024)
 bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
@@ -392,9 +394,10 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
              float* __restrict__ xs, float* __restrict__ logfn, unsigned W, float a,
              unsigned step0, unsigned nsteps, int partner, int mode,
              unsigned long long* __restrict__ accept, double* __restrict__ sums,
-             const unsigned long long* __restrict__ soff, float* __restrict__ series) {
+             const unsigned long long* __restrict__ soff, float* __restrict__ series,
+             const unsigned* __restrict__ order, unsigned base) {
     extern __shared__ double lds_d[];
-    const unsigned b = blockIdx.x;
+    const unsigned b = order[base + blockIdx.x];
     const unsigned dim = dims[b];
     const unsigned K = W / 2;
     const unsigned t = threadIdx.x, nb = blockDim.x;

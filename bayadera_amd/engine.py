@@ -395,6 +395,14 @@ class HipBatch(_Handle):
         call("bh_batch_burn_in", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a))
         return self
 
+    def autotune(self, steps: int = 20) -> dict:
+        """Times the candidate launch shapes from the current state and keeps
+        the fastest (the state is restored)."""
+        nb, nc, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        call("bh_batch_autotune", self._h, int(steps), ctypes.byref(nb), ctypes.byref(nc),
+             ctypes.byref(ms))
+        return {"threads_per_problem": nb.value, "launch_classes": nc.value, "ms": ms.value}
+
     def run(self, seed0: int, n: int, a: float, series=None) -> dict:
         """n accounted steps.  ``series``: optional (device or host) float
         array of n * sum(dims) values that receives each problem's

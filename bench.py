@@ -421,6 +421,9 @@ def main_c4(args):
     bt.init_position(3, lim)
     bt.burn_in(1000003, 2, A)  # warm-up
     bt.init_position(3, lim)
+    # launch shape (threads per problem, dimension classes) chosen by timing
+    # the candidates from this state; the state is restored (setup, untimed)
+    tuned = bt.autotune(10)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bt.burn_in(5, burn, A)
@@ -453,7 +456,8 @@ def main_c4(args):
         "config": {"workload": f"{len(dims)} hierarchical beta-binomial posteriors, G_i in "
                                f"[5, 50] (DIM 7-52), {C4_W} walkers each, {burn} burn-in + "
                                f"{run} run steps (a=2), one work-group per problem",
-                   "walkers_per_workgroup": C4_W // 2, "partner": "flat"},
+                   "threads_per_problem": tuned["threads_per_problem"],
+                   "launch_classes": tuned["launch_classes"], "partner": "flat"},
         "walker_steps_per_s": round(walker_steps / wall, 1),
         "burn_in_gpu_ms": round(burn_ms, 2), "run_gpu_ms": round(run_ms, 2),
         "ess_sum": round(float(ess[ok].sum()), 1),

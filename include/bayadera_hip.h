@@ -281,6 +281,15 @@ int bh_batch_set_partner(bh_batch* b, int partner_mode);
  * problems x max_dim x 2 (rows beyond a problem's dim are ignored). */
 int bh_batch_init_position(bh_batch* b, uint32_t seed, const float* limits);
 int bh_batch_burn_in(bh_batch* b, uint32_t seed0, uint64_t n, float a);
+/* Chooses the launch shape of the batch by timing `steps` bare steps of each
+ * candidate from the current state (restored afterwards): threads per
+ * problem (walkers per work-group pass; 64 .. 1024) and launch classes
+ * (problems split by dimension into 1-4 launches, each with the LDS its
+ * largest problem needs).  Positions do not depend on the shape; the
+ * accounting sums' rounding does (their tree has one leaf per thread).  No
+ * reference counterpart. */
+int bh_batch_autotune(bh_batch* b, uint32_t steps, uint32_t* threads_per_problem,
+                      uint32_t* launch_classes, double* best_ms);
 /* n steps with accounting: per problem the acceptance rate and, over the n
  * steps and all walkers, the posterior mean and sd of each coordinate
  * (problems x max_dim; NULL to skip). */

@@ -96,3 +96,28 @@ def test_batch_hierarchical_beta_binomial_runs(fac):
     assert np.all(res["acceptance-rate"] > 0.005)
     _, lf = bt.state()
     assert np.all(np.isfinite(lf))
+
+
+def test_batch_autotune_keeps_trajectories_and_restores_state(fac):
+    """The chosen launch shape (threads per problem, dimension classes)
+    changes no position; the tuning runs leave the state as it was."""
+    B, W, MAXD = 40, 512, 6
+    dims, params = _mvn_problems(B, 8)
+    lim = np.zeros((B, MAXD, 2), np.float32)
+    lim[:, :, 0], lim[:, :, 1] = -4.0, 4.0
+    runs = []
+    for tune in (False, True):
+        bt = fac.batch(models.mvn_model(MAXD), params, dims, [0] * B, W)
+        bt.init_position(3, lim)
+        before = bt.state()[0]
+        if tune:
+            t = bt.autotune(5)
+            assert t["threads_per_problem"] in (64, 128, 256) and t["launch_classes"] >= 1
+            after = bt.state()[0]
+            assert all(np.array_equal(x, y) for x, y in zip(before, after))
+        bt.burn_in(21, 12, 2.0)
+        res = bt.run(22, 60, 2.0)
+        runs.append((bt.state()[0], res["acceptance-rate"]))
+        bt.release()
+    assert all(np.array_equal(x, y) for x, y in zip(runs[0][0], runs[1][0]))
+    assert np.array_equal(runs[0][1], runs[1][1])
