@@ -419,10 +419,13 @@ unsigned resident_block(const bh_stretch* s) {
 
 size_t resident_lds(const bh_stretch* s, int mode) {
     const unsigned nb = resident_block(s);
-    // positions, logfn values, and the accounting staging area (the
-    // kernel's BH_ACCU_CHUNK = min(DIM, 8) floats per thread)
+    // positions, logfn values, and the accounting staging areas (the
+    // kernel's BH_ACCU_CHUNK = min(DIM, 8) floats per thread, the odd half's
+    // group sums and the groups' accept counts)
     const size_t chunk = s->dim < 8 ? s->dim : 8;
-    size_t floats = (size_t)s->W * s->dim + s->W + (mode == 1 ? chunk * nb : 0);
+    const size_t groups = s->K / s->wgs;
+    size_t floats = (size_t)s->W * s->dim + s->W +
+                    (mode == 1 ? chunk * nb + groups * s->dim + groups : 0);
     return floats * 4 + (nb / 64) * 4;
 }
 

@@ -224,8 +224,76 @@ bh_logfn(unsigned data_len, unsigned params_len, const float* __restrict__ param
     if (w < n) res[w] = LOGFN(data_len, params_len, params, DIM, x + (size_t)w * DIM);
 }
 
+// The same accounting inside the resident kernel, without global
+// read-modify-writes in the step loop: a slice's accept count goes to LDS
+// (acc_lds[g], added to accept[g] once at the end of the launch), the odd
+// half's per-group sums wait in LDS (odd_lds[g][d]) and the even half stores
+// (0 + odd) + even to the step's means slot -- the value the two
+// read-modify-writes of the per-half-step kernels leave in a zeroed slot.
+__device__ __forceinline__ void bh_accu_resident(const float (&xk)[DIM], bool accepted, unsigned g,
+                                                 unsigned groups, unsigned mstep, int half,
+                                                 unsigned* acc_lds, float* odd_lds, float* means,
+                                                 float* red, unsigned* wave_counts) {
+    const unsigned cnt = bh_slice_count(accepted, wave_counts);
+    const bool leader = (threadIdx.x & (WGS - 1)) == 0;
+    if (leader) acc_lds[g] += cnt;
+    float* m = means + (size_t)groups * DIM * mstep + g;
+#pragma unroll
+    for (int d0 = 0; d0 < DIM; d0 += BH_ACCU_CHUNK) {
+        float v[BH_ACCU_CHUNK], sums[BH_ACCU_CHUNK];
+#pragma unroll
+        for (int c = 0; c < BH_ACCU_CHUNK; ++c) v[c] = (d0 + c < DIM) ? xk[d0 + c] : 0.0f;
+        bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums);
+        if (leader) {
+#pragma unroll
+            for (int c = 0; c < BH_ACCU_CHUNK; ++c) {
+                if (d0 + c < DIM) {
+                    float* o = odd_lds + (size_t)g * DIM + d0 + c;
+                    if (half == 0) *o = sums[c];
+                    else m[(size_t)(d0 + c) * groups] = (0.0f + *o) + sums[c];
+                }
+            }
+        }
+    }
+}
+
 // One half-step over K walkers, one thread per walker, WGS walkers per block
 // (stretch_move_bare / stretch_move_accu, amd-gcn-mcmc-stretch.cl:83-134).
+// The walker's own record and log density are read and written once per
+// half-step: streamed with non-temporal loads and stores, so that the
+// caches keep the complementary half, which is read at random.
+__device__ __forceinline__ void bh_load_walker_nt(const float* p, float (&v)[DIM]) {
+    if constexpr (DIM % 4 == 0) {
+        typedef float bh_f4 __attribute__((ext_vector_type(4)));
+        const bh_f4* q = reinterpret_cast<const bh_f4*>(__builtin_assume_aligned(p, 16));
+#pragma unroll
+        for (int i = 0; i < DIM / 4; ++i) {
+            const bh_f4 t = __builtin_nontemporal_load(q + i);
+            v[4 * i] = t.x;
+            v[4 * i + 1] = t.y;
+            v[4 * i + 2] = t.z;
+            v[4 * i + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) v[i] = __builtin_nontemporal_load(p + i);
+    }
+}
+
+__device__ __forceinline__ void bh_store_walker_nt(float* p, const float (&v)[DIM]) {
+    if constexpr (DIM % 4 == 0) {
+        typedef float bh_f4 __attribute__((ext_vector_type(4)));
+        bh_f4* q = reinterpret_cast<bh_f4*>(__builtin_assume_aligned(p, 16));
+#pragma unroll
+        for (int i = 0; i < DIM / 4; ++i)
+            __builtin_nontemporal_store(bh_f4{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]},
+                                        q + i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) __builtin_nontemporal_store(v[i], p + i);
+    }
+}
+
 template <bool ACCU, int PARTNER>
 __device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_even,
                                                 unsigned data_len, unsigned params_len,
@@ -238,13 +306,13 @@ __device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_e
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float xk[DIM];
     float* xp = X + (size_t)k * DIM;
-    bh_load_walker(xp, xk);
-    float lx = logfn_X[k];
+    bh_load_walker_nt(xp, xk);
+    float lx = __builtin_nontemporal_load(logfn_X + k);
     const bool accepted = bh_stretch_walker<PARTNER>(k, K, seed, odd_or_even, step, a, beta, S,
                                                      xk, lx, data_len, params_len, params);
     if (accepted) {
-        bh_store_walker(xp, xk);
-        logfn_X[k] = lx;
+        bh_store_walker_nt(xp, xk);
+        __builtin_nontemporal_store(lx, logfn_X + k);
     }
     if (ACCU) {
         __shared__ float red[BH_ACCU_CHUNK * WGS];
@@ -305,11 +373,15 @@ __device__ __forceinline__ void bh_resident_impl(
     float* sx = lds;                       // W*DIM
     float* sl = sx + (size_t)W * DIM;      // W
     float* red = sl + W;                   // BH_ACCU_CHUNK*nb (mode 1)
-    unsigned* wave_counts = (unsigned*)(red + (mode == 1 ? BH_ACCU_CHUNK * nb : 0));
+    const unsigned groups = K / WGS;
+    float* odd_lds = red + (mode == 1 ? BH_ACCU_CHUNK * nb : 0);       // groups*DIM (mode 1)
+    unsigned* acc_lds = (unsigned*)(odd_lds + (mode == 1 ? groups * DIM : 0));  // groups
+    unsigned* wave_counts = acc_lds + (mode == 1 ? groups : 0);
     for (unsigned i = t; i < W * DIM; i += nb) sx[i] = xs[i];
     for (unsigned i = t; i < W; i += nb) sl[i] = logfn[i];
+    if (mode == 1)
+        for (unsigned i = t; i < groups; i += nb) acc_lds[i] = 0u;
     __syncthreads();
-    const unsigned groups = K / WGS;
     unsigned long long written = 0;
     for (unsigned s = 0; s < nsteps; ++s) {
         const unsigned step = step0 + s;
@@ -335,7 +407,8 @@ __device__ __forceinline__ void bh_resident_impl(
                     L[k] = lx;
                 }
                 if (mode == 1)
-                    bh_accu(xk, acc, k / WGS, groups, step, accept, means, red, wave_counts);
+                    bh_accu_resident(xk, acc, k / WGS, groups, step, h, acc_lds, odd_lds, means,
+                                     red, wave_counts);
             }
             __syncthreads();
         }
@@ -348,6 +421,10 @@ __device__ __forceinline__ void bh_resident_impl(
     }
     for (unsigned i = t; i < W * DIM; i += nb) xs[i] = sx[i];
     for (unsigned i = t; i < W; i += nb) logfn[i] = sl[i];
+    if (mode == 1) {
+        __syncthreads();
+        for (unsigned i = t; i < groups; i += nb) accept[i] += acc_lds[i];
+    }
 }
 
 extern "C" __global__ void __launch_bounds__(1024)
