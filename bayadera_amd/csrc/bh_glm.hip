@@ -36,6 +36,7 @@ __device__ __forceinline__ void glm_tile(unsigned groups, unsigned& group, unsig
 // kc*32 + lg*8 .. +7.
 template <int D>
 struct Bf16Rows {
+    using vec = bf16x8;
     const __bf16* X;
     unsigned lr, lg;
     __device__ void full(unsigned rb, bf16x8 (&a)[D / 32]) const {

@@ -48,6 +48,7 @@ __device__ __forceinline__ bf16x8 fp8x8_to_bf16x8(uint2 v) {
 // A fragments from fp8 X (row stride D bytes).
 template <int D>
 struct Fp8Rows {
+    using vec = bf16x8;
     const unsigned char* X;
     unsigned lr, lg;
     __device__ void full(unsigned rb, bf16x8 (&a)[D / 32]) const {

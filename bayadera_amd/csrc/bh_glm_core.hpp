@@ -1,15 +1,25 @@
 // Device building blocks of the GLM likelihood kernels, shared by the
 // single-posterior kernels (bh_glm.hip) and the batched ones
-// (bh_glm_batch.hip): the softplus epilogue, the v_mfma_f32_16x16x32_bf16
-// products of one 16-row block, and the software-pipelined loop over the
-// row blocks of a range with the epilogue spread between the products.
+// (bh_glm_batch.hip): the softplus epilogue, the 16x16x32 matrix-core
+// products of one 16-row block (bf16 or fp16 operands, fp32 accumulators),
+// and the software-pipelined loop over the row blocks of a range with the
+// epilogue spread between the products.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace bh {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_{bf16,f16}: the same lane layout for both operand types
+__device__ __forceinline__ f32x4 mfma_16x16x32(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma_16x16x32(const f16x8& a, const f16x8& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
 
 // sum over four values of |u|/2 + log2(1 + 2^-|u|): four v_exp_f32 and one
 // v_log_f32 of the product of the four factors 1 + 2^-|u|, which lies in
@@ -25,9 +35,8 @@ __device__ __forceinline__ float softplus2_rest4(float u0, float u1, float u2, f
     return __builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p01 * p23));
 }
 
-template <int D, int NS>
-__device__ __forceinline__ void glm_mfma_block(const bf16x8 (&a)[D / 32],
-                                               const bf16x8 (&bfrag)[4][D / 32][NS],
+template <int D, int NS, class V>
+__device__ __forceinline__ void glm_mfma_block(const V (&a)[D / 32], const V (&bfrag)[4][D / 32][NS],
                                                f32x4 (&c)[4]) {
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
@@ -36,8 +45,7 @@ __device__ __forceinline__ void glm_mfma_block(const bf16x8 (&a)[D / 32],
         for (int s = NS - 1; s >= 0; --s)
 #pragma unroll
             for (int kc = 0; kc < D / 32; ++kc)
-                c[wb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], c[wb], 0,
-                                                                0, 0);
+                c[wb] = mfma_16x16x32(a[kc], bfrag[wb][kc][s], c[wb]);
     }
 }
 
@@ -119,9 +127,9 @@ __device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& 
 // epilogue piece between two products, where it runs on the VALU while the
 // matrix core works.  The per-column-block product order is that of
 // glm_mfma_block (split terms smallest first, chunks in order).
-template <int D, int NS, int H>
-__device__ __forceinline__ void glm_mfma_overlap(const bf16x8 (&a)[D / 32],
-                                                 const bf16x8 (&bfrag)[4][D / 32][NS],
+template <int D, int NS, int H, class V>
+__device__ __forceinline__ void glm_mfma_overlap(const V (&a)[D / 32],
+                                                 const V (&bfrag)[4][D / 32][NS],
                                                  f32x4 (&cn)[4], const f32x4 (&c)[4],
                                                  double (&acc)[4], GlmEpiState& st) {
     constexpr int KC = D / 32;
@@ -138,8 +146,7 @@ __device__ __forceinline__ void glm_mfma_overlap(const bf16x8 (&a)[D / 32],
             for (int wb = 0; wb < 4; ++wb) {
                 const f32x4 acc0 = (s == NS - 1 && kc == 0) ? f32x4{0.0f, 0.0f, 0.0f, 0.0f}
                                                             : cn[wb];
-                cn[wb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bfrag[wb][kc][s], acc0, 0,
-                                                                 0, 0);
+                cn[wb] = mfma_16x16x32(a[kc], bfrag[wb][kc][s], acc0);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int pc = m * NP / NM; pc < (m + 1) * NP / NM; ++pc)
@@ -164,10 +171,12 @@ __device__ __forceinline__ void glm_epilogue_second(const f32x4 (&c)[4], double 
 // the wave holds: full 16-row blocks through the software pipeline, then one
 // masked block.  `ld` loads the A fragments of the block starting at a row
 // (ld.full(rb, a): all 16 rows valid; ld.masked(rb, r1, a): rows >= r1 read
-// as zeros).  acc[wb] accumulates this lane's four rows of column block wb.
+// as zeros; Loader::vec is the operand type).  acc[wb] accumulates this
+// lane's four rows of column block wb.
 template <int D, int NS, class Loader>
 __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
-                                              const bf16x8 (&bfrag)[4][D / 32][NS], unsigned r0,
+                                              const typename Loader::vec (&bfrag)[4][D / 32][NS],
+                                              unsigned r0,
                                               unsigned r1, unsigned lg, double (&acc)[4]) {
     constexpr int KC = D / 32;
     const unsigned rfull = r0 + (r1 - r0) / 16 * 16;
@@ -180,7 +189,7 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
         // clamped to it (loaded, never used).
         const unsigned rlast = rfull - 16;
         auto clamp = [&](unsigned rb) { return rb <= rlast ? rb : rlast; };
-        bf16x8 a0[KC], a1[KC];
+        typename Loader::vec a0[KC], a1[KC];
         ld.full(r0, a0);
         ld.full(clamp(r0 + 16), a1);
         glm_mfma_block<D, NS>(a0, bfrag, c0);
@@ -202,7 +211,7 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
         }
     }
     if (rfull < r1) {
-        bf16x8 a[KC];
+        typename Loader::vec a[KC];
         ld.masked(rfull, r1, a);
         f32x4 c[4];
         glm_mfma_block<D, NS>(a, bfrag, c);
