@@ -273,6 +273,8 @@ struct GlmData {
     DevBuf lin;               // logistic: double[d]: X'y; linear regression: the
                               // four constants of its log posterior
     DevBuf lin_half;          // logistic: double[d]: X'y - (1/2) X'1 (log2-unit kernels)
+    DevBuf X16;               // logistic: fp16 copy of X, made on first fp16 use
+    mutable unsigned bpc[BH_GLM_FP16X2 + 1] = {};  // resident work-groups per CU, by precision
     DevBuf Xf, y;             // linear regression: fp32 N x D and N
 };
 
@@ -292,7 +294,8 @@ int glm_contract(const GlmData& g, int prec, hipStream_t st, const float* P, uns
         BH_HIP(bh::launch_glm_gauss(prec, st, g.Xf.as<float>(), g.y.as<float>(), g.N, g.D, P,
                                     pstride, g.d, n, nranges, rpr, partial));
     } else {
-        BH_HIP(bh::launch_glm_logistic(prec, st, g.X.p, g.N, g.d, P, ys, n, nranges, rpr, yscale,
+        const void* X = prec == BH_GLM_FP16X2 ? g.X16.p : g.X.p;
+        BH_HIP(bh::launch_glm_logistic(prec, st, X, g.N, g.d, P, ys, n, nranges, rpr, yscale,
                                        partial));
     }
     return BH_OK;
@@ -350,7 +353,7 @@ struct bh_stretch {
     DevBuf lp, gY, gz, gu2, gsplit, gpartial;
     // guarded accept test of the reduced precisions (guard 0: off)
     double glm_guard = 0.0;
-    DevBuf gflag, glist, gcount, gtotals, gpart64;
+    DevBuf gflag, glist, gcount, gtotals, gpart64, gpts64;
     // event pairs around the dominant kernel's launches (the stretch move
     // kernels, or the GLM likelihood contraction), up to the pool's size
     bool ktiming = false;
@@ -360,10 +363,13 @@ struct bh_stretch {
 
 constexpr size_t kKernelTimerPairs = 1024;
 
+// rounds of work-groups per logistic likelihood launch
+constexpr unsigned kGlmRounds = 4;
+
 // Guard fallback: walkers re-evaluated in fp64 per half-step at most, and
 // the row ranges of that evaluation.
 constexpr unsigned kGuardListMax = 1024;
-constexpr unsigned kGuardRanges = 256;
+constexpr unsigned kGuardRanges = 1024;
 constexpr unsigned kGuardResolveBlocks = 16;
 
 namespace {
@@ -463,13 +469,50 @@ int run_resident(bh_stretch* s, int mode, uint32_t seed_odd, uint32_t seed_even,
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr float kLog2e = 1.44269504088896340736f;
 
-int glm_nsplit(int prec) { return prec == BH_GLM_BF16X3 ? 3 : (prec == BH_GLM_BF16 ? 1 : 0); }
+int glm_nsplit(int prec) {
+    return prec == BH_GLM_BF16X3 ? 3 : (prec == BH_GLM_FP16X2 ? 2 : (prec == BH_GLM_BF16 ? 1 : 0));
+}
+int glm_f16(int prec) { return prec == BH_GLM_FP16X2 ? 1 : 0; }
 
-// row ranges of one likelihood launch over n points: a multiple of 8 that
-// puts about four blocks on every compute unit
+// Checks a precision against the model and makes what it needs: the fp16
+// copy of X for BH_GLM_FP16X2 (refused when a value of X is outside the
+// fp16 range).
+int glm_prepare(bh_stretch_factory* f, int prec) {
+    BH_ARG(prec >= BH_GLM_FP64 && prec <= BH_GLM_FP16X2, "bad precision %d", prec);
+    GlmData& g = *f->glm;
+    BH_ARG(g.kind != BH_GLM_GAUSSIAN || prec <= BH_GLM_FP32,
+           "linear-regression models run in fp64 or fp32");
+    if (prec != BH_GLM_FP16X2 || g.X16.p) return BH_OK;
+    bh_ctx* c = f->ctx;
+    BH_HIP(hipSetDevice(c->device));
+    const unsigned long long count = g.N * g.d;
+    DevBuf x16, over;
+    BH_TRY(x16.alloc(count * sizeof(uint16_t)));
+    BH_TRY(over.alloc(sizeof(unsigned)));
+    BH_HIP(hipMemsetAsync(over.p, 0, sizeof(unsigned), c->stream));
+    BH_HIP(bh::launch_glm_x_to_f16(c->stream, g.X.p, count, x16.p, over.as<unsigned>()));
+    unsigned bad = 0;
+    BH_HIP(hipMemcpyAsync(&bad, over.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    BH_TRY(sync(c->stream));
+    BH_ARG(bad == 0, "%u values of X are outside the fp16 range (|x| > 65504)", bad);
+    std::swap(g.X16.p, x16.p);
+    std::swap(g.X16.bytes, x16.bytes);
+    return BH_OK;
+}
+
+// row ranges of one likelihood launch over n points: a multiple of 8.  The
+// logistic kernels get whole rounds of work-groups (the compute units times
+// the work-groups one holds at once for the kernel's registers): a partial
+// last round leaves compute units idle; a few rounds of smaller ranges even
+// out the ranges' finishing times.
 unsigned glm_ranges(const bh_ctx* c, const GlmData& g, int prec, unsigned n) {
-    const unsigned groups = n / bh::glm_point_quantum(prec);
-    unsigned r = (4u * (unsigned)c->cu + groups - 1) / std::max(groups, 1u);
+    const unsigned groups = std::max(n / bh::glm_point_quantum(prec), 1u);
+    unsigned blocks = 4u * (unsigned)c->cu;
+    if (g.kind == BH_GLM_LOGISTIC) {
+        if (!g.bpc[prec]) g.bpc[prec] = bh::glm_blocks_per_cu(prec, g.d);
+        blocks = kGlmRounds * g.bpc[prec] * (unsigned)c->cu;
+    }
+    unsigned r = (blocks + groups - 1) / groups;
     r = (r + 7) / 8 * 8;
     const unsigned long long max_r = std::max(8ull, (g.N / 64 + 7) / 8 * 8);
     if (r > max_r) r = (unsigned)max_r;
@@ -484,10 +527,11 @@ int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
     const unsigned nranges = glm_ranges(s->ctx, g, prec, n);
     const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
     int nsplit = glm_nsplit(prec);
+    int f16 = glm_f16(prec);
     float yscale = kLog2e;
-    __bf16* ys = s->gsplit.as<__bf16>();
+    void* ys = s->gsplit.p;
     if (nsplit) {
-        void* a[] = {&P, &n, &ys, &yscale, &nsplit};
+        void* a[] = {&P, &n, &ys, &yscale, &nsplit, &f16};
         BH_TRY(launch(f->glm_split, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a));
     }
     double* partial = s->gpartial.as<double>();
@@ -514,15 +558,16 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     float* Y = s->gY.as<float>();
     float* z = s->gz.as<float>();
     float* u2 = s->gu2.as<float>();
-    __bf16* ys = s->gsplit.as<__bf16>();
+    void* ys = s->gsplit.p;
     int partner = s->partner;
     int nsplit = glm_nsplit(prec);
+    int f16 = glm_f16(prec);
     float yscale = kLog2e;
     const float* Xc = X;
     const bool guarded = s->glm_guard > 0.0 && prec != BH_GLM_FP64;
     unsigned* gcount = guarded ? s->gcount.as<unsigned>() : nullptr;
     void* pa[] = {&seed, &oe, &S, &Xc, &a, &step, &K, &partner, &Y, &z, &u2, &ys, &yscale, &nsplit,
-                  &gcount};
+                  &f16, &gcount};
     BH_TRY(launch(f->glm_propose, K / s->wgs, s->wgs, 0, s->stream, pa));
     unsigned nranges = glm_ranges(s->ctx, g, prec, K);
     const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
@@ -549,11 +594,12 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         unsigned lmax = kGuardListMax;
         double guard = s->glm_guard * std::max(1.0f, beta);
         const double* lpc = lp;
+        double* pts64 = s->gpts64.as<double>();
         void* sa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &lpc, &beta, &K, &guard,
-                      &flag, &list, &gcount, &lmax};
+                      &flag, &list, &gcount, &lmax, &Xc, &pts64};
         BH_TRY(launch(f->glm_screen, K / s->wgs, s->wgs, 0, s->stream, sa));
         double* part64 = s->gpart64.as<double>();
-        BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, Y, X, list, gcount, lmax,
+        BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, pts64, gcount, lmax,
                                         kGuardRanges, part64));
         const double* p64 = part64;
         unsigned nr64 = kGuardRanges, stride = 2 * kGuardListMax;
@@ -576,7 +622,7 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
 int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
     const GlmData& g = *s->f->glm;
     size_t maxr = 0;
-    for (int prec = BH_GLM_FP64; prec <= BH_GLM_BF16X3; ++prec)
+    for (int prec = BH_GLM_FP64; prec <= BH_GLM_FP16X2; ++prec)
         for (unsigned n : {n_points, k_half})
             if (n && n % bh::glm_point_quantum(prec) == 0)
                 maxr = std::max(maxr, (size_t)glm_ranges(s->ctx, g, prec, n) * n);
@@ -591,6 +637,7 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
         BH_TRY(s->gcount.alloc(sizeof(unsigned)));
         BH_TRY(s->gtotals.alloc(2 * sizeof(unsigned long long)));
         BH_TRY(s->gpart64.alloc((size_t)kGuardRanges * 2 * kGuardListMax * sizeof(double)));
+        BH_TRY(s->gpts64.alloc((size_t)2 * kGuardListMax * g.d * sizeof(double)));
         BH_HIP(hipMemsetAsync(s->gtotals.p, 0, 2 * sizeof(unsigned long long), s->stream));
     }
     return BH_OK;
@@ -1235,10 +1282,8 @@ int bh_glm_log_density(bh_stretch_factory* f, const float* beta, uint64_t n, int
                        double* out) {
     BH_ARG(f && beta && out, "NULL argument");
     BH_ARG(f->glm, "not a GLM factory");
-    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
-    BH_ARG(f->glm->kind != BH_GLM_GAUSSIAN || precision <= BH_GLM_FP32,
-           "linear-regression models run in fp64 or fp32");
     BH_ARG(n > 0 && n < (1ull << 26), "bad point count");
+    BH_TRY(glm_prepare(f, precision));
     bh_ctx* c = f->ctx;
     BH_HIP(hipSetDevice(c->device));
     const unsigned q = bh::glm_point_quantum(precision);
@@ -1384,9 +1429,7 @@ int bh_stretch_set_exec(bh_stretch* s, int mode) {
 int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
     BH_ARG(s, "NULL sampler");
     BH_ARG(s->f->glm, "not a GLM sampler");
-    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
-    BH_ARG(s->f->glm->kind != BH_GLM_GAUSSIAN || precision <= BH_GLM_FP32,
-           "linear-regression samplers run in fp64 or fp32");
+    BH_TRY(glm_prepare(s->f, precision));
     s->glm_prec = precision;
     return BH_OK;
 }
@@ -1799,7 +1842,7 @@ int gb_logpost_all(bh_glm_batch* b) {
     const int ns = glm_nsplit(b->prec);
     BH_HIP(bh::launch_gb_points(b->stream, b->xs.as<float>(), b->dims.as<unsigned>(),
                                 b->tscale.as<float>(), b->P, b->W, b->ysc.as<float>(),
-                                b->ysplit.p, ns));
+                                b->ysplit.p, ns, glm_f16(b->prec)));
     BH_TRY(gb_contract(b, b->xs.as<float>(), b->W));
     BH_HIP(bh::launch_gb_logpost(b->stream, b->xs.as<float>(), b->dims.as<unsigned>(),
                                  b->ubeg.as<unsigned>(), b->partial.as<double>(), b->P, b->W,
@@ -1820,7 +1863,7 @@ int gb_steps(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a, int accu,
                                          b->dims.as<unsigned>(), b->tscale.as<float>(),
                                          b->xs.as<float>(), b->P, b->K, b->Y.as<float>(),
                                          b->z.as<float>(), b->u2.as<float>(), b->ysc.as<float>(),
-                                         b->ysplit.p, ns));
+                                         b->ysplit.p, ns, glm_f16(b->prec)));
             BH_TRY(gb_contract(b, b->Y.as<float>(), b->K));
             BH_HIP(bh::launch_gb_accept(b->stream, half, b->dims.as<unsigned>(),
                                         b->ubeg.as<unsigned>(), b->Y.as<float>(),
@@ -1990,7 +2033,7 @@ int bh_glm_batch_destroy(bh_glm_batch* b) {
 
 int bh_glm_batch_set_precision(bh_glm_batch* b, int precision) {
     BH_ARG(b, "NULL batch");
-    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_FP16X2, "bad precision %d", precision);
     b->prec = precision;
     return BH_OK;
 }
@@ -2077,7 +2120,7 @@ int bh_glm_batch_state_get(const bh_glm_batch* b, float* xs, double* lp) {
 
 int bh_glm_batch_log_density(bh_glm_batch* b, int precision, double* lp) {
     BH_ARG(b && lp, "NULL argument");
-    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_BF16X3, "bad precision %d", precision);
+    BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_FP16X2, "bad precision %d", precision);
     BH_HIP(hipSetDevice(b->ctx->device));
     DevBuf keep;
     BH_TRY(keep.alloc((size_t)b->P * b->W * 8));

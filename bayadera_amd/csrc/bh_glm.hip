@@ -32,47 +32,48 @@ __device__ __forceinline__ void glm_tile(unsigned groups, unsigned& group, unsig
     range = (idx / groups) * 8 + (b & 7);
 }
 
-// A fragments from bf16 X (row-major N x D): lane row lr, features
-// kc*32 + lg*8 .. +7.
-template <int D>
-struct Bf16Rows {
-    using vec = bf16x8;
-    const __bf16* X;
+// A fragments from 16-bit X (bf16 or fp16, row-major N x D): lane row lr,
+// features kc*32 + lg*8 .. +7.
+template <int D, class T, class V>
+struct Rows16 {
+    using vec = V;
+    const T* X;
     unsigned lr, lg;
-    __device__ void full(unsigned rb, bf16x8 (&a)[D / 32]) const {
-        const __bf16* p = X + (size_t)(rb + lr) * D + lg * 8;
+    __device__ void full(unsigned rb, V (&a)[D / 32]) const {
+        const T* p = X + (size_t)(rb + lr) * D + lg * 8;
 #pragma unroll
-        for (int kc = 0; kc < D / 32; ++kc) a[kc] = *reinterpret_cast<const bf16x8*>(p + kc * 32);
+        for (int kc = 0; kc < D / 32; ++kc) a[kc] = *reinterpret_cast<const V*>(p + kc * 32);
     }
-    __device__ void masked(unsigned rb, unsigned r1, bf16x8 (&a)[D / 32]) const {
+    __device__ void masked(unsigned rb, unsigned r1, V (&a)[D / 32]) const {
         const unsigned row = rb + lr;
 #pragma unroll
         for (int kc = 0; kc < D / 32; ++kc) {
             if (row < r1) {
-                a[kc] = *reinterpret_cast<const bf16x8*>(X + (size_t)row * D + kc * 32 + lg * 8);
+                a[kc] = *reinterpret_cast<const V*>(X + (size_t)row * D + kc * 32 + lg * 8);
             } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) a[kc][j] = (__bf16)0.0f;
+                for (int j = 0; j < 8; ++j) a[kc][j] = (T)0.0f;
             }
         }
     }
 };
 
-// MFMA kernel: v_mfma_f32_16x16x32_bf16.  A block of 4 waves owns 256
-// points, 64 per wave as four 16-point column blocks whose B fragments (all
-// D features, NS split terms) stay in registers for the whole row range.
-// Lane layout (16x16x32, checked on the device): A lane l holds row l%16,
-// features (l/16)*8..+7 of a 32-feature chunk; B lane l holds point l%16,
-// the same features; the accumulator lane l holds rows (l/16)*4..+3 of
-// point l%16.  The row loop (glm_mfma_rows) is software-pipelined: the
-// matrix-core products of row block i+1 are issued with the softplus
-// epilogue of block i placed between them.  Split terms are accumulated
-// smallest first.  Rows are masked only in the last block.
-template <int D, int NS>
+// MFMA kernel: v_mfma_f32_16x16x32_bf16 (T = __bf16) or _f16 (T =
+// _Float16).  A block of 4 waves owns 256 points, 64 per wave as four
+// 16-point column blocks whose B fragments (all D features, NS split terms)
+// stay in registers for the whole row range.  Lane layout (16x16x32, checked
+// on the device): A lane l holds row l%16, features (l/16)*8..+7 of a
+// 32-feature chunk; B lane l holds point l%16, the same features; the
+// accumulator lane l holds rows (l/16)*4..+3 of point l%16.  The row loop
+// (glm_mfma_rows) is software-pipelined: the matrix-core products of row
+// block i+1 are issued with the softplus epilogue of block i placed between
+// them.  Split terms are accumulated smallest first.  Rows are masked only in
+// the last block.
+template <int D, int NS, class T, class V>
 __global__ void __launch_bounds__(256)
-glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned N,
-                         const __bf16* __restrict__ ys, unsigned n, unsigned groups,
-                         unsigned nranges, unsigned rows_per_range, double* __restrict__ partial) {
+glm_logistic_mfma_kernel(const T* __restrict__ X, unsigned N, const T* __restrict__ ys, unsigned n,
+                         unsigned groups, unsigned nranges, unsigned rows_per_range,
+                         double* __restrict__ partial) {
     constexpr int KC = D / 32;
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
@@ -82,20 +83,20 @@ glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned N,
     const unsigned w0 = group * 256u + wave * 64u;
     const unsigned lr = lane & 15u, lg = lane >> 4;
 
-    bf16x8 bfrag[4][KC][NS];
+    V bfrag[4][KC][NS];
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb)
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-                bfrag[wb][kc][s] = *reinterpret_cast<const bf16x8*>(
+                bfrag[wb][kc][s] = *reinterpret_cast<const V*>(
                     ys + ((size_t)s * n + w0 + wb * 16 + lr) * D + kc * 32 + lg * 8);
 
     const unsigned r0 = range * rows_per_range;
     const unsigned r1 = (r0 >= N) ? r0 : ((N - r0 < rows_per_range) ? N : r0 + rows_per_range);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (r0 < r1) glm_mfma_rows<D, NS>(Bf16Rows<D>{X, lr, lg}, bfrag, r0, r1, lg, acc);
+    if (r0 < r1) glm_mfma_rows<D, NS>(Rows16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
         double v = acc[wb];
@@ -103,6 +104,26 @@ glm_logistic_mfma_kernel(const __bf16* __restrict__ X, unsigned N,
         v += __shfl_xor(v, 32, 64);
         if (lg == 0) partial[(size_t)range * n + w0 + wb * 16 + lr] = v;
     }
+}
+
+// fp16 copy of bf16 X for the fp16 matrix-core kernels, eight values per
+// thread; counts the values outside the fp16 range (|x| > 65504).
+__global__ void __launch_bounds__(256)
+glm_x_to_f16_kernel(const __bf16* __restrict__ X, unsigned long long n8,
+                    _Float16* __restrict__ out, unsigned* __restrict__ overflow) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n8) return;
+    const bf16x8 v = reinterpret_cast<const bf16x8*>(X)[i];
+    f16x8 h;
+    unsigned bad = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float f = (float)v[j];
+        bad += __builtin_fabsf(f) > 65504.0f ? 1u : 0u;
+        h[j] = (_Float16)f;
+    }
+    reinterpret_cast<f16x8*>(out)[i] = h;
+    if (bad) atomicAdd(overflow, bad);
 }
 
 // Plain fp32 kernel (no matrix cores): one point per thread, its D
@@ -300,94 +321,89 @@ glm_gauss_fp64_kernel(const float* __restrict__ X, const float* __restrict__ yv,
     partial[(size_t)range * n + k] = acc;
 }
 
-// fp64 softplus sums of listed points (the guarded accept test's fallback).
-// Block r owns row range r; the 2c listed points (c = min(*count, lmax))
-// run eight at a time, their coefficients in LDS (double).  Each thread
-// takes rows of the range one at a time, converts the row to double once,
-// and evaluates the eight points on it: eight independent fma chains, then
-// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)) in double as the fp64
-// kernel does.  Per-point sums are reduced over the block.
+// fp64 log-likelihood sums of the points listed by the guarded accept test.
+// pts holds them in double, row i < c a proposal and row lmax + i its
+// walker's position (c = min(*count, lmax), read on the device).  A block
+// takes one row range; each thread a row at a time, the row converted to
+// double eight features at a time, against eight points whose coefficients
+// are uniform across the wave (scalar loads).  The same arithmetic as the
+// fp64 kernel: fma chains over the features in order, then
+// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)).  partial64[r * 2 lmax + i]
+// gets the block's sum for point row i.
 template <int D>
 __global__ void __launch_bounds__(256)
 glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
-                     const float* __restrict__ Y, const float* __restrict__ W,
-                     const unsigned* __restrict__ list, const unsigned* __restrict__ count,
+                     const double* __restrict__ pts, const unsigned* __restrict__ count,
                      unsigned lmax, unsigned long long rows_per_range,
                      double* __restrict__ partial64) {
     constexpr int PC = 8;
-    __shared__ double b[PC][D];
     __shared__ double red[PC][4];
     const unsigned n = *count;
     const unsigned c = n < lmax ? n : lmax;
     if (c == 0) return;
-    const unsigned np = 2 * c;
     const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_range;
     unsigned long long r1 = r0 + rows_per_range;
     if (r1 > N) r1 = N;
-    for (unsigned p0 = 0; p0 < np; p0 += PC) {
-        __syncthreads();
-        for (unsigned e = t; e < PC * D; e += 256) {
-            const unsigned p = p0 + e / D;
-            double v = 0.0;
-            if (p < np) {
-                const unsigned w = list[p < c ? p : p - c];
-                v = (double)(p < c ? Y : W)[(size_t)w * D + e % D];
-            }
-            b[e / D][e % D] = v;
-        }
-        __syncthreads();
-        double acc[PC];
+    const unsigned chunks = (c + PC - 1) / PC;
+    for (unsigned h = 0; h < 2; ++h) {
+        for (unsigned ch = 0; ch < chunks; ++ch) {
+            const unsigned p0 = h * lmax + ch * PC;
+            const double* b = pts + (size_t)p0 * D;
+            double acc[PC];
 #pragma unroll
-        for (int q = 0; q < PC; ++q) acc[q] = 0.0;
-        for (unsigned long long row = r0 + t; row < r1; row += 256) {
-            // the coefficients are read from LDS row by row (a compiler
-            // barrier keeps them from being hoisted into registers)
-            asm volatile("" ::: "memory");
-            const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
-            double eta[PC];
+            for (int q = 0; q < PC; ++q) acc[q] = 0.0;
+            for (unsigned long long row = r0 + t; row < r1; row += 256) {
+                // the coefficients are read again for every row (a compiler
+                // barrier keeps all 8 x D of them from being hoisted into
+                // scalar registers, which would spill)
+                asm volatile("" ::: "memory");
+                const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
+                double eta[PC];
 #pragma unroll
-            for (int q = 0; q < PC; ++q) eta[q] = 0.0;
+                for (int q = 0; q < PC; ++q) eta[q] = 0.0;
 #pragma unroll
-            for (int v = 0; v < D / 8; ++v) {
-                const bf16x8 xv = xr[v];
+                for (int v = 0; v < D / 8; ++v) {
+                    const bf16x8 xv = xr[v];
+                    double x[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const double x = (double)(float)xv[e];
+                    for (int e = 0; e < 8; ++e) x[e] = (double)(float)xv[e];
 #pragma unroll
-                    for (int q = 0; q < PC; ++q) eta[q] = __builtin_fma(x, b[q][v * 8 + e], eta[q]);
+                    for (int q = 0; q < PC; ++q)
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            eta[q] = __builtin_fma(x[e], b[q * D + v * 8 + e], eta[q]);
                 }
+#pragma unroll
+                for (int q = 0; q < PC; ++q)
+                    acc[q] += __builtin_fmax(eta[q], 0.0) + log1p(exp(-__builtin_fabs(eta[q])));
             }
 #pragma unroll
-            for (int q = 0; q < PC; ++q)
-                acc[q] += __builtin_fmax(eta[q], 0.0) + log1p(exp(-__builtin_fabs(eta[q])));
-        }
+            for (int q = 0; q < PC; ++q) {
+                double v = acc[q];
 #pragma unroll
-        for (int q = 0; q < PC; ++q) {
-            double v = acc[q];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) red[q][wave] = v;
+                for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                if (lane == 0) red[q][wave] = v;
+            }
+            __syncthreads();
+            if (t < PC && ch * PC + t < c)
+                partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
+                    (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+            __syncthreads();
         }
-        __syncthreads();
-        if (t < PC && p0 + t < np)
-            partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
-                (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
     }
 }
 
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
-                                const float* Y, const float* W, const unsigned* list,
-                                const unsigned* count, unsigned lmax, unsigned nranges,
-                                double* partial64) {
+                                const double* pts, const unsigned* count, unsigned lmax,
+                                unsigned nranges, double* partial64) {
+    if (lmax % 8) return hipErrorInvalidValue;
     const unsigned long long rpr = (N + nranges - 1) / nranges;
     const __bf16* x = static_cast<const __bf16*>(X);
     if (d == 64)
-        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
-                                                         partial64);
+        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, pts, count, lmax, rpr, partial64);
     else if (d == 32)
-        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, Y, W, list, count, lmax, rpr,
-                                                         partial64);
+        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, pts, count, lmax, rpr, partial64);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -427,20 +443,30 @@ glm_lin_final_kernel(const double* __restrict__ part, unsigned chunks, unsigned 
 
 // --------------------------------------------------------------------------
 template <int D>
-static hipError_t launch_d(int precision, hipStream_t s, const __bf16* X, unsigned long long N,
-                           const float* P, const __bf16* ys, unsigned n, unsigned nranges,
+static hipError_t launch_d(int precision, hipStream_t s, const void* Xv, unsigned long long N,
+                           const float* P, const void* ysv, unsigned n, unsigned nranges,
                            unsigned long long rpr, float yscale, double* partial) {
+    const __bf16* X = static_cast<const __bf16*>(Xv);
+    const __bf16* ys = static_cast<const __bf16*>(ysv);
+    const _Float16* X16 = static_cast<const _Float16*>(Xv);
+    const _Float16* ys16 = static_cast<const _Float16*>(ysv);
     switch (precision) {
     case kGlmBf16: {
         const unsigned groups = n / 256;
-        glm_logistic_mfma_kernel<D, 1><<<groups * nranges, 256, 0, s>>>(
+        glm_logistic_mfma_kernel<D, 1, __bf16, bf16x8><<<groups * nranges, 256, 0, s>>>(
             X, (unsigned)N, ys, n, groups, nranges, (unsigned)rpr, partial);
         break;
     }
     case kGlmBf16x3: {
         const unsigned groups = n / 256;
-        glm_logistic_mfma_kernel<D, 3><<<groups * nranges, 256, 0, s>>>(
+        glm_logistic_mfma_kernel<D, 3, __bf16, bf16x8><<<groups * nranges, 256, 0, s>>>(
             X, (unsigned)N, ys, n, groups, nranges, (unsigned)rpr, partial);
+        break;
+    }
+    case kGlmFp16x2: {
+        const unsigned groups = n / 256;
+        glm_logistic_mfma_kernel<D, 2, _Float16, f16x8><<<groups * nranges, 256, 0, s>>>(
+            X16, (unsigned)N, ys16, n, groups, nranges, (unsigned)rpr, partial);
         break;
     }
     case kGlmFp32: {
@@ -468,15 +494,56 @@ unsigned glm_point_quantum(int precision) {
     return 256u;
 }
 
+unsigned glm_blocks_per_cu(int precision, unsigned d) {
+    const void* k = nullptr;
+    switch (precision) {
+    case kGlmBf16:
+        k = d == 64 ? (const void*)glm_logistic_mfma_kernel<64, 1, __bf16, bf16x8>
+                    : (const void*)glm_logistic_mfma_kernel<32, 1, __bf16, bf16x8>;
+        break;
+    case kGlmBf16x3:
+        k = d == 64 ? (const void*)glm_logistic_mfma_kernel<64, 3, __bf16, bf16x8>
+                    : (const void*)glm_logistic_mfma_kernel<32, 3, __bf16, bf16x8>;
+        break;
+    case kGlmFp16x2:
+        k = d == 64 ? (const void*)glm_logistic_mfma_kernel<64, 2, _Float16, f16x8>
+                    : (const void*)glm_logistic_mfma_kernel<32, 2, _Float16, f16x8>;
+        break;
+    case kGlmFp32:
+        k = d == 64 ? (const void*)glm_logistic_fp32_kernel<64>
+                    : (const void*)glm_logistic_fp32_kernel<32>;
+        break;
+    case kGlmFp64:
+        k = d == 64 ? (const void*)glm_logistic_fp64_kernel<64>
+                    : (const void*)glm_logistic_fp64_kernel<32>;
+        break;
+    default:
+        return 1;
+    }
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0) != hipSuccess ||
+        blocks < 1)
+        return 1;
+    return (unsigned)blocks;
+}
+
 hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsigned long long N,
                                unsigned d, const float* P, const void* ys, unsigned n,
                                unsigned nranges, unsigned long long rows_per_range, float yscale,
                                double* partial) {
-    const __bf16* x = static_cast<const __bf16*>(X);
-    const __bf16* y = static_cast<const __bf16*>(ys);
-    if (d == 64) return launch_d<64>(precision, s, x, N, P, y, n, nranges, rows_per_range, yscale, partial);
-    if (d == 32) return launch_d<32>(precision, s, x, N, P, y, n, nranges, rows_per_range, yscale, partial);
+    if (d == 64) return launch_d<64>(precision, s, X, N, P, ys, n, nranges, rows_per_range, yscale, partial);
+    if (d == 32) return launch_d<32>(precision, s, X, N, P, ys, n, nranges, rows_per_range, yscale, partial);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long count, void* out,
+                               unsigned* overflow) {
+    if (count % 8) return hipErrorInvalidValue;
+    const unsigned long long n8 = count / 8;
+    if (n8 == 0) return hipSuccess;
+    glm_x_to_f16_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, s>>>(
+        static_cast<const __bf16*>(X), n8, static_cast<_Float16*>(out), overflow);
+    return hipGetLastError();
 }
 
 hipError_t launch_glm_gauss(int precision, hipStream_t s, const float* X, const float* yv,

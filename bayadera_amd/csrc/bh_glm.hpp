@@ -9,25 +9,33 @@ constexpr int kGlmFp64 = 0;
 constexpr int kGlmFp32 = 1;
 constexpr int kGlmBf16 = 2;
 constexpr int kGlmBf16x3 = 3;
+constexpr int kGlmFp16x2 = 4;
 
 bool glm_supported_dim(unsigned d);
 // the number of points n of a launch must be a multiple of this
 unsigned glm_point_quantum(int precision);
-// partial[r][k] for r < nranges (a multiple of 8), k < n; X bf16 N x d,
-// P fp32 n x d (fp32/fp64 kernels), ys the bf16 split of yscale*P
-// (NS x n x d, MFMA kernels).
+// work-groups of the logistic likelihood kernel of a precision that one
+// compute unit holds at once (the device's occupancy for its registers)
+unsigned glm_blocks_per_cu(int precision, unsigned d);
+// partial[r][k] for r < nranges (a multiple of 8), k < n; X bf16 N x d
+// (fp16 for kGlmFp16x2), P fp32 n x d (fp32/fp64 kernels), ys the 16-bit
+// split of yscale*P (NS x n x d, MFMA kernels; fp16 terms for kGlmFp16x2).
 hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsigned long long N,
                                unsigned d, const float* P, const void* ys, unsigned n,
                                unsigned nranges, unsigned long long rows_per_range, float yscale,
                                double* partial);
-// fp64 softplus sums of the points listed by the guarded accept test:
-// columns i < c are Y[list[i]], columns c + i are W[list[i]] (c =
-// min(*count, lmax), read on the device), partial64[r * 2 lmax + col] for
-// r < nranges.  Does nothing when the list is empty.
+// fp16 copy of `count` bf16 values (a multiple of 8); *overflow is
+// incremented by the number of values outside the fp16 range.
+hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long count, void* out,
+                               unsigned* overflow);
+// fp64 softplus sums of the points listed by the guarded accept test: pts
+// (double, 2 lmax x d) holds in rows i < c the proposals and in rows
+// lmax + i their walkers' positions (c = min(*count, lmax), read on the
+// device); partial64[r * 2 lmax + row] for r < nranges.  Does nothing when
+// the list is empty.  lmax a multiple of 8.
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
-                                const float* Y, const float* W, const unsigned* list,
-                                const unsigned* count, unsigned lmax, unsigned nranges,
-                                double* partial64);
+                                const double* pts, const unsigned* count, unsigned lmax,
+                                unsigned nranges, double* partial64);
 // Linear regression sums of squared residuals (fp32 or fp64 kernels): X fp32
 // N x D (D = 16, 32 or 64, columns past d zero), points P with row stride
 // pstride whose first d coordinates are the coefficients.

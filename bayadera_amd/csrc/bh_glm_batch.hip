@@ -5,9 +5,10 @@
 // stride D_p = 32 (d_p <= 32) or 64, padding columns zero; walkers are
 // padded to 64 floats.  The contraction runs over "units" (a problem's row
 // range of at most kGbUnitRows rows) x 256-point groups: the MFMA kernel
-// decodes fp8 rows to bf16 in registers (exact: every e4m3 value is a bf16
-// value) and multiplies them with the split-bf16 proposals, whose column
-// scales are folded in (x_ij b_j = q_ij (s_j b_j)).  The walker-side kernels
+// decodes fp8 rows to bf16 or fp16 in registers (exact: every e4m3 value is
+// a bf16 and an fp16 value) and multiplies them with the split proposals
+// (three bf16 or two fp16 terms), whose column scales are folded in
+// (x_ij b_j = q_ij (s_j b_j)).  The walker-side kernels
 // follow the single-posterior GLM sampler: the same proposal and accept
 // arithmetic, with each problem's own dimension and seeds.  Partners are
 // whole walkers of the complementary half.
@@ -45,26 +46,44 @@ __device__ __forceinline__ bf16x8 fp8x8_to_bf16x8(uint2 v) {
     return r;
 }
 
-// A fragments from fp8 X (row stride D bytes).
-template <int D>
+// Eight fp8 bytes -> eight fp16 (v_cvt_pkrtz_f16_f32 of the exact floats:
+// every e4m3 value is an fp16 value, so the rounding mode never acts).
+__device__ __forceinline__ f16x8 fp8x8_to_f16x8(uint2 v) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    float f[8];
+    fp8x4_to_f32(v.x, *reinterpret_cast<float(*)[4]>(&f[0]));
+    fp8x4_to_f32(v.y, *reinterpret_cast<float(*)[4]>(&f[4]));
+    u32x4 w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        w[i] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(f[2 * i], f[2 * i + 1]));
+    return __builtin_bit_cast(f16x8, w);
+}
+
+__device__ __forceinline__ void fp8x8_decode(uint2 v, bf16x8& a) { a = fp8x8_to_bf16x8(v); }
+__device__ __forceinline__ void fp8x8_decode(uint2 v, f16x8& a) { a = fp8x8_to_f16x8(v); }
+
+// A fragments from fp8 X (row stride D bytes), decoded to V (bf16x8 or
+// f16x8).
+template <int D, class V>
 struct Fp8Rows {
-    using vec = bf16x8;
+    using vec = V;
     const unsigned char* X;
     unsigned lr, lg;
-    __device__ void full(unsigned rb, bf16x8 (&a)[D / 32]) const {
+    __device__ void full(unsigned rb, V (&a)[D / 32]) const {
         const unsigned char* p = X + (size_t)(rb + lr) * D + lg * 8;
 #pragma unroll
         for (int kc = 0; kc < D / 32; ++kc)
-            a[kc] = fp8x8_to_bf16x8(*reinterpret_cast<const uint2*>(p + kc * 32));
+            fp8x8_decode(*reinterpret_cast<const uint2*>(p + kc * 32), a[kc]);
     }
-    __device__ void masked(unsigned rb, unsigned r1, bf16x8 (&a)[D / 32]) const {
+    __device__ void masked(unsigned rb, unsigned r1, V (&a)[D / 32]) const {
         const unsigned row = rb + lr;
 #pragma unroll
         for (int kc = 0; kc < D / 32; ++kc) {
             const uint2 v = row < r1 ? *reinterpret_cast<const uint2*>(X + (size_t)row * D +
                                                                        kc * 32 + lg * 8)
                                      : uint2{0u, 0u};
-            a[kc] = fp8x8_to_bf16x8(v);
+            fp8x8_decode(v, a[kc]);
         }
     }
 };
@@ -105,12 +124,13 @@ gb_init_kernel(unsigned seed, const unsigned* __restrict__ dims,
     }
 }
 
-// Writes y * t (t = log2(e) * column scale) as fp32 and as nsplit bf16
-// terms (hi, mid, lo), for point k of problem p.
+// Writes y * t (t = log2(e) * column scale) as fp32 and as nsplit 16-bit
+// terms (bf16, or fp16 when f16 is set; hi, mid, lo), for point k of
+// problem p.
 __device__ __forceinline__ void gb_store_point(const float (&y)[kGbD], const float* __restrict__ t,
                                                unsigned d, unsigned P, unsigned p, unsigned n,
                                                unsigned k, float* __restrict__ ysc,
-                                               __bf16* __restrict__ ysplit, int nsplit) {
+                                               void* __restrict__ ysplit, int nsplit, int f16) {
     float* o = ysc + ((size_t)p * n + k) * kGbD;
 #pragma unroll
     for (int i = 0; i < kGbD; ++i) {
@@ -118,9 +138,16 @@ __device__ __forceinline__ void gb_store_point(const float (&y)[kGbD], const flo
         o[i] = v;
         float r = v;
         for (int sp = 0; sp < nsplit; ++sp) {
-            const __bf16 h = (__bf16)r;
-            ysplit[(((size_t)sp * P + p) * n + k) * kGbD + i] = h;
-            r = r - (float)h;
+            const size_t at = (((size_t)sp * P + p) * n + k) * kGbD + i;
+            if (f16) {
+                const _Float16 h = (_Float16)r;
+                static_cast<_Float16*>(ysplit)[at] = h;
+                r = r - (float)h;
+            } else {
+                const __bf16 h = (__bf16)r;
+                static_cast<__bf16*>(ysplit)[at] = h;
+                r = r - (float)h;
+            }
         }
     }
 }
@@ -130,7 +157,7 @@ __device__ __forceinline__ void gb_store_point(const float (&y)[kGbD], const flo
 __global__ void __launch_bounds__(256)
 gb_points_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dims,
                  const float* __restrict__ tscale, unsigned P, unsigned n,
-                 float* __restrict__ ysc, __bf16* __restrict__ ysplit, int nsplit) {
+                 float* __restrict__ ysc, void* __restrict__ ysplit, int nsplit, int f16) {
     const unsigned p = blockIdx.y;
     const unsigned k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
@@ -138,7 +165,7 @@ gb_points_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dims
     const float* x = xs + ((size_t)p * n + k) * kGbD;
 #pragma unroll
     for (int i = 0; i < kGbD; ++i) y[i] = x[i];
-    gb_store_point(y, tscale + (size_t)p * kGbD, dims[p], P, p, n, k, ysc, ysplit, nsplit);
+    gb_store_point(y, tscale + (size_t)p * kGbD, dims[p], P, p, n, k, ysc, ysplit, nsplit, f16);
 }
 
 // Proposals of one half-step for all problems (the move of
@@ -149,7 +176,7 @@ gb_propose_kernel(unsigned seed0, int half, unsigned oe, unsigned step, float a,
                   const unsigned* __restrict__ dims, const float* __restrict__ tscale,
                   const float* __restrict__ xs, unsigned P, unsigned K, float* __restrict__ Y,
                   float* __restrict__ zbuf, float* __restrict__ u2buf, float* __restrict__ ysc,
-                  __bf16* __restrict__ ysplit, int nsplit) {
+                  void* __restrict__ ysplit, int nsplit, int f16) {
     const unsigned p = blockIdx.y;
     const unsigned k = blockIdx.x * 256 + threadIdx.x;
     const unsigned d = dims[p];
@@ -175,7 +202,7 @@ gb_propose_kernel(unsigned seed0, int half, unsigned oe, unsigned step, float a,
     for (int i = 0; i < kGbD; ++i) yo[i] = y[i];
     zbuf[(size_t)p * K + k] = z;
     u2buf[(size_t)p * K + k] = u.z;
-    gb_store_point(y, tscale + (size_t)p * kGbD, d, P, p, K, k, ysc, ysplit, nsplit);
+    gb_store_point(y, tscale + (size_t)p * kGbD, d, P, p, K, k, ysc, ysplit, nsplit, f16);
 }
 
 // Block -> (unit, 256-point group), XCD-aware as glm_tile: the consecutive
@@ -192,11 +219,11 @@ __device__ __forceinline__ bool gb_tile(unsigned nunits, unsigned groups, unsign
 // MFMA contraction of the units of one row stride D (units: {problem, r0,
 // r1, global unit}); partial[unit][point] as the single-posterior kernel
 // (log2 units, |u|/2 + log2(1 + 2^-|u|) summed over the unit's rows).
-template <int D, int NS>
+template <int D, int NS, class T, class V>
 __global__ void __launch_bounds__(256)
 gb_mfma_kernel(const unsigned char* __restrict__ X8, const unsigned long long* __restrict__ xoff,
                const uint4* __restrict__ units, unsigned nunits, unsigned P,
-               const __bf16* __restrict__ ysplit, unsigned n, unsigned groups,
+               const T* __restrict__ ysplit, unsigned n, unsigned groups,
                double* __restrict__ partial) {
     constexpr int KC = D / 32;
     unsigned ui, g;
@@ -206,19 +233,19 @@ gb_mfma_kernel(const unsigned char* __restrict__ X8, const unsigned long long* _
     const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const unsigned lr = lane & 15u, lg = lane >> 4;
     const unsigned w0 = g * 256u + wave * 64u;
-    bf16x8 bfrag[4][KC][NS];
+    V bfrag[4][KC][NS];
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb)
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-                bfrag[wb][kc][s] = *reinterpret_cast<const bf16x8*>(
+                bfrag[wb][kc][s] = *reinterpret_cast<const V*>(
                     ysplit + (((size_t)s * P + p) * n + w0 + wb * 16 + lr) * kGbD + kc * 32 +
                     lg * 8);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (unit.y < unit.z)
-        glm_mfma_rows<D, NS>(Fp8Rows<D>{X8 + xoff[p], lr, lg}, bfrag, unit.y, unit.z, lg, acc);
+        glm_mfma_rows<D, NS>(Fp8Rows<D, V>{X8 + xoff[p], lr, lg}, bfrag, unit.y, unit.z, lg, acc);
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
         double v = acc[wb];
@@ -460,19 +487,19 @@ hipError_t launch_gb_init(hipStream_t s, unsigned seed, const unsigned* dims, co
 
 hipError_t launch_gb_points(hipStream_t s, const float* xs, const unsigned* dims,
                             const float* tscale, unsigned P, unsigned n, float* ysc, void* ysplit,
-                            int nsplit) {
-    gb_points_kernel<<<dim3((n + 255) / 256, P), 256, 0, s>>>(
-        xs, dims, tscale, P, n, ysc, static_cast<__bf16*>(ysplit), nsplit);
+                            int nsplit, int f16) {
+    gb_points_kernel<<<dim3((n + 255) / 256, P), 256, 0, s>>>(xs, dims, tscale, P, n, ysc, ysplit,
+                                                             nsplit, f16);
     return hipGetLastError();
 }
 
 hipError_t launch_gb_propose(hipStream_t s, unsigned seed0, int half, unsigned oe, unsigned step,
                              float a, const unsigned* dims, const float* tscale, const float* xs,
                              unsigned P, unsigned K, float* Y, float* z, float* u2, float* ysc,
-                             void* ysplit, int nsplit) {
+                             void* ysplit, int nsplit, int f16) {
     gb_propose_kernel<<<dim3(K / 256, P), 256, 0, s>>>(seed0, half, oe, step, a, dims, tscale,
-                                                       xs, P, K, Y, z, u2, ysc,
-                                                       static_cast<__bf16*>(ysplit), nsplit);
+                                                       xs, P, K, Y, z, u2, ysc, ysplit, nsplit,
+                                                       f16);
     return hipGetLastError();
 }
 
@@ -487,15 +514,21 @@ hipError_t launch_gb_contract(int precision, int D, hipStream_t s, const void* X
     const unsigned char* x = static_cast<const unsigned char*>(X8);
     const uint4* un = static_cast<const uint4*>(units);
     const __bf16* ys = static_cast<const __bf16*>(ysplit);
+    const _Float16* ys16 = static_cast<const _Float16*>(ysplit);
 #define BH_GB_CASES(DD)                                                                         \
     switch (precision) {                                                                        \
     case kGlmBf16x3:                                                                            \
-        gb_mfma_kernel<DD, 3><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P, ys, n, groups,     \
-                                                     partial);                                  \
+        gb_mfma_kernel<DD, 3, __bf16, bf16x8><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P,    \
+                                                                     ys, n, groups, partial);   \
+        break;                                                                                  \
+    case kGlmFp16x2:                                                                            \
+        gb_mfma_kernel<DD, 2, _Float16, f16x8><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P,   \
+                                                                      ys16, n, groups,          \
+                                                                      partial);                 \
         break;                                                                                  \
     case kGlmBf16:                                                                              \
-        gb_mfma_kernel<DD, 1><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P, ys, n, groups,     \
-                                                     partial);                                  \
+        gb_mfma_kernel<DD, 1, __bf16, bf16x8><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P,    \
+                                                                     ys, n, groups, partial);   \
         break;                                                                                  \
     case kGlmFp32:                                                                              \
         gb_fp32_kernel<DD><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P, ysc, n, groups,       \

@@ -1,7 +1,7 @@
 // Launchers of the batched GLM kernels (bh_glm_batch.hip).  Layouts: walkers
 // xs[P][W][64] (padded floats, halves contiguous), per-half proposals
-// Y[P][K][64], z/u2[P][K], scaled points ysc[P][n][64] and their bf16 splits
-// [NS][P][n][64]; units {problem, r0, r1, global unit id} (uint4) per row
+// Y[P][K][64], z/u2[P][K], scaled points ysc[P][n][64] and their 16-bit
+// splits [NS][P][n][64] (bf16, or fp16 with f16 set); units {problem, r0, r1, global unit id} (uint4) per row
 // stride; partial[unit][n] (double).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -16,12 +16,13 @@ hipError_t launch_gb_init(hipStream_t s, unsigned seed, const unsigned* dims, co
                           float* xs, unsigned P, unsigned W);
 hipError_t launch_gb_points(hipStream_t s, const float* xs, const unsigned* dims,
                             const float* tscale, unsigned P, unsigned n, float* ysc, void* ysplit,
-                            int nsplit);
+                            int nsplit, int f16);
 hipError_t launch_gb_propose(hipStream_t s, unsigned seed0, int half, unsigned oe, unsigned step,
                              float a, const unsigned* dims, const float* tscale, const float* xs,
                              unsigned P, unsigned K, float* Y, float* z, float* u2, float* ysc,
-                             void* ysplit, int nsplit);
-// precision: kGlmFp64 / kGlmFp32 / kGlmBf16 / kGlmBf16x3; D: 32 or 64.
+                             void* ysplit, int nsplit, int f16);
+// precision: kGlmFp64 / kGlmFp32 / kGlmBf16 / kGlmBf16x3 / kGlmFp16x2; D: 32
+// or 64.
 // Ypts: the unscaled points (fp64 kernel), ysc / ysplit the scaled ones.
 hipError_t launch_gb_contract(int precision, int D, hipStream_t s, const void* X8,
                               const unsigned long long* xoff, const void* units, unsigned nunits,

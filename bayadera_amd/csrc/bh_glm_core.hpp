@@ -75,14 +75,15 @@ __device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double 
 
 // The epilogue of two consecutive 16-row blocks, eight values per lane and
 // column block, with one v_log_f32 for the eight: the product of the eight
-// factors 1 + 2^-|u| lies in [1, 256].  It is cut into pieces of a few VALU
-// instructions so that they can be placed between matrix-core products:
-// five per column block for the first block of the pair (exponentials, the
-// product and |u| sum carried in st), seven for the second (the same, then
-// the logarithm and the double accumulation).
+// factors 1 + 2^-|u| lies in [1, 256] and is built as a chain
+// p <- fma(e, p, p) = p (1 + e), one rounding per factor.  It is cut into
+// pieces of a few VALU instructions so that they can be placed between
+// matrix-core products: five per column block for the first block of the
+// pair (exponentials, product and |u| sum carried in st), seven for the
+// second (the same, then the logarithm and the double accumulation).
 struct GlmEpiState {
-    float p[4];  // product of the first block's four factors, per column block
-    float a[4];  // sum of its four |u|
+    float p[4];  // product of the factors so far, per column block
+    float a[4];  // sum of the |u| so far
 };
 
 template <int H>
@@ -98,12 +99,16 @@ __device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& 
         e[3] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[3]));
         break;
     case 2:
-        e[0] = (1.0f + e[0]) * (1.0f + e[1]);
-        e[2] = (1.0f + e[2]) * (1.0f + e[3]);
+        if (H == 0) {
+            st.p[wb] = 1.0f + e[0];
+        } else {
+            st.p[wb] = __builtin_fmaf(e[0], st.p[wb], st.p[wb]);
+        }
+        st.p[wb] = __builtin_fmaf(e[1], st.p[wb], st.p[wb]);
         break;
     case 3:
-        if (H == 0) st.p[wb] = e[0] * e[2];
-        else st.p[wb] = st.p[wb] * (e[0] * e[2]);
+        st.p[wb] = __builtin_fmaf(e[2], st.p[wb], st.p[wb]);
+        st.p[wb] = __builtin_fmaf(e[3], st.p[wb], st.p[wb]);
         break;
     case 4:
         e[1] = (__builtin_fabsf(c[0]) + __builtin_fabsf(c[1])) +

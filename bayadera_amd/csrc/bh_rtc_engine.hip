@@ -642,28 +642,36 @@ bh_batch_logfn(const unsigned* __restrict__ dims, const unsigned long long* __re
 // walker-side halves: the proposal, and the accept test with its accounting,
 // with the same arithmetic as bh_stretch_walker.
 
-// Writes yscale*Y as nsplit bf16 terms, hi + mid + lo, each the nearest
-// bf16 of what the previous terms leave (exact residuals in fp32).
+// Writes yscale*Y as nsplit 16-bit terms (bf16, or fp16 when f16 is set),
+// hi + mid + lo, each the nearest value of what the previous terms leave
+// (exact residuals in fp32).
 __device__ __forceinline__ void bh_glm_split(const float (&y)[DIM], float yscale, int nsplit,
-                                             __bf16* out, unsigned k, unsigned n) {
+                                             int f16, void* out, unsigned k, unsigned n) {
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         float r = y[i] * yscale;
+        const size_t o = (size_t)k * DIM + i;
         for (int sp = 0; sp < nsplit; ++sp) {
-            const __bf16 h = (__bf16)r;
-            out[((size_t)sp * n + k) * DIM + i] = h;
-            r = r - (float)h;
+            if (f16) {
+                const _Float16 h = (_Float16)r;
+                static_cast<_Float16*>(out)[(size_t)sp * n * DIM + o] = h;
+                r = r - (float)h;
+            } else {
+                const __bf16 h = (__bf16)r;
+                static_cast<__bf16*>(out)[(size_t)sp * n * DIM + o] = h;
+                r = r - (float)h;
+            }
         }
     }
 }
 
-// Proposals of one half-step: Y (fp32), z and u2 per walker, and the bf16
+// Proposals of one half-step: Y (fp32), z and u2 per walker, and the 16-bit
 // split of log2(e)*Y for the MFMA kernels; clears the guard list's count.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                const float* __restrict__ X, float a, unsigned step, unsigned K, int partner,
                float* __restrict__ Y, float* __restrict__ zbuf, float* __restrict__ u2buf,
-               __bf16* __restrict__ ysplit, float yscale, int nsplit,
+               void* __restrict__ ysplit, float yscale, int nsplit, int f16,
                unsigned* __restrict__ guard_count) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     if (guard_count && k == 0) *guard_count = 0u;
@@ -679,19 +687,19 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
     for (int i = 0; i < DIM; ++i) Y[(size_t)k * DIM + i] = y[i];
     zbuf[k] = z;
     u2buf[k] = u2;
-    if (nsplit > 0) bh_glm_split(y, yscale, nsplit, ysplit, k, K);
+    if (nsplit > 0) bh_glm_split(y, yscale, nsplit, f16, ysplit, k, K);
 }
 
-// bf16 split of n given points (for the initial log-densities).
+// 16-bit split of n given points (for the initial log-densities).
 extern "C" __global__ void __launch_bounds__(WGS)
-bh_glm_split_points(const float* __restrict__ P, unsigned n, __bf16* __restrict__ ysplit,
-                    float yscale, int nsplit) {
+bh_glm_split_points(const float* __restrict__ P, unsigned n, void* __restrict__ ysplit,
+                    float yscale, int nsplit, int f16) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     if (k >= n) return;
     float y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = P[(size_t)k * DIM + i];
-    bh_glm_split(y, yscale, nsplit, ysplit, k, n);
+    bh_glm_split(y, yscale, nsplit, f16, ysplit, k, n);
 }
 
 // Log posterior of point y from the sum s over data rows of the likelihood
@@ -785,7 +793,8 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
               unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
               const double* __restrict__ lp, float beta, unsigned K, double guard,
               unsigned* __restrict__ flag, unsigned* __restrict__ list,
-              unsigned* __restrict__ count, unsigned lmax) {
+              unsigned* __restrict__ count, unsigned lmax, const float* __restrict__ Xpos,
+              double* __restrict__ pts64) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float y[DIM];
 #pragma unroll
@@ -800,6 +809,13 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
             if (i < lmax) {
                 list[i] = k;
                 f = 3;
+                // the two points in double for the fp64 likelihood kernel:
+                // row i the proposal, row lmax + i the current position
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) {
+                    pts64[(size_t)i * DIM + j] = (double)y[j];
+                    pts64[((size_t)lmax + i) * DIM + j] = (double)Xpos[(size_t)k * DIM + j];
+                }
             }
         }
     }
@@ -808,8 +824,8 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
 
 // Decides the listed walkers on their fp64 log posteriors, one wave per
 // walker: partial64 holds, per row range r (row stride `stride`), column
-// i < c the softplus sum of Y[list[i]] and column c + i that of X[list[i]]
-// (natural units).
+// i < c the softplus sum of Y[list[i]] and column lmax + i that of
+// X[list[i]] (natural units).
 // totals[0] += listed walkers, totals[1] += walkers past lmax.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
@@ -828,7 +844,7 @@ bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
     const unsigned waves = WGS / 64;
     for (unsigned i = blockIdx.x * waves + threadIdx.x / 64; i < c; i += gridDim.x * waves) {
         const double sy = bh_wave_range_sum(partial64, nranges64, stride, i);
-        const double sx = bh_wave_range_sum(partial64, nranges64, stride, c + i);
+        const double sx = bh_wave_range_sum(partial64, nranges64, stride, lmax + i);
         if ((threadIdx.x & 63u) == 0) {
             const unsigned k = list[i];
             float y[DIM], xk[DIM];

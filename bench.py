@@ -6,14 +6,17 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
   Bernoulli(sigmoid(X beta*)); prior beta ~ N(0, 5^2).  16384 walkers (8192
   per half), whole-walker partners, a = 2, 500 burn-in + 2000 run-sampler
   steps; walkers start uniform in a +-4 sd box around the posterior mode
-  (Newton steps on the host, not timed).  Three runs from the same seeds:
-    bf16x3  likelihood contraction on the matrix cores (split-bf16x3 terms,
-            fp32 accumulate, fp64 row sums) with the guarded accept test,
+  (Newton steps on the host, not timed).  Runs from the same seeds:
+    MFMA    likelihood contraction on the matrix cores (--c3-mfma: fp16x2 =
+            fp16 copy of X with proposals split in two fp16 terms, or
+            bf16x3 = bf16 X with three bf16 terms; fp32 accumulate, fp64 row
+            sums) with the guarded accept test,
     fp32    the plain fp32 VALU likelihood kernel,
-    fp64    the fp64 likelihood kernel (the oracle path on the GPU).
+    fp64    the fp64 likelihood kernel (coupled windows; the whole job with
+            --c3-fp64-full).
   ESS_d = Var_d(final ensemble) / sigma_d^2 with sigma_d from the engine's
   acor of the run-sampler means; value = min_d ESS_d per second of the
-  bf16x3 job (burn-in + run, inputs resident in HBM).  Posterior-mean error
+  MFMA job (burn-in + run, inputs resident in HBM).  Posterior-mean error
   = |mean_d - mean_fp64,d| / sigma_fp64,d on these coupled runs.
   Roofline: the MFMA likelihood kernel, 2 N d K = 1.05e12 algorithmic flops
   per launch, its launches timed with HIP events on the sampler's stream
@@ -66,11 +69,16 @@ ROOF_STEPS = 20
 C3_N, C3_D, C3_W = 1_000_000, 64, 16384
 C3_BURN, C3_RUN = 500, 2000
 C3_PRIOR_SD = 5.0
-# guard of the bf16x3 accept test, in log units: walkers whose accept
-# margin is smaller are decided on fp64 log posteriors.  The measured error
-# of lp(Y) - lp(X) at this size has sd 1.0e-3 and max 3.4e-3 (fp32 and
+# guard of the reduced-precision accept test, in log units: walkers whose
+# accept margin is smaller are decided on fp64 log posteriors.  The measured
+# error of lp(Y) - lp(X) at this size has sd 1.0e-3 and max 3.4e-3 (fp32 and
 # bf16x3 alike); 1e-2 is ten standard deviations.
 C3_GUARD = 1e-2
+# matrix-core precisions of the measured leg: (BH_GLM_*, products per
+# matrix-core block, kernel)
+C3_MFMA = {"bf16x3": (3, 3, "glm_logistic_mfma_kernel<64,3,bf16>"),
+           "fp16x2": (4, 2, "glm_logistic_mfma_kernel<64,2,fp16>")}
+C3_MFMA_DEFAULT = "fp16x2"
 # steps of each coupled window against the fp64 sampler
 C3_WINDOW = 60
 
@@ -748,11 +756,16 @@ def main_c3(args):
     fac = engine.HipBayaderaFactory(0, 256)
     sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=C3_PRIOR_SD))
     burn, run = args.c3_burn, args.c3_run
+    mname = args.c3_mfma
+    mprec, msplit, mkern = C3_MFMA[mname]
     # warm-up: every kernel of the guarded and plain paths once
-    c3_run(sf, _lib.BH_GLM_BF16X3, limits, 2, 64, guard=args.c3_guard)
+    c3_run(sf, mprec, limits, 2, 64, guard=args.c3_guard)
     c3_run(sf, _lib.BH_GLM_FP32, limits, 1, 50)
     torch.cuda.synchronize()
-    legs = [("bf16x3", _lib.BH_GLM_BF16X3, args.c3_guard), ("fp32", _lib.BH_GLM_FP32, 0.0)]
+    legs = [(mname, mprec, args.c3_guard), ("fp32", _lib.BH_GLM_FP32, 0.0)]
+    if args.c3_other_mfma:
+        other = "bf16x3" if mname == "fp16x2" else "fp16x2"
+        legs.append((other, C3_MFMA[other][0], args.c3_guard))
     if args.c3_bf16:
         legs.append(("bf16", _lib.BH_GLM_BF16, 0.0))
     res = {}
@@ -765,24 +778,23 @@ def main_c3(args):
         ref = c3_run(sf, _lib.BH_GLM_FP64, limits, burn, run)
     windows = []
     if not args.no_fp64:
-        # coupled windows against the fp64 sampler from the bf16x3 job's own
+        # coupled windows against the fp64 sampler from the MFMA job's own
         # states: after burn-in (accounted steps) and at the end (bare steps)
         print("bench: fp64 windows", file=sys.stderr, flush=True)
-        r = res["bf16x3"]
+        r = res[mname]
         for where, st, accounted in (("after burn-in", r["after_burn"], True),
                                      ("after run", r["end"], False)):
             x64, m64, _ = c3_window(sf, st, _lib.BH_GLM_FP64, 0.0, args.c3_window, accounted)
-            xg, mg, gs = c3_window(sf, st, _lib.BH_GLM_BF16X3, args.c3_guard, args.c3_window,
-                                   accounted)
+            xg, mg, gs = c3_window(sf, st, mprec, args.c3_guard, args.c3_window, accounted)
             xf, mf, _ = c3_window(sf, st, _lib.BH_GLM_FP32, 0.0, args.c3_window, accounted)
             w = {"from": where, "steps": args.c3_window,
                  "kind": "run-sampler!" if accounted else "burn-in!",
-                 "bf16x3_guarded_positions_equal_fp64": bool(np.array_equal(xg, x64)),
-                 "bf16x3_guarded_walkers_differing": int(np.any(xg != x64, axis=1).sum()),
+                 f"{mname}_guarded_positions_equal_fp64": bool(np.array_equal(xg, x64)),
+                 f"{mname}_guarded_walkers_differing": int(np.any(xg != x64, axis=1).sum()),
                  "fp32_plain_walkers_differing": int(np.any(xf != x64, axis=1).sum()),
                  "guard_listed": gs["listed"], "guard_overflow": gs["overflow"]}
             if accounted:
-                w["bf16x3_guarded_means_equal_fp64"] = bool(np.array_equal(mg, m64))
+                w[f"{mname}_guarded_means_equal_fp64"] = bool(np.array_equal(mg, m64))
             windows.append(w)
     out = {}
     for name, prec, g in legs:
@@ -820,38 +832,38 @@ def main_c3(args):
                        "ms_per_step_run": round(ref["run_gpu_ms"] / run, 3),
                        "acceptance_rate": round(ref["acc"], 4),
                        "likelihood_kernel_ms": round(ref["kernel"]["avg_ms"], 4)}
-    kt = res["bf16x3"]["kernel"]
+    kt = res[mname]["kernel"]
     flops = 2.0 * C3_N * C3_D * (C3_W // 2)
     achieved = flops / (kt["avg_ms"] * 1e-3) / 1e12
     line = {
-        "metric": METRIC, "value": out["bf16x3"]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
+        "metric": METRIC, "value": out[mname]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
         "steps": burn + run, "warmup": 66,
-        "ms_per_step": round(res["bf16x3"]["wall_s"] / (burn + run) * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "ms_per_step": round(res[mname]["wall_s"] / (burn + run) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": mname,
         "data": "synthetic (seeded logistic-regression data, X stored bf16)",
         "config": {"workload": f"logistic regression {C3_N}x{C3_D}, {C3_W} walkers, "
                                f"{burn} burn-in + {run} run-sampler steps (a=2), prior N(0,5^2)",
-                   "likelihood": "MFMA split-bf16x3, fp32 accumulate, fp64 row sums, "
+                   "likelihood": f"MFMA split-{mname}, fp32 accumulate, fp64 row sums, "
                                  f"guarded accept test ({args.c3_guard})",
                    "partner": "walker"},
-        "mfma_vs_fp32_ess_per_s": round(out["bf16x3"]["ess_per_s"] / out["fp32"]["ess_per_s"], 2),
-        "mfma_vs_fp32_walker_steps_per_s": round(out["bf16x3"]["walker_steps_per_s"]
+        "mfma_vs_fp32_ess_per_s": round(out[mname]["ess_per_s"] / out["fp32"]["ess_per_s"], 2),
+        "mfma_vs_fp32_walker_steps_per_s": round(out[mname]["walker_steps_per_s"]
                                                  / out["fp32"]["walker_steps_per_s"], 2),
         "precisions": out,
         "fp64_coupled_windows": windows,
-        "roofline": {"kernel": "glm_logistic_mfma_kernel<64,3> (1M rows x 8192 points)",
+        "roofline": {"kernel": f"{mkern} (1M rows x 8192 points)",
                      "bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK,
                      "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK, 4),
-                     "traffic": traffic_of("glm_logistic_mfma_1Mx64_traffic.json"),
+                     "traffic": traffic_of(f"glm_logistic_mfma_{mname}_1Mx64_traffic.json"),
                      "avg_kernel_ms": round(kt["avg_ms"], 4), "launches_timed": kt["launches"],
-                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": 3},
+                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": msplit},
         "setup_s": round(t_setup, 1),
     }
     if not args.no_roofline:
         line["plain_move_roofline"] = [move_roofline(fac, 4), move_roofline(fac, 16)]
     if not args.no_cpu_baseline:
         cb = c3_cpu_baseline(Xd, y, limits)
-        ess_per_ws = out["bf16x3"]["ess_min"] / (C3_W * (burn + run))
+        ess_per_ws = out[mname]["ess_min"] / (C3_W * (burn + run))
         line["cpu_baseline"] = {
             "value": round(cb["walker_steps_per_s"] * ess_per_ws, 4), "unit": "ESS/s",
             "cores": cb["threads"], "kind": "port",
@@ -878,6 +890,10 @@ def main():
     ap.add_argument("--c3-burn", type=int, default=C3_BURN)
     ap.add_argument("--c3-run", type=int, default=C3_RUN)
     ap.add_argument("--c3-guard", type=float, default=C3_GUARD)
+    ap.add_argument("--c3-mfma", choices=sorted(C3_MFMA), default=C3_MFMA_DEFAULT,
+                    help="matrix-core precision of the measured leg")
+    ap.add_argument("--c3-other-mfma", action="store_true",
+                    help="also run the other matrix-core precision (guarded)")
     ap.add_argument("--c3-bf16", action="store_true", help="also the single-bf16 likelihood")
     ap.add_argument("--no-fp64", action="store_true", help="skip the coupled fp64 windows")
     ap.add_argument("--c3-fp64-full", action="store_true",
@@ -889,6 +905,8 @@ def main():
                     help="only the move-bare! roofline runs (DIM 4 and 16), plus a 256 MiB "
                          "device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
+    if 0 < args.c2_user_steps < 50:
+        ap.error("--c2-user-steps: run-sampler! needs at least 50 steps (0 skips the run)")
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(sharding across GPUs is designed in DESIGN.md, not built)", file=sys.stderr)

@@ -56,6 +56,9 @@ extern "C" {
 #define BH_GLM_FP32 1       /* plain fp32 VALU contraction and softplus */
 #define BH_GLM_BF16 2       /* MFMA, bf16 x bf16 -> fp32, one product */
 #define BH_GLM_BF16X3 3     /* MFMA, proposals split in three bf16 terms (fp32-accurate) */
+#define BH_GLM_FP16X2 4     /* MFMA on an fp16 copy of X, proposals split in two fp16 terms
+                               (fp32-accurate; refused when |x| > 65504 somewhere; values
+                               below 2^-17 in magnitude keep an absolute error <= 2^-25) */
 
 typedef struct bh_ctx bh_ctx;
 typedef struct bh_dist_engine bh_dist_engine;

@@ -39,10 +39,12 @@ def test_glm_log_density_precisions(fac, N, d):
     assert np.max(np.abs(f64 - ref) / scale) < 1e-12
     f32 = sf.log_density(beta, _lib.BH_GLM_FP32)
     x3 = sf.log_density(beta, _lib.BH_GLM_BF16X3)
+    h2 = sf.log_density(beta, _lib.BH_GLM_FP16X2)
     b1 = sf.log_density(beta, _lib.BH_GLM_BF16)
     # fp32-accurate contractions: error of a few 1e-7 of |ll| per point
     assert np.max(np.abs(f32 - ref) / scale) < 2e-6
     assert np.max(np.abs(x3 - ref) / scale) < 2e-6
+    assert np.max(np.abs(h2 - ref) / scale) < 2e-6
     # one bf16 product: the proposals lose 16 mantissa bits
     assert np.max(np.abs(b1 - ref) / scale) < 2e-2
 
@@ -68,7 +70,8 @@ def test_glm_sampler_fp64_matches_oracle_trajectory(fac):
     assert same > 0.99, same
 
 
-@pytest.mark.parametrize("prec", [_lib.BH_GLM_FP32, _lib.BH_GLM_BF16X3, _lib.BH_GLM_BF16])
+@pytest.mark.parametrize("prec", [_lib.BH_GLM_FP32, _lib.BH_GLM_BF16X3, _lib.BH_GLM_FP16X2,
+                                  _lib.BH_GLM_BF16])
 def test_glm_sampler_runs_and_accounts(fac, prec):
     N, d, W = 4096, 64, 2048
     Xb, Xd, y = _problem(N, d, 13)
@@ -110,7 +113,7 @@ def _coupled_run(sf, prec, guard, d, W=2048, burn=25, run=60):
     return xs, res, stats
 
 
-@pytest.mark.parametrize("prec", [_lib.BH_GLM_BF16X3, _lib.BH_GLM_FP32])
+@pytest.mark.parametrize("prec", [_lib.BH_GLM_BF16X3, _lib.BH_GLM_FP16X2, _lib.BH_GLM_FP32])
 def test_glm_guard_makes_the_fp64_decisions(fac, prec):
     """Same seeds, same initial walkers: with the guarded accept test the
     reduced-precision sampler's trajectory is the fp64 sampler's, position
@@ -141,6 +144,24 @@ def test_glm_guard_lists_near_threshold_walkers(fac):
     xg, rg, sg = _coupled_run(sf, _lib.BH_GLM_BF16X3, 0.5, d, W=1024)
     assert sg["listed"] > 0 and sg["overflow"] == 0
     assert np.array_equal(xg, x64)
+
+
+def test_glm_fp16_refuses_values_outside_its_range(fac):
+    """X with a value beyond the fp16 range (65504) cannot take the fp16
+    matrix-core path; every other precision still runs."""
+    Xb, Xd, y = _problem(256, 32, 3)
+    X = models.from_bf16(Xb).reshape(256, 32).copy()
+    X[100, 7] = 70000.0
+    sf = fac.mcmc_factory(models.glm_logistic_model(models.to_bf16(X), y))
+    beta = np.zeros((4, 32), np.float32)
+    with pytest.raises(_lib.ArgumentError, match="fp16 range"):
+        sf.log_density(beta, _lib.BH_GLM_FP16X2)
+    s = sf.create_sampler(1, 1024, np.zeros(1, np.float32))
+    with pytest.raises(_lib.ArgumentError, match="fp16 range"):
+        s.set_glm_precision(_lib.BH_GLM_FP16X2)
+    s.set_glm_precision(_lib.BH_GLM_BF16X3)
+    assert np.all(np.isfinite(sf.log_density(beta, _lib.BH_GLM_BF16X3)))
+    s.release()
 
 
 def test_glm_guard_rejects_bad_values(fac):
