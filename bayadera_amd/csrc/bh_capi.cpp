@@ -888,6 +888,36 @@ int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs) {
     return BH_OK;
 }
 
+int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s) {
+    BH_ARG(ctx, "ctx is NULL");
+    BH_HIP(hipSetDevice(ctx->device));
+    const unsigned blocks = 8u * (unsigned)ctx->cu;
+    const int iters = 2048;
+    DevBuf out;
+    BH_TRY(out.alloc((size_t)blocks * 256 * sizeof(float)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BH_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    double rate[2] = {0.0, 0.0};
+    for (int which = 0; which < 2 && err == hipSuccess; ++which) {
+        err = bh::launch_glm_rate(ctx->stream, which, out.as<float>(), blocks, iters);  // warm-up
+        if (err == hipSuccess) err = hipEventRecord(e0, ctx->stream);
+        if (err == hipSuccess) err = bh::launch_glm_rate(ctx->stream, which, out.as<float>(), blocks, iters);
+        if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err == hipSuccess && ms > 0.0f)
+            rate[which] = (double)blocks * 256.0 * 8.0 * iters / ((double)ms * 1e-3);
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    BH_HIP(err);
+    if (exp2_per_s) *exp2_per_s = rate[0];
+    if (fma_per_s) *fma_per_s = rate[1];
+    return BH_OK;
+}
+
 int bh_uniform_sample(bh_ctx* ctx, uint32_t seed, float lower, float upper, float* out,
                       uint64_t n) {
     BH_ARG(ctx && out, "NULL argument");
@@ -1892,7 +1922,7 @@ int gb_steps(bh_glm_batch* b, uint32_t seed0, uint64_t n, float a, int accu,
 
 extern "C" {
 
-int bh_fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n) {
+static int fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n, int f16) {
     BH_ARG(ctx && in && out, "NULL argument");
     BH_ARG(n > 0 && n % 4 == 0, "n must be a positive multiple of 4");
     BH_HIP(hipSetDevice(ctx->device));
@@ -1900,9 +1930,17 @@ int bh_fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n) {
     BH_TRY(din.init(in, n, ctx->stream));
     DevOut dout;
     BH_TRY(dout.init(out, n * sizeof(float)));
-    BH_HIP(bh::launch_gb_fp8_decode(ctx->stream, din.dev, dout.as<float>(), n));
+    BH_HIP(bh::launch_gb_fp8_decode(ctx->stream, din.dev, dout.as<float>(), n, f16));
     BH_TRY(dout.finish(ctx->stream));
     return sync(ctx->stream);
+}
+
+int bh_fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n) {
+    return fp8_decode(ctx, in, out, n, 0);
+}
+
+int bh_fp8_decode_f16(bh_ctx* ctx, const void* in, float* out, uint64_t n) {
+    return fp8_decode(ctx, in, out, n, 1);
 }
 
 int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t* rows,

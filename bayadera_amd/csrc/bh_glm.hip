@@ -536,6 +536,37 @@ hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsi
     return hipErrorInvalidValue;
 }
 
+// Issue-rate kernels: eight independent chains per thread of v_exp_f32
+// (the softplus epilogue's transcendental) or of v_fma_f32.  Each thread
+// writes one float.
+__global__ void __launch_bounds__(256) glm_rate_exp_kernel(float* __restrict__ out, int iters) {
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = 0.001f * (float)(threadIdx.x + i);
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = __builtin_amdgcn_exp2f(-__builtin_fabsf(x[i]));
+    out[blockIdx.x * 256 + threadIdx.x] = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+}
+
+__global__ void __launch_bounds__(256) glm_rate_fma_kernel(float* __restrict__ out, int iters) {
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = 0.001f * (float)(threadIdx.x + i);
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = __builtin_fmaf(x[i], 0.999f, 1e-4f);
+    out[blockIdx.x * 256 + threadIdx.x] = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+}
+
+hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks, int iters) {
+    if (which == 0)
+        glm_rate_exp_kernel<<<blocks, 256, 0, s>>>(out, iters);
+    else
+        glm_rate_fma_kernel<<<blocks, 256, 0, s>>>(out, iters);
+    return hipGetLastError();
+}
+
 hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long count, void* out,
                                unsigned* overflow) {
     if (count % 8) return hipErrorInvalidValue;

@@ -46,17 +46,23 @@ __device__ __forceinline__ bf16x8 fp8x8_to_bf16x8(uint2 v) {
     return r;
 }
 
-// Eight fp8 bytes -> eight fp16 (v_cvt_pkrtz_f16_f32 of the exact floats:
-// every e4m3 value is an fp16 value, so the rounding mode never acts).
+// The fp16 matrix-core path reads fp8 values times 2^-8 (and the proposals
+// times 2^8): every e4m3 value times 2^-8 is still an fp16 value (its lowest
+// bit, 2^-17, is above fp16's 2^-24), and the proposals' column-scaled
+// coefficients, scaled up, keep their low split term out of fp16's
+// subnormal range.
+constexpr float kGbF16Down = 0x1p-8f;
+constexpr float kGbF16Up = 0x1p8f;
+
+// Eight fp8 bytes -> eight fp16 values times 2^-8 (v_cvt_scalef32_pk_f16_fp8,
+// two bytes per instruction: word 0 then word 1 of each dword).
 __device__ __forceinline__ f16x8 fp8x8_to_f16x8(uint2 v) {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    float f[8];
-    fp8x4_to_f32(v.x, *reinterpret_cast<float(*)[4]>(&f[0]));
-    fp8x4_to_f32(v.y, *reinterpret_cast<float(*)[4]>(&f[4]));
     u32x4 w;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        w[i] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(f[2 * i], f[2 * i + 1]));
+    w[0] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_scalef32_pk_f16_fp8(v.x, kGbF16Down, false));
+    w[1] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_scalef32_pk_f16_fp8(v.x, kGbF16Down, true));
+    w[2] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_scalef32_pk_f16_fp8(v.y, kGbF16Down, false));
+    w[3] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_scalef32_pk_f16_fp8(v.y, kGbF16Down, true));
     return __builtin_bit_cast(f16x8, w);
 }
 
@@ -88,13 +94,24 @@ struct Fp8Rows {
     }
 };
 
+// The device's decoding of fp8 bytes, four per thread: f16 = 0 the fp32
+// path's (v_cvt_pk_f32_fp8), f16 = 1 the fp16 matrix-core path's (scaled
+// fp16 values, returned times 2^8 as floats).
 __global__ void __launch_bounds__(256)
 gb_fp8_decode_kernel(const unsigned char* __restrict__ in, float* __restrict__ out,
-                     unsigned long long n) {
+                     unsigned long long n, int f16) {
     const unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
     if (4 * i + 3 < n) {
+        const unsigned v = *reinterpret_cast<const unsigned*>(in + 4 * i);
         float f[4];
-        fp8x4_to_f32(*reinterpret_cast<const unsigned*>(in + 4 * i), f);
+        if (f16) {
+            const uint2 vv{v, 0u};
+            const f16x8 h = fp8x8_to_f16x8(vv);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) f[m] = (float)h[m] * kGbF16Up;
+        } else {
+            fp8x4_to_f32(v, f);
+        }
 #pragma unroll
         for (int m = 0; m < 4; ++m) out[4 * i + m] = f[m];
     }
@@ -125,8 +142,8 @@ gb_init_kernel(unsigned seed, const unsigned* __restrict__ dims,
 }
 
 // Writes y * t (t = log2(e) * column scale) as fp32 and as nsplit 16-bit
-// terms (bf16, or fp16 when f16 is set; hi, mid, lo), for point k of
-// problem p.
+// terms (bf16; or fp16 of y * t * 2^8 when f16 is set; hi, mid, lo), for
+// point k of problem p.
 __device__ __forceinline__ void gb_store_point(const float (&y)[kGbD], const float* __restrict__ t,
                                                unsigned d, unsigned P, unsigned p, unsigned n,
                                                unsigned k, float* __restrict__ ysc,
@@ -136,7 +153,7 @@ __device__ __forceinline__ void gb_store_point(const float (&y)[kGbD], const flo
     for (int i = 0; i < kGbD; ++i) {
         const float v = (unsigned)i < d ? y[i] * t[i] : 0.0f;
         o[i] = v;
-        float r = v;
+        float r = f16 ? v * kGbF16Up : v;
         for (int sp = 0; sp < nsplit; ++sp) {
             const size_t at = (((size_t)sp * P + p) * n + k) * kGbD + i;
             if (f16) {
@@ -471,10 +488,11 @@ gb_step_kernel(const unsigned* __restrict__ dims, const float* __restrict__ msum
 }
 
 // --------------------------------------------------------------------------
-hipError_t launch_gb_fp8_decode(hipStream_t s, const void* in, float* out, unsigned long long n) {
+hipError_t launch_gb_fp8_decode(hipStream_t s, const void* in, float* out, unsigned long long n,
+                                int f16) {
     const unsigned long long quads = n / 4;
     gb_fp8_decode_kernel<<<(unsigned)((quads + 255) / 256), 256, 0, s>>>(
-        static_cast<const unsigned char*>(in), out, n);
+        static_cast<const unsigned char*>(in), out, n, f16);
     return hipGetLastError();
 }
 

@@ -11,7 +11,9 @@ namespace bh {
 // rows of one unit of the batched contraction (a multiple of 16)
 constexpr unsigned kGbUnitRows = 2048;
 
-hipError_t launch_gb_fp8_decode(hipStream_t s, const void* in, float* out, unsigned long long n);
+// f16 = 0: the fp32 path's decoding; 1: the fp16 matrix-core path's.
+hipError_t launch_gb_fp8_decode(hipStream_t s, const void* in, float* out, unsigned long long n,
+                                int f16);
 hipError_t launch_gb_init(hipStream_t s, unsigned seed, const unsigned* dims, const float* limits,
                           float* xs, unsigned P, unsigned W);
 hipError_t launch_gb_points(hipStream_t s, const float* xs, const unsigned* dims,

@@ -73,14 +73,15 @@ __device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double 
     }
 }
 
-// The epilogue of two consecutive 16-row blocks, eight values per lane and
-// column block, with one v_log_f32 for the eight: the product of the eight
-// factors 1 + 2^-|u| lies in [1, 256] and is built as a chain
-// p <- fma(e, p, p) = p (1 + e), one rounding per factor.  It is cut into
-// pieces of a few VALU instructions so that they can be placed between
-// matrix-core products: five per column block for the first block of the
-// pair (exponentials, product and |u| sum carried in st), seven for the
-// second (the same, then the logarithm and the double accumulation).
+// The epilogue of a run of two or four consecutive 16-row blocks, eight or
+// sixteen values per lane and column block, with one v_log_f32 for the run:
+// the product of its factors 1 + 2^-|u| (each in (1, 2], so at most 2^16)
+// is built as a chain p <- fma(e, p, p) = p (1 + e), one rounding per
+// factor.  It is cut into pieces of a few VALU instructions so that they can
+// be placed between matrix-core products: five per column block for a block
+// that starts the run (H = 0) or continues it (H = 2) (exponentials, product
+// and |u| sum carried in st), seven for the block that ends it (H = 1: the
+// same, then the logarithm and the double accumulation).
 struct GlmEpiState {
     float p[4];  // product of the factors so far, per column block
     float a[4];  // sum of the |u| so far
@@ -116,7 +117,7 @@ __device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& 
         if (H == 0) st.a[wb] = e[1];
         else st.a[wb] = st.a[wb] + e[1];
         break;
-    case 5:
+    case 5:  // H = 1 only
         e[3] = __builtin_fmaf(0.5f, st.a[wb], __builtin_amdgcn_logf(st.p[wb]));
         break;
     default:
@@ -126,7 +127,7 @@ __device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& 
 }
 
 // The products of one 16-row block into cn (zeroed by the first product of
-// each chain) with half H of the pair epilogue of the previous block's c
+// each chain) with part H of the run epilogue of the previous block's c
 // spread between them: products of the four column blocks alternate, so
 // consecutive products are independent, and scheduling barriers keep each
 // epilogue piece between two products, where it runs on the VALU while the
@@ -139,7 +140,7 @@ __device__ __forceinline__ void glm_mfma_overlap(const V (&a)[D / 32],
                                                  double (&acc)[4], GlmEpiState& st) {
     constexpr int KC = D / 32;
     constexpr int NM = 4 * KC * NS;
-    constexpr int PW = H == 0 ? 5 : 7;  // pieces per column block
+    constexpr int PW = H == 1 ? 7 : 5;  // pieces per column block
     constexpr int NP = 4 * PW;
     float e[4][4];
     int m = 0;
@@ -187,11 +188,13 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
     const unsigned rfull = r0 + (r1 - r0) / 16 * 16;
     f32x4 c0[4], c1[4];
     if (rfull > r0) {
-        // software pipeline over 16-row blocks, unrolled by two so the
-        // accumulator sets c0/c1 alternate without copies: the products of
-        // block i+1 run with the epilogue of block i, whose logarithm is
-        // shared with block i-1 or i+1.  Row blocks past the last are
-        // clamped to it (loaded, never used).
+        // software pipeline over 16-row blocks, the accumulator sets c0/c1
+        // alternating without copies: the products of block i+1 run with the
+        // epilogue of block i.  Runs of four blocks share a logarithm, then
+        // runs of two for what is left.  Invariant at each loop head: c0
+        // holds the products of block rb - 16 (epilogue pending), a1 the
+        // rows of block rb.  Row blocks past the last are clamped to it
+        // (loaded, never used).
         const unsigned rlast = rfull - 16;
         auto clamp = [&](unsigned rb) { return rb <= rlast ? rb : rlast; };
         typename Loader::vec a0[KC], a1[KC];
@@ -200,6 +203,17 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
         glm_mfma_block<D, NS>(a0, bfrag, c0);
         GlmEpiState st;
         unsigned rb = r0 + 16;
+        // runs of four blocks rb - 16 .. rb + 32 (products of rb .. rb + 48)
+        for (; rb + 48 < rfull; rb += 64) {
+            ld.full(clamp(rb + 16), a0);
+            glm_mfma_overlap<D, NS, 0>(a1, bfrag, c1, c0, acc, st);
+            ld.full(clamp(rb + 32), a1);
+            glm_mfma_overlap<D, NS, 2>(a0, bfrag, c0, c1, acc, st);
+            ld.full(clamp(rb + 48), a0);
+            glm_mfma_overlap<D, NS, 2>(a1, bfrag, c1, c0, acc, st);
+            ld.full(clamp(rb + 64), a1);
+            glm_mfma_overlap<D, NS, 1>(a0, bfrag, c0, c1, acc, st);
+        }
         // pairs of blocks (rb - 16, rb): the first's epilogue runs with the
         // products of rb, the second's with those of rb + 16
         for (; rb + 16 < rfull; rb += 32) {

@@ -145,11 +145,13 @@ class HipBayaderaFactory(_Handle):
     def glm_batch(self, model, walkers: int) -> "HipGlmBatch":
         return HipGlmBatch(self, model, walkers)
 
-    def fp8_decode(self, codes) -> np.ndarray:
-        """The device's e4m3 decoding of a byte array (length a multiple of 4)."""
+    def fp8_decode(self, codes, f16: bool = False) -> np.ndarray:
+        """The device's e4m3 decoding of a byte array (length a multiple of
+        4): the fp32 path's, or with f16 the fp16 matrix-core path's."""
         c = np.ascontiguousarray(codes, np.uint8).reshape(-1)
         out = np.zeros(c.size, np.float32)
-        call("bh_fp8_decode", self.handle, c.ctypes.data, out.ctypes.data, c.size)
+        call("bh_fp8_decode_f16" if f16 else "bh_fp8_decode", self.handle, c.ctypes.data,
+             out.ctypes.data, c.size)
         return out
 
     def mcmc_factory(self, model) -> "HipStretchFactory":
@@ -159,6 +161,12 @@ class HipBayaderaFactory(_Handle):
 
     def processing_elements(self) -> int:
         return self.compute_units * self.wgs
+
+    def device_rates(self) -> dict:
+        """Measured lane-op rates of v_exp_f32 and v_fma_f32 (per second)."""
+        e, f = ctypes.c_double(), ctypes.c_double()
+        call("bh_device_rates", self.handle, ctypes.byref(e), ctypes.byref(f))
+        return {"exp2_per_s": e.value, "fma_per_s": f.value}
 
 
 class HipDirectSamplerEngine:

@@ -496,6 +496,7 @@ def main_c4(args):
         "walker_steps_per_s": round(walker_steps / wall, 1),
         "burn_in_gpu_ms": round(burn_ms, 2), "run_gpu_ms": round(run_ms, 2),
         "ess_sum": round(float(ess[ok].sum()), 1),
+        "ess_estimator": {"acor": int(acor_ok.sum()), "sokal": int((~acor_ok).sum())},
         "ess_per_problem_median": round(float(np.median(ess[ok])), 2),
         "problems_with_finite_ess": int(ok.sum()),
         "tau_max_median": round(float(np.median(tau_max[ok])), 1),
@@ -568,16 +569,18 @@ def main_c5(args):
     burn, run = args.c5_burn, args.c5_run
     dims = m.dims.astype(np.int64)
     series = torch.empty(run * int(dims.sum()), dtype=torch.float32, device="cuda")
-    bt, _, _, _, _ = c5_run(fac, m, lim, _lib.BH_GLM_BF16X3, 2, 50)  # warm-up
+    mname = args.c5_mfma
+    mprec, msplit, _ = C3_MFMA[mname]
+    bt, _, _, _, _ = c5_run(fac, m, lim, mprec, 2, 50)  # warm-up
     bt.release()
     torch.cuda.synchronize()
-    print("bench: bf16x3 batched run", file=sys.stderr, flush=True)
-    bt, res, wall, burn_ms, run_ms = c5_run(fac, m, lim, _lib.BH_GLM_BF16X3, burn, run, series)
+    print(f"bench: {mname} batched run", file=sys.stderr, flush=True)
+    bt, res, wall, burn_ms, run_ms = c5_run(fac, m, lim, mprec, burn, run, series)
     # after the timed region: ESS per problem from the engine's acor of its
-    # recorded means; the precision bias of the bf16x3 chain's stationary
-    # distribution, estimated per problem by reweighting its final walkers
-    # with exp(lp_fp64 - lp_bf16x3) (the fp64 likelihood kernel at the same
-    # positions), in MC-SE units
+    # recorded means; the precision bias of the matrix-core chain's
+    # stationary distribution, estimated per problem by reweighting its final
+    # walkers with exp(lp_fp64 - lp_mfma) (the fp64 likelihood kernel at the
+    # same positions), in MC-SE units
     acor = fac.acor_engine()
     xs, lp_b = bt.state()
     lp_64 = bt.log_density(_lib.BH_GLM_FP64)
@@ -585,11 +588,19 @@ def main_c5(args):
     bias = np.zeros(m.problems)
     sd_bias = np.zeros(m.problems)
     lp_err = np.zeros(m.problems)
+    acor_ok = np.zeros(m.problems, bool)
     for p, blk in enumerate(bt.series_blocks(series, run)):
         d = int(dims[p])
         ac = acor.acor(blk)
         var = res["sd"][p, :d] ** 2
-        e = var / np.square(ac.sigma.astype(np.float64))
+        sig2 = np.square(ac.sigma.astype(np.float64))
+        acor_ok[p] = bool(np.all(np.isfinite(sig2)))
+        if not acor_ok[p]:
+            # the reference's acor gives no value when tau is long against
+            # the series: Sokal's windowed estimate of the same MC-SE^2
+            b = np.asarray(blk, np.float64)
+            sig2 = sokal_tau(b) * b.var(0) / b.shape[0]
+        e = var / sig2
         ess[p] = float(np.min(e))
         mcse = np.sqrt(var / e)
         x = xs[p].astype(np.float64)
@@ -610,26 +621,28 @@ def main_c5(args):
         "metric": METRIC, "value": round(float(ess[ok].sum()) / wall, 1), "unit": "ESS/s",
         "n_gpus": 1, "steps": burn + run, "warmup": 52,
         "ms_per_step": round(wall / (burn + run) * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "scaling": "weak", "vs_baseline": None, "dtype": mname,
         "data": "synthetic (seeded logistic-regression problems, X stored fp8 e4m3 + "
                 "per-column fp32 scales)",
         "config": {"workload": f"{m.problems} logistic regressions, N in [1e3, 2e5] "
                                f"(log-uniform), d in {{8,16,32,64}}, {C5_W} walkers each, "
                                f"{burn} burn-in + {run} run steps (a=2), prior N(0,5^2)",
-                   "likelihood": "grouped MFMA split-bf16x3 over fp8-decoded rows, fp32 "
+                   "likelihood": f"grouped MFMA split-{mname} over fp8-decoded rows, fp32 "
                                  "accumulate, fp64 row sums", "partner": "walker"},
         "walker_steps_per_s": round(m.problems * C5_W * (burn + run) / wall, 1),
         "burn_in_gpu_ms": round(burn_ms, 1), "run_gpu_ms": round(run_ms, 1),
         "ess_sum": round(float(ess[ok].sum()), 1),
         "ess_per_problem_median": round(float(np.median(ess[ok])), 1),
         "acceptance_rate_median": round(float(np.median(res["acceptance-rate"])), 4),
-        "precision_bias_mean_over_mcse": {"median": round(float(np.median(bias)), 4),
-                                          "max": round(float(bias.max()), 4),
-                                          "problems_over_0.1": int((bias > 0.1).sum())},
+        "precision_bias_mean_over_mcse": {"median": round(float(np.nanmedian(bias)), 4),
+                                          "max": round(float(np.nanmax(bias)), 4),
+                                          "problems_over_0.1": int((bias > 0.1).sum()),
+                                          "problems_without_mcse": int(np.isnan(bias).sum())},
         "precision_bias_sd_rel": {"median": float(np.median(sd_bias)),
                                   "max": float(sd_bias.max())},
         "lp_error_spread_max": float(lp_err.max()),
-        "roofline": {"kernel": "half-step (propose + grouped gb_mfma_kernel<D,3> + accept), "
+        "roofline": {"kernel": f"half-step (propose + grouped gb_mfma_kernel<D,{msplit}> + "
+                               "accept), "
                                f"{m.problems} problems x 1024 points", "bound": "mfma",
                      "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_BF16_PEAK, 4), "traffic": None,
@@ -835,6 +848,11 @@ def main_c3(args):
     kt = res[mname]["kernel"]
     flops = 2.0 * C3_N * C3_D * (C3_W // 2)
     achieved = flops / (kt["avg_ms"] * 1e-3) / 1e12
+    # the co-bound: one v_exp_f32 per contraction value and one v_log_f32 per
+    # eight, against the device's measured transcendental issue rate
+    rates = fac.device_rates()
+    trans = C3_N * (C3_W // 2) * (1.0 + 1.0 / 8.0)
+    trans_rate = trans / (kt["avg_ms"] * 1e-3)
     line = {
         "metric": METRIC, "value": out[mname]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
         "steps": burn + run, "warmup": 66,
@@ -856,7 +874,14 @@ def main_c3(args):
                      "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK, 4),
                      "traffic": traffic_of(f"glm_logistic_mfma_{mname}_1Mx64_traffic.json"),
                      "avg_kernel_ms": round(kt["avg_ms"], 4), "launches_timed": kt["launches"],
-                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": msplit},
+                     "algorithmic_flops_per_launch": flops, "mfma_issue_factor": msplit,
+                     "epilogue": {"bound": "transcendental issue (v_exp_f32, v_log_f32)",
+                                  "achieved": round(trans_rate / 1e12, 3),
+                                  "peak_measured": round(rates["exp2_per_s"] / 1e12, 3),
+                                  "unit": "T lane-ops/s",
+                                  "frac": round(trans_rate / rates["exp2_per_s"], 4),
+                                  "transcendentals_per_launch": trans,
+                                  "fma_rate_measured": round(rates["fma_per_s"] / 1e12, 3)}},
         "setup_s": round(t_setup, 1),
     }
     if not args.no_roofline:
@@ -885,6 +910,8 @@ def main():
     ap.add_argument("--c5-burn", type=int, default=C5_BURN)
     ap.add_argument("--c5-run", type=int, default=C5_RUN)
     ap.add_argument("--c5-fp32", action="store_true", help="also the plain fp32 batched run")
+    ap.add_argument("--c5-mfma", choices=sorted(C3_MFMA), default=C3_MFMA_DEFAULT,
+                    help="matrix-core precision of the batched run")
     ap.add_argument("--c4-burn", type=int, default=C4_BURN)
     ap.add_argument("--c4-run", type=int, default=C4_RUN)
     ap.add_argument("--c3-burn", type=int, default=C3_BURN)

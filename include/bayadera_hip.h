@@ -88,6 +88,12 @@ int bh_ctx_create(int device, uint32_t wgs, bh_ctx** out);
 int bh_ctx_destroy(bh_ctx* ctx);
 /* compute units and WGS; processing-elements = cu * wgs (amd_gcn.clj:663-664) */
 int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs);
+/* Measured issue rates of the device (no reference counterpart): lane
+ * operations per second of v_exp_f32 (the transcendental of the GLM
+ * likelihood's softplus epilogue) and of v_fma_f32, each from one timed
+ * launch of independent chains on all compute units (about 2 ms).  The
+ * roofline of the epilogue, for which no published peak exists. */
+int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s);
 
 /* ---------------------------------------------------------------------- */
 /* Direct sampler: GCNDirectSamplerEngine.sample for the uniform model      */
@@ -347,6 +353,9 @@ int bh_glm_batch_log_density(bh_glm_batch* b, int precision, double* lp);
 int bh_glm_batch_last_gpu_ms(const bh_glm_batch* b, double* ms);
 /* The device's fp8 e4m3 decoding of n bytes (n a multiple of 4). */
 int bh_fp8_decode(bh_ctx* ctx, const void* in, float* out, uint64_t n);
+/* The fp16 matrix-core path's decoding of the same bytes (fp16 values of
+ * x * 2^-8 on the device, returned times 2^8 as floats). */
+int bh_fp8_decode_f16(bh_ctx* ctx, const void* in, float* out, uint64_t n);
 
 /* Sampler state for resume: positions (DIM x W), logfn (W), and the host
  * counters {bare seed, move seed, bare counter, move counter, iteration

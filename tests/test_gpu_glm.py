@@ -251,3 +251,10 @@ def test_glm_gaussian_sampler_moves_and_rejects_guard(fac):
     ref = _linreg_lp64(X, y, xs[:64])
     assert np.allclose(lp[:64], ref, rtol=2e-6)
     s.release()
+
+
+def test_device_rates_are_measured(fac):
+    """The issue rates that bound the softplus epilogue: both positive, and
+    the transcendental slower than the fma."""
+    r = fac.device_rates()
+    assert r["exp2_per_s"] > 1e11 and r["fma_per_s"] > r["exp2_per_s"]

@@ -36,9 +36,12 @@ def _problems(seed, shapes, pow2_scales=False):
     return m
 
 
-def test_device_fp8_decoding_matches_the_host_table(fac):
+@pytest.mark.parametrize("f16", [False, True])
+def test_device_fp8_decoding_matches_the_host_table(fac, f16):
+    """Both device decodings (fp32 path; fp16 matrix-core path, scaled by
+    2^-8 on the device) give every e4m3 code's exact value."""
     codes = np.arange(256, dtype=np.uint8)
-    dev = fac.fp8_decode(codes)
+    dev = fac.fp8_decode(codes, f16=f16)
     ref = models.FP8_E4M3_VALUES
     assert np.array_equal(np.isnan(dev), np.isnan(ref))
     ok = ~np.isnan(ref)
@@ -50,7 +53,7 @@ SHAPES = [(1000, 8), (2049, 16), (4100, 32), (20011, 64), (16, 64), (300, 3), (7
 
 
 @pytest.mark.parametrize("prec,tol", [(_lib.BH_GLM_FP64, 1e-12), (_lib.BH_GLM_FP32, 2e-6),
-                                      (_lib.BH_GLM_BF16X3, 2e-6)])
+                                      (_lib.BH_GLM_BF16X3, 2e-6), (_lib.BH_GLM_FP16X2, 2e-6)])
 def test_batch_initial_log_posteriors_vs_oracle(fac, prec, tol):
     # the oracle holds X in float32, where scale * fp8 is exact only for
     # power-of-two scales: those make the fp64 comparison exact to 1e-12
@@ -107,9 +110,10 @@ def test_batch_problem_equals_single_glm_sampler(fac, d):
     bt.release()
 
 
-def test_batch_run_accounting_and_series(fac):
+@pytest.mark.parametrize("prec", [_lib.BH_GLM_BF16X3, _lib.BH_GLM_FP16X2])
+def test_batch_run_accounting_and_series(fac, prec):
     m = _problems(6, SHAPES)
-    bt = fac.glm_batch(m, 512).set_precision(_lib.BH_GLM_BF16X3)
+    bt = fac.glm_batch(m, 512).set_precision(prec)
     lim = np.tile(np.array([-0.3, 0.3], np.float32), (m.problems, 64, 1))
     bt.init_position(7, lim)
     bt.burn_in(1, 30, 2.0)
@@ -131,3 +135,22 @@ def test_batch_walker_rule(fac):
     m = _problems(7, [(100, 8)])
     with pytest.raises(_lib.BayaderaError):
         fac.glm_batch(m, 768)
+
+
+def test_batch_matrix_core_precisions_agree_with_fp64_on_moves(fac):
+    """From the same state, a few bare steps in fp16x2 and in bf16x3 take
+    (nearly) the moves of the fp64 batch: the contractions are fp32-accurate,
+    so only walkers within ~1e-4 of an accept threshold can differ."""
+    m = _problems(8, [(3000, 32), (8000, 64), (600, 10)])
+    W = 1024
+    out = {}
+    for prec in (_lib.BH_GLM_FP64, _lib.BH_GLM_BF16X3, _lib.BH_GLM_FP16X2):
+        bt = fac.glm_batch(m, W).set_precision(prec)
+        bt.init_position(41, np.tile(np.array([-0.4, 0.4], np.float32), (m.problems, 64, 1)))
+        bt.burn_in(50, 4, 2.0)
+        out[prec] = bt.state()[0].copy()
+        bt.release()
+    for prec in (_lib.BH_GLM_BF16X3, _lib.BH_GLM_FP16X2):
+        for p in range(m.problems):
+            same = np.all(out[prec][p] == out[_lib.BH_GLM_FP64][p], axis=1).mean()
+            assert same > 0.99, (prec, p, same)
