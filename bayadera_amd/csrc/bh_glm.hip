@@ -321,14 +321,62 @@ glm_gauss_fp64_kernel(const float* __restrict__ X, const float* __restrict__ yv,
     partial[(size_t)range * n + k] = acc;
 }
 
+// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)) in double, to an
+// absolute error of a few 1e-16 (the guard sums it over up to 2^32 rows and
+// needs the total to ~1e-9, far above this; the library's exp and log1p aim
+// at last-bit accuracy and cost about three times as many instructions).
+//   t = exp(-a): a = n ln2 + r with n = rint(a / ln2), ln2 split in a
+//     32-bit head (n ln2_hi exact) and a tail, |r| <= ln2 / 2; exp(-r) by
+//     its Taylor series to degree 13 (remainder < 5e-18), scaled by 2^-n.
+//   log1p(t), t in [0, 1]: 1 + t = 2^k (1 + f) with k = 1 when
+//     t > sqrt(2) - 1 (f = (t - 1) / 2), else k = 0 (f = t), so
+//     f in [-0.293, 0.415]; log(1 + f) = 2 atanh(s), s = f / (2 + f),
+//     |s| <= 0.172, by its series to s^23 (remainder < 1e-18), plus k ln2.
+// Its constants, read from memory by the kernel that uses it (as immediates
+// the compiler keeps them all in scalar registers, which the list kernel's
+// coefficient loads need): ln2 head (32 significant bits) and tail, 1/ln2,
+// the Taylor coefficients 1/13! .. 1/0!, then 2/23 .. 2/1.
+__constant__ double kGlmSoftplus64[3 + 14 + 12] = {
+    0.6931471803691238, 1.9082149292705877e-10, 1.4426950408889634,
+    1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
+    1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0,
+    1.0 / 6.0, 0.5, 1.0, 1.0,
+    2.0 / 23.0, 2.0 / 21.0, 2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0,
+    2.0 / 11.0, 2.0 / 9.0, 2.0 / 7.0, 2.0 / 5.0, 2.0 / 3.0, 2.0};
+
+__device__ __forceinline__ double glm_softplus64(double eta, const double* __restrict__ k) {
+    const double a = __builtin_fabs(eta);
+    const double n = __builtin_rint(a * k[2]);
+    double r = __builtin_fma(-n, k[0], a);
+    r = __builtin_fma(-n, k[1], r);
+    // exp(-r) = sum_j (-r)^j / j!, Horner from degree 13
+    const double x = -r;
+    double q = k[3];
+#pragma unroll
+    for (int j = 4; j < 17; ++j) q = __builtin_fma(q, x, k[j]);
+    const bool small = a < 745.0;  // else exp(-a) underflows to 0
+    const double t = small ? __builtin_ldexp(q, -(int)(small ? n : 0.0)) : 0.0;
+    const bool up = t > 0.41421356237309503;  // sqrt(2) - 1
+    const double f = up ? (t - 1.0) * 0.5 : t;
+    const double sv = f / (2.0 + f);
+    const double s2 = sv * sv;
+    // 2 atanh(s) = s (2 + 2 s^2/3 + ... + 2 s^22/23)
+    double g = k[17];
+#pragma unroll
+    for (int j = 18; j < 29; ++j) g = __builtin_fma(g, s2, k[j]);
+    double l = sv * g;
+    if (up) l = k[0] + (k[1] + l);
+    return eta != eta ? eta : __builtin_fmax(eta, 0.0) + l;
+}
+
 // fp64 log-likelihood sums of the points listed by the guarded accept test.
 // pts holds them in double, row i < c a proposal and row lmax + i its
 // walker's position (c = min(*count, lmax), read on the device).  A block
 // takes one row range; each thread a row at a time, the row converted to
 // double eight features at a time, against eight points whose coefficients
-// are uniform across the wave (scalar loads).  The same arithmetic as the
-// fp64 kernel: fma chains over the features in order, then
-// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)).  partial64[r * 2 lmax + i]
+// are uniform across the wave (scalar loads).  fma chains over the features
+// in order as the fp64 kernel, then glm_softplus64 (within a few 1e-16 per
+// row of the fp64 kernel's library softplus).  partial64[r * 2 lmax + i]
 // gets the block's sum for point row i.
 template <int D>
 __global__ void __launch_bounds__(256)
@@ -374,9 +422,10 @@ glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
                         for (int e = 0; e < 8; ++e)
                             eta[q] = __builtin_fma(x[e], b[q * D + v * 8 + e], eta[q]);
                 }
+                // the softplus constants are read after the coefficients
+                asm volatile("" ::: "memory");
 #pragma unroll
-                for (int q = 0; q < PC; ++q)
-                    acc[q] += __builtin_fmax(eta[q], 0.0) + log1p(exp(-__builtin_fabs(eta[q])));
+                for (int q = 0; q < PC; ++q) acc[q] += glm_softplus64(eta[q], kGlmSoftplus64);
             }
 #pragma unroll
             for (int q = 0; q < PC; ++q) {
