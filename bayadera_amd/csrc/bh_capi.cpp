@@ -888,7 +888,8 @@ int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs) {
     return BH_OK;
 }
 
-int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s) {
+int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s,
+                    double* mfma_f16_flops_per_s) {
     BH_ARG(ctx, "ctx is NULL");
     BH_HIP(hipSetDevice(ctx->device));
     const unsigned blocks = 8u * (unsigned)ctx->cu;
@@ -898,8 +899,8 @@ int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BH_HIP(hipEventCreate(&e0));
     hipError_t err = hipEventCreate(&e1);
-    double rate[2] = {0.0, 0.0};
-    for (int which = 0; which < 2 && err == hipSuccess; ++which) {
+    double rate[3] = {0.0, 0.0, 0.0};
+    for (int which = 0; which < 3 && err == hipSuccess; ++which) {
         err = bh::launch_glm_rate(ctx->stream, which, out.as<float>(), blocks, iters);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(e0, ctx->stream);
         if (err == hipSuccess) err = bh::launch_glm_rate(ctx->stream, which, out.as<float>(), blocks, iters);
@@ -907,14 +908,18 @@ int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s) {
         if (err == hipSuccess) err = hipEventSynchronize(e1);
         float ms = 0.0f;
         if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-        if (err == hipSuccess && ms > 0.0f)
-            rate[which] = (double)blocks * 256.0 * 8.0 * iters / ((double)ms * 1e-3);
+        // lane operations; for the products, flops (8 per wave and
+        // iteration, 16 x 16 x 32 x 2 each)
+        const double ops = which < 2 ? (double)blocks * 256.0 * 8.0 * iters
+                                     : (double)blocks * 4.0 * 8.0 * iters * 16384.0;
+        if (err == hipSuccess && ms > 0.0f) rate[which] = ops / ((double)ms * 1e-3);
     }
     (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     BH_HIP(err);
     if (exp2_per_s) *exp2_per_s = rate[0];
     if (fma_per_s) *fma_per_s = rate[1];
+    if (mfma_f16_flops_per_s) *mfma_f16_flops_per_s = rate[2];
     return BH_OK;
 }
 

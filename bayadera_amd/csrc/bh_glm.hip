@@ -608,11 +608,34 @@ __global__ void __launch_bounds__(256) glm_rate_fma_kernel(float* __restrict__ o
     out[blockIdx.x * 256 + threadIdx.x] = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
 }
 
+// Eight independent accumulators per wave of v_mfma_f32_16x16x32_f16 (the
+// fp16 likelihood kernel's product).  Each thread writes one float.
+__global__ void __launch_bounds__(256) glm_rate_mfma_kernel(float* __restrict__ out, int iters) {
+    f16x8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a[i] = (_Float16)(0.001f * (float)(threadIdx.x + i));
+        b[i] = (_Float16)(0.002f * (float)i);
+    }
+    f32x4 c[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c[j] = mfma_16x16x32(a, b, c[j]);
+    float sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += c[j][0] + c[j][3];
+    out[blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
 hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks, int iters) {
     if (which == 0)
         glm_rate_exp_kernel<<<blocks, 256, 0, s>>>(out, iters);
-    else
+    else if (which == 1)
         glm_rate_fma_kernel<<<blocks, 256, 0, s>>>(out, iters);
+    else
+        glm_rate_mfma_kernel<<<blocks, 256, 0, s>>>(out, iters);
     return hipGetLastError();
 }
 

@@ -24,8 +24,9 @@ hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsi
                                unsigned d, const float* P, const void* ys, unsigned n,
                                unsigned nranges, unsigned long long rows_per_range, float yscale,
                                double* partial);
-// Issue-rate kernels (which = 0: v_exp_f32, 1: v_fma_f32), 8 x iters
-// operations per thread, out: blocks x 256 floats.
+// Issue-rate kernels (which = 0: v_exp_f32, 1: v_fma_f32, 8 x iters
+// operations per thread; 2: v_mfma_f32_16x16x32_f16, 8 x iters per wave),
+// out: blocks x 256 floats.
 hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks, int iters);
 // fp16 copy of `count` bf16 values (a multiple of 8); *overflow is
 // incremented by the number of values outside the fp16 range.

@@ -163,10 +163,11 @@ class HipBayaderaFactory(_Handle):
         return self.compute_units * self.wgs
 
     def device_rates(self) -> dict:
-        """Measured lane-op rates of v_exp_f32 and v_fma_f32 (per second)."""
-        e, f = ctypes.c_double(), ctypes.c_double()
-        call("bh_device_rates", self.handle, ctypes.byref(e), ctypes.byref(f))
-        return {"exp2_per_s": e.value, "fma_per_s": f.value}
+        """Measured rates: v_exp_f32 and v_fma_f32 lane-ops per second,
+        v_mfma_f32_16x16x32_f16 flops per second."""
+        e, f, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        call("bh_device_rates", self.handle, ctypes.byref(e), ctypes.byref(f), ctypes.byref(m))
+        return {"exp2_per_s": e.value, "fma_per_s": f.value, "mfma_f16_flops_per_s": m.value}
 
 
 class HipDirectSamplerEngine:

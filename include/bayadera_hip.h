@@ -90,10 +90,13 @@ int bh_ctx_destroy(bh_ctx* ctx);
 int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs);
 /* Measured issue rates of the device (no reference counterpart): lane
  * operations per second of v_exp_f32 (the transcendental of the GLM
- * likelihood's softplus epilogue) and of v_fma_f32, each from one timed
- * launch of independent chains on all compute units (about 2 ms).  The
- * roofline of the epilogue, for which no published peak exists. */
-int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s);
+ * likelihood's softplus epilogue) and of v_fma_f32, and flops per second of
+ * v_mfma_f32_16x16x32_f16 (the likelihood's product), each from one timed
+ * launch of independent chains on all compute units (under 2 ms each): the
+ * practical rates against which the likelihood kernel is read.  Any output
+ * may be NULL. */
+int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s,
+                    double* mfma_f16_flops_per_s);
 
 /* ---------------------------------------------------------------------- */
 /* Direct sampler: GCNDirectSamplerEngine.sample for the uniform model      */

@@ -258,3 +258,4 @@ def test_device_rates_are_measured(fac):
     the transcendental slower than the fma."""
     r = fac.device_rates()
     assert r["exp2_per_s"] > 1e11 and r["fma_per_s"] > r["exp2_per_s"]
+    assert 1e14 < r["mfma_f16_flops_per_s"] < 3e15
