@@ -889,13 +889,18 @@ int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs) {
 }
 
 int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s,
-                    double* mfma_f16_flops_per_s) {
+                    double* mfma_f16_flops_per_s, double* gathers_per_s) {
     BH_ARG(ctx, "ctx is NULL");
     BH_HIP(hipSetDevice(ctx->device));
     const unsigned blocks = 8u * (unsigned)ctx->cu;
     const int iters = 2048;
-    DevBuf out;
+    const unsigned n16 = 1u << 23;  // gather records of 16 bytes: 128 MiB
+    DevBuf out, src, gout;
     BH_TRY(out.alloc((size_t)blocks * 256 * sizeof(float)));
+    if (gathers_per_s) {
+        BH_TRY(src.alloc((size_t)n16 * 16));
+        BH_TRY(gout.alloc((size_t)n16 * sizeof(float)));
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BH_HIP(hipEventCreate(&e0));
     hipError_t err = hipEventCreate(&e1);
@@ -914,9 +919,26 @@ int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s,
                                      : (double)blocks * 4.0 * 8.0 * iters * 16384.0;
         if (err == hipSuccess && ms > 0.0f) rate[which] = ops / ((double)ms * 1e-3);
     }
+    // random 16-byte gathers from 128 MiB (above the L2s, within the
+    // Infinity Cache, as the stretch move's partner reads at 2^23 walkers)
+    double grate = 0.0;
+    if (err == hipSuccess && gathers_per_s) {
+        err = hipMemsetAsync(src.p, 0, (size_t)n16 * 16, ctx->stream);
+        for (int rep = 0; rep < 2 && err == hipSuccess; ++rep)
+            err = bh::launch_glm_gather(ctx->stream, src.p, gout.as<float>(), n16);
+        if (err == hipSuccess) err = hipEventRecord(e0, ctx->stream);
+        for (int rep = 0; rep < 5 && err == hipSuccess; ++rep)
+            err = bh::launch_glm_gather(ctx->stream, src.p, gout.as<float>(), n16);
+        if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err == hipSuccess && ms > 0.0f) grate = 5.0 * n16 / ((double)ms * 1e-3);
+    }
     (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     BH_HIP(err);
+    if (gathers_per_s) *gathers_per_s = grate;
     if (exp2_per_s) *exp2_per_s = rate[0];
     if (fma_per_s) *fma_per_s = rate[1];
     if (mfma_f16_flops_per_s) *mfma_f16_flops_per_s = rate[2];

@@ -629,6 +629,22 @@ __global__ void __launch_bounds__(256) glm_rate_mfma_kernel(float* __restrict__ 
     out[blockIdx.x * 256 + threadIdx.x] = sum;
 }
 
+// Random 16-byte gathers: thread k reads record (k * 2654435761) mod n16
+// of src (n16 a power of two; 16-byte records) and writes their sum.
+__global__ void __launch_bounds__(256) glm_rate_gather_kernel(const float4* __restrict__ src,
+                                                              float* __restrict__ out,
+                                                              unsigned n16) {
+    const unsigned k = blockIdx.x * 256 + threadIdx.x;
+    const float4 v = src[(k * 2654435761u) & (n16 - 1u)];
+    out[k] = (v.x + v.y) + (v.z + v.w);
+}
+
+hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16) {
+    if (n16 == 0 || (n16 & (n16 - 1)) || n16 % 256) return hipErrorInvalidValue;
+    glm_rate_gather_kernel<<<n16 / 256, 256, 0, s>>>(static_cast<const float4*>(src), out, n16);
+    return hipGetLastError();
+}
+
 hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks, int iters) {
     if (which == 0)
         glm_rate_exp_kernel<<<blocks, 256, 0, s>>>(out, iters);

@@ -28,6 +28,9 @@ hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsi
 // operations per thread; 2: v_mfma_f32_16x16x32_f16, 8 x iters per wave),
 // out: blocks x 256 floats.
 hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks, int iters);
+// n16 random 16-byte gathers from src (n16 x 16 bytes; n16 a power of two and
+// a multiple of 256), one per thread, sums to out (n16 floats).
+hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16);
 // fp16 copy of `count` bf16 values (a multiple of 8); *overflow is
 // incremented by the number of values outside the fp16 range.
 hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long count, void* out,

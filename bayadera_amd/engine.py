@@ -164,10 +164,13 @@ class HipBayaderaFactory(_Handle):
 
     def device_rates(self) -> dict:
         """Measured rates: v_exp_f32 and v_fma_f32 lane-ops per second,
-        v_mfma_f32_16x16x32_f16 flops per second."""
-        e, f, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        call("bh_device_rates", self.handle, ctypes.byref(e), ctypes.byref(f), ctypes.byref(m))
-        return {"exp2_per_s": e.value, "fma_per_s": f.value, "mfma_f16_flops_per_s": m.value}
+        v_mfma_f32_16x16x32_f16 flops per second, random 16-byte gathers
+        per second."""
+        e, f, m, g = (ctypes.c_double() for _ in range(4))
+        call("bh_device_rates", self.handle, ctypes.byref(e), ctypes.byref(f), ctypes.byref(m),
+             ctypes.byref(g))
+        return {"exp2_per_s": e.value, "fma_per_s": f.value, "mfma_f16_flops_per_s": m.value,
+                "gathers_per_s": g.value}
 
 
 class HipDirectSamplerEngine:

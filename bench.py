@@ -149,9 +149,12 @@ def traffic_of(name):
     return None
 
 
-def move_roofline(fac, dim):
+def move_roofline(fac, dim, rates=None):
     """HBM roofline of the move-bare! kernel at 2^24 walkers of a dim-D
-    correlated Gaussian."""
+    correlated Gaussian, and its bound from random line gathers: every
+    walker reads one partner record from a random line of the other half,
+    so a launch needs K line gathers (at the device's measured random-gather
+    rate) plus its streamed bytes at the HBM peak."""
     from bayadera_amd import _lib, models
     _, _, params, limits = mvn_problem(dim, 1001 + dim)
     sf = fac.mcmc_factory(models.mvn_model(dim))
@@ -171,13 +174,22 @@ def move_roofline(fac, dim):
     achieved = walkers_per_launch * bytes_per_walker / (avg_ms * 1e-3) / 1e9
     s.release()
     sf.release()
+    gather = None
+    if rates and rates.get("gathers_per_s"):
+        streamed = walkers_per_launch * (bytes_per_walker - 4 * dim)
+        bound_ms = (walkers_per_launch / rates["gathers_per_s"] + streamed / (HBM_PEAK_GBS * 1e9)) * 1e3
+        gather = {"partner_line_gathers_per_launch": walkers_per_launch,
+                  "gathers_per_s_measured": round(rates["gathers_per_s"], 0),
+                  "streamed_bytes_per_launch": int(streamed),
+                  "bound_ms": round(bound_ms, 4), "frac_of_bound": round(bound_ms / avg_ms, 4)}
     return {"kernel": f"bh_stretch_move_bare (DIM={dim}, K=2^23 walkers per launch)",
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_of(f"stretch_move_bare_dim{dim}_2p24_traffic.json"),
             "avg_kernel_ms": round(avg_ms, 4), "launches_timed": kt["launches"],
             "alpha": round(alpha, 4),
-            "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker)}
+            "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker),
+            "gather_bound": gather}
 
 
 # ---------------------------------------------------------------------------
@@ -282,8 +294,9 @@ def main_c1(args):
                      for k in ("t_burn", "t_run", "t_sample")},
     }
     if not args.no_roofline:
-        line["roofline"] = move_roofline(fac, DIM)
-        line["plain_move_roofline_dim16"] = move_roofline(fac, 16)
+        rates = fac.device_rates()
+        line["roofline"] = move_roofline(fac, DIM, rates)
+        line["plain_move_roofline_dim16"] = move_roofline(fac, 16, rates)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(params, limits)
     print(json.dumps(line), flush=True)
@@ -899,7 +912,7 @@ def main_c3(args):
         "setup_s": round(t_setup, 1),
     }
     if not args.no_roofline:
-        line["plain_move_roofline"] = [move_roofline(fac, 4), move_roofline(fac, 16)]
+        line["plain_move_roofline"] = [move_roofline(fac, 4, rates), move_roofline(fac, 16, rates)]
     if not args.no_cpu_baseline:
         cb = c3_cpu_baseline(Xd, y, limits)
         ess_per_ws = out[mname]["ess_min"] / (C3_W * (burn + run))
@@ -950,13 +963,14 @@ def main():
         ap.error("--c2-user-steps: run-sampler! needs at least 50 steps (0 skips the run)")
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding across GPUs is designed in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if args.roofline_only:
         import torch
         from bayadera_amd import engine
         fac = engine.HipBayaderaFactory(0, 256)
-        r = [move_roofline(fac, 4), move_roofline(fac, 16)]
+        rates = fac.device_rates()
+        r = [move_roofline(fac, 4, rates), move_roofline(fac, 16, rates)]
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)

@@ -88,15 +88,17 @@ int bh_ctx_create(int device, uint32_t wgs, bh_ctx** out);
 int bh_ctx_destroy(bh_ctx* ctx);
 /* compute units and WGS; processing-elements = cu * wgs (amd_gcn.clj:663-664) */
 int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs);
-/* Measured issue rates of the device (no reference counterpart): lane
- * operations per second of v_exp_f32 (the transcendental of the GLM
- * likelihood's softplus epilogue) and of v_fma_f32, and flops per second of
+/* Measured rates of the device (no reference counterpart): lane operations
+ * per second of v_exp_f32 (the transcendental of the GLM likelihood's
+ * softplus epilogue) and of v_fma_f32, flops per second of
  * v_mfma_f32_16x16x32_f16 (the likelihood's product), each from one timed
- * launch of independent chains on all compute units (under 2 ms each): the
- * practical rates against which the likelihood kernel is read.  Any output
- * may be NULL. */
+ * launch of independent chains on all compute units, and random 16-byte
+ * gathers per second from a 128 MiB buffer (the stretch move's partner
+ * reads; each costs a memory line).  The practical rates against which the
+ * likelihood and move kernels are read; a few ms in all.  Any output may be
+ * NULL (the gathers are skipped then). */
 int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s,
-                    double* mfma_f16_flops_per_s);
+                    double* mfma_f16_flops_per_s, double* gathers_per_s);
 
 /* ---------------------------------------------------------------------- */
 /* Direct sampler: GCNDirectSamplerEngine.sample for the uniform model      */

@@ -259,3 +259,4 @@ def test_device_rates_are_measured(fac):
     r = fac.device_rates()
     assert r["exp2_per_s"] > 1e11 and r["fma_per_s"] > r["exp2_per_s"]
     assert 1e14 < r["mfma_f16_flops_per_s"] < 3e15
+    assert 1e9 < r["gathers_per_s"] < 1e12
