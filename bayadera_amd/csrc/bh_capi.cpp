@@ -618,18 +618,20 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
 }
 
 // Work buffers of a GLM sampler (or of a log-density call) for up to n
-// points per launch.
+// points per launch.  Per-point buffers hold the walker's dimension
+// (s->dim: d, or d + 1 for the linear regression's log sigma).
 int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
     const GlmData& g = *s->f->glm;
+    const size_t dim = s->dim;
     size_t maxr = 0;
     for (int prec = BH_GLM_FP64; prec <= BH_GLM_FP16X2; ++prec)
         for (unsigned n : {n_points, k_half})
             if (n && n % bh::glm_point_quantum(prec) == 0)
                 maxr = std::max(maxr, (size_t)glm_ranges(s->ctx, g, prec, n) * n);
     BH_TRY(s->gpartial.alloc(std::max<size_t>(maxr, 1) * sizeof(double)));
-    BH_TRY(s->gsplit.alloc((size_t)3 * n_points * g.d * sizeof(uint16_t)));
+    BH_TRY(s->gsplit.alloc((size_t)3 * n_points * dim * sizeof(uint16_t)));
     if (k_half) {
-        BH_TRY(s->gY.alloc((size_t)k_half * g.d * sizeof(float)));
+        BH_TRY(s->gY.alloc((size_t)k_half * dim * sizeof(float)));
         BH_TRY(s->gz.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gu2.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gflag.alloc((size_t)k_half * sizeof(unsigned)));
@@ -637,7 +639,7 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
         BH_TRY(s->gcount.alloc(sizeof(unsigned)));
         BH_TRY(s->gtotals.alloc(2 * sizeof(unsigned long long)));
         BH_TRY(s->gpart64.alloc((size_t)kGuardRanges * 2 * kGuardListMax * sizeof(double)));
-        BH_TRY(s->gpts64.alloc((size_t)2 * kGuardListMax * g.d * sizeof(double)));
+        BH_TRY(s->gpts64.alloc((size_t)2 * kGuardListMax * dim * sizeof(double)));
         BH_HIP(hipMemsetAsync(s->gtotals.p, 0, 2 * sizeof(unsigned long long), s->stream));
     }
     return BH_OK;

@@ -260,3 +260,22 @@ def test_device_rates_are_measured(fac):
     assert r["exp2_per_s"] > 1e11 and r["fma_per_s"] > r["exp2_per_s"]
     assert 1e14 < r["mfma_f16_flops_per_s"] < 3e15
     assert 1e9 < r["gathers_per_s"] < 1e12
+
+
+def test_glm_gaussian_sampler_state_at_8192_walkers(fac):
+    """The linear regression's walkers carry d + 1 coordinates (log sigma):
+    its work buffers are sized by the walker dimension, so at a large
+    ensemble the stored log posteriors still equal the fp64 formula's."""
+    X, y, pts = _linreg(20000, 16, 43)
+    sf = fac.mcmc_factory(models.glm_gaussian_model(X, y, 10.0, 1.0))
+    s = sf.create_sampler(5, 8192, np.zeros(1, np.float32)).set_partner(_lib.BH_PARTNER_WALKER)
+    lo = pts.min(0) - 0.01
+    hi = pts.max(0) + 0.01
+    s.init(5)
+    s.init_position(6, np.stack([lo, hi], 1).reshape(-1))
+    s.burn_in(30, 2.0)
+    xs, lp, _ = s.state()
+    idx = np.r_[0:64, 4096:4160, 8128:8192]
+    ref = _linreg_lp64(X, y, xs[idx])
+    assert np.allclose(lp[idx], ref, rtol=2e-6)
+    s.release()
