@@ -383,8 +383,14 @@ def main_c2(args):
     xs = xs.reshape(-1, C2_D + 1)
     lp64 = c2_loglik64(rows, xs[:2048])
     lp_err = np.abs(lpv[:2048].astype(np.float64) - lp64) / np.abs(lp64)
-    gm = res["means"].astype(np.float64).mean(0)
-    err = np.abs(gm[:C2_D] - bh) / ac.sigma[:C2_D].astype(np.float64)
+    m = res["means"].astype(np.float64)
+    gm = m.mean(0)
+    # MC-SE per dim from the reference's acor, Sokal's estimate where it
+    # gives none
+    sig = ac.sigma.astype(np.float64)
+    sok = np.sqrt(sokal_tau(m) * m.var(0) / m.shape[0])
+    sig = np.where(np.isfinite(sig), sig, sok)
+    err = np.abs(gm[:C2_D] - bh) / sig[:C2_D]
     flops_ws = C2_N * (2 * C2_D + 3)
     achieved = (C2_W // 2) * flops_ws / (kt["avg_ms"] * 1e-3) / 1e12
     line = {
