@@ -141,32 +141,66 @@ gb_init_kernel(unsigned seed, const unsigned* __restrict__ dims,
     }
 }
 
-// Writes y * t (t = log2(e) * column scale) as fp32 and as nsplit 16-bit
-// terms (bf16; or fp16 of y * t * 2^8 when f16 is set; hi, mid, lo), for
-// point k of problem p.
-__device__ __forceinline__ void gb_store_point(const float (&y)[kGbD], const float* __restrict__ t,
+// Writes r * t (t = log2(e) * column scale) as fp32 and as nsplit 16-bit
+// terms (bf16; or fp16 of r * t * 2^8 when f16 is set; hi, mid, lo), for
+// point k of problem p: the first D of its 64 slots (D = 32 or 64, the
+// problem's row stride, all that the contraction kernels read; slots from
+// d on hold zeros), in 16-byte stores.  r is overwritten (the scaled
+// values, then the split residuals).
+template <int D>
+__device__ __forceinline__ void gb_store_point(float (&r)[D], const float* __restrict__ t,
                                                unsigned d, unsigned P, unsigned p, unsigned n,
                                                unsigned k, float* __restrict__ ysc,
                                                void* __restrict__ ysplit, int nsplit, int f16) {
-    float* o = ysc + ((size_t)p * n + k) * kGbD;
 #pragma unroll
-    for (int i = 0; i < kGbD; ++i) {
-        const float v = (unsigned)i < d ? y[i] * t[i] : 0.0f;
-        o[i] = v;
-        float r = f16 ? v * kGbF16Up : v;
-        for (int sp = 0; sp < nsplit; ++sp) {
-            const size_t at = (((size_t)sp * P + p) * n + k) * kGbD + i;
+    for (int i = 0; i < D; ++i) r[i] = (unsigned)i < d ? r[i] * t[i] : 0.0f;
+    float4* o = reinterpret_cast<float4*>(ysc + ((size_t)p * n + k) * kGbD);
+#pragma unroll
+    for (int c = 0; c < D / 4; ++c) o[c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+    if (f16) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) r[i] *= kGbF16Up;
+    }
+    for (int sp = 0; sp < nsplit; ++sp) {
+        const size_t at = (((size_t)sp * P + p) * n + k) * kGbD;
+#pragma unroll
+        for (int c = 0; c < D / 8; ++c) {
             if (f16) {
-                const _Float16 h = (_Float16)r;
-                static_cast<_Float16*>(ysplit)[at] = h;
-                r = r - (float)h;
+                f16x8 h;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    h[e] = (_Float16)r[8 * c + e];
+                    r[8 * c + e] -= (float)h[e];
+                }
+                reinterpret_cast<f16x8*>(static_cast<_Float16*>(ysplit) + at)[c] = h;
             } else {
-                const __bf16 h = (__bf16)r;
-                static_cast<__bf16*>(ysplit)[at] = h;
-                r = r - (float)h;
+                bf16x8 h;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    h[e] = (__bf16)r[8 * c + e];
+                    r[8 * c + e] -= (float)h[e];
+                }
+                reinterpret_cast<bf16x8*>(static_cast<__bf16*>(ysplit) + at)[c] = h;
             }
         }
     }
+}
+
+template <int D>
+__device__ __forceinline__ void gb_points_one(const float* __restrict__ x, const float* t,
+                                              unsigned d, unsigned P, unsigned p, unsigned n,
+                                              unsigned k, float* ysc, void* ysplit, int nsplit,
+                                              int f16) {
+    float y[D];
+#pragma unroll
+    for (int c = 0; c < D / 4; ++c) {
+        const float4 v = reinterpret_cast<const float4*>(x)[c];
+        y[4 * c] = v.x;
+        y[4 * c + 1] = v.y;
+        y[4 * c + 2] = v.z;
+        y[4 * c + 3] = v.w;
+    }
+    gb_store_point<D>(y, t, d, P, p, n, k, ysc, ysplit, nsplit, f16);
 }
 
 // Scaled points (and their splits) of all walkers, for the initial log
@@ -178,11 +212,41 @@ gb_points_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dims
     const unsigned p = blockIdx.y;
     const unsigned k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
-    float y[kGbD];
+    const unsigned d = dims[p];
     const float* x = xs + ((size_t)p * n + k) * kGbD;
+    const float* t = tscale + (size_t)p * kGbD;
+    if (d <= 32)
+        gb_points_one<32>(x, t, d, P, p, n, k, ysc, ysplit, nsplit, f16);
+    else
+        gb_points_one<64>(x, t, d, P, p, n, k, ysc, ysplit, nsplit, f16);
+}
+
+// The proposal of walker k of problem p (dimension d <= D): reads of its
+// position and its partner's, and writes of Y, in 16-byte pieces of the
+// first D slots.
+template <int D>
+__device__ __forceinline__ void gb_propose_one(const float* __restrict__ xk,
+                                               const float* __restrict__ sj, float z, unsigned d,
+                                               float* __restrict__ yo, const float* t, unsigned P,
+                                               unsigned p, unsigned K, unsigned k, float* ysc,
+                                               void* ysplit, int nsplit, int f16) {
+    float y[D];
 #pragma unroll
-    for (int i = 0; i < kGbD; ++i) y[i] = x[i];
-    gb_store_point(y, tscale + (size_t)p * kGbD, dims[p], P, p, n, k, ysc, ysplit, nsplit, f16);
+    for (int c = 0; c < D / 4; ++c) {
+        const float4 xv = reinterpret_cast<const float4*>(xk)[c];
+        const float4 sv = reinterpret_cast<const float4*>(sj)[c];
+        const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+        const float sa[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const unsigned i = 4 * c + e;
+            y[i] = i < d ? __builtin_fmaf(z, xa[e] - sa[e], sa[e]) : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D / 4; ++c)
+        reinterpret_cast<float4*>(yo)[c] = make_float4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
+    gb_store_point<D>(y, t, d, P, p, K, k, ysc, ysplit, nsplit, f16);
 }
 
 // Proposals of one half-step for all problems (the move of
@@ -208,18 +272,16 @@ gb_propose_kernel(unsigned seed0, int half, unsigned oe, unsigned step, float a,
     const float z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
     unsigned j = (unsigned)(u.x * (float)K);
     j = j < K ? j : K - 1;
-    float y[kGbD];
     const float* xk = X + (size_t)k * kGbD;
     const float* sj = S + (size_t)j * kGbD;
-#pragma unroll
-    for (int i = 0; i < kGbD; ++i)
-        y[i] = (unsigned)i < d ? __builtin_fmaf(z, xk[i] - sj[i], sj[i]) : 0.0f;
     float* yo = Y + ((size_t)p * K + k) * kGbD;
-#pragma unroll
-    for (int i = 0; i < kGbD; ++i) yo[i] = y[i];
+    const float* t = tscale + (size_t)p * kGbD;
+    if (d <= 32)
+        gb_propose_one<32>(xk, sj, z, d, yo, t, P, p, K, k, ysc, ysplit, nsplit, f16);
+    else
+        gb_propose_one<64>(xk, sj, z, d, yo, t, P, p, K, k, ysc, ysplit, nsplit, f16);
     zbuf[(size_t)p * K + k] = z;
     u2buf[(size_t)p * K + k] = u.z;
-    gb_store_point(y, tscale + (size_t)p * kGbD, d, P, p, K, k, ysc, ysplit, nsplit, f16);
 }
 
 // Block -> (unit, 256-point group), XCD-aware as glm_tile: the consecutive
@@ -375,8 +437,18 @@ __device__ __forceinline__ double gb_logpost(const float* __restrict__ y, unsign
                                              unsigned ue, unsigned n, unsigned k,
                                              const double* __restrict__ lin, double scale,
                                              double prior_prec) {
+    // the units' partial sums in unit order, loaded eight at a time so the
+    // loads are in flight together
     double s = 0.0;
-    for (unsigned u = ub; u < ue; ++u) s += partial[(size_t)u * n + k];
+    unsigned u = ub;
+    for (; u + 8 <= ue; u += 8) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = partial[(size_t)(u + e) * n + k];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s += v[e];
+    }
+    for (; u < ue; ++u) s += partial[(size_t)u * n + k];
     double l = 0.0, q = 0.0;
     for (unsigned i = 0; i < d; ++i) {
         l = __builtin_fma(lin[i], (double)y[i], l);
@@ -398,6 +470,103 @@ gb_logpost_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dim
                                        prior_prec);
 }
 
+// The accept test and accounting of walker k of problem p (dimension
+// d <= D), in 16-byte accesses over the first D slots of its proposal and
+// position (the slots from d on are zeros in both, and so are the linear
+// coefficients there).
+template <int D>
+__device__ __forceinline__ void gb_accept_one(int half, unsigned p, unsigned k, unsigned d,
+                                              const unsigned* __restrict__ ubeg,
+                                              const float* __restrict__ Y,
+                                              const float* __restrict__ zbuf,
+                                              const float* __restrict__ u2buf,
+                                              const double* __restrict__ partial,
+                                              const double* __restrict__ lin, double scale,
+                                              double prior_prec, float* __restrict__ xs,
+                                              double* __restrict__ lp, float beta, unsigned K,
+                                              int accu, unsigned long long* __restrict__ acc_cnt,
+                                              float* __restrict__ msum,
+                                              float* __restrict__ msum2) {
+    __shared__ float red[2][4][kGbD];
+    __shared__ unsigned cnt[4];
+    __shared__ double lin_s[D];
+    if (threadIdx.x < D) lin_s[threadIdx.x] = lin[(size_t)p * kGbD + threadIdx.x];
+    __syncthreads();
+    const float4* y4 = reinterpret_cast<const float4*>(Y + ((size_t)p * K + k) * kGbD);
+    // lp(Y): the units' partial sums in unit order, eight loads in flight
+    double s = 0.0;
+    unsigned u = ubeg[p];
+    const unsigned ue = ubeg[p + 1];
+    for (; u + 8 <= ue; u += 8) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = partial[(size_t)(u + e) * K + k];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s += v[e];
+    }
+    for (; u < ue; ++u) s += partial[(size_t)u * K + k];
+    double l = 0.0, q = 0.0;
+#pragma unroll 4
+    for (int c = 0; c < D / 4; ++c) {
+        const float4 v = y4[c];
+        const float a[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            l = __builtin_fma(lin_s[4 * c + e], (double)a[e], l);
+            q = __builtin_fma((double)a[e], (double)a[e], q);
+        }
+    }
+    const double lpy = l - scale * s - 0.5 * prior_prec * q;
+    const size_t w = (size_t)p * 2 * K + (half ? K : 0) + k;
+    float4* xk4 = reinterpret_cast<float4*>(xs + w * kGbD);
+    const float z = zbuf[(size_t)p * K + k];
+    float zp = 1.0f;
+    for (unsigned i = 0; i + 1 < d; ++i) zp *= z;
+    const float dl = (float)(lpy - lp[w]);
+    const float qa = __builtin_isfinite(lpy) ? zp * expf(beta * dl) : 0.0f;
+    const bool accepted = u2buf[(size_t)p * K + k] <= qa;
+    if (accepted) {
+#pragma unroll 4
+        for (int c = 0; c < D / 4; ++c) xk4[c] = y4[c];
+        lp[w] = lpy;
+    }
+    if (!accu) return;
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const unsigned long long ball = __ballot(accepted);
+    if (lane == 0) cnt[wave] = (unsigned)__popcll(ball);
+    // the walker's coordinates after the move, four at a time, reduced over
+    // the wave coordinate by coordinate
+    const float4* src = accepted ? y4 : xk4;
+    for (unsigned c = 0; 4 * c < d; ++c) {
+        const float4 v = src[c];
+        const float a[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float s1 = a[e], s2 = a[e] * a[e];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                s1 += __shfl_xor(s1, off, 64);
+                s2 += __shfl_xor(s2, off, 64);
+            }
+            if (lane == 0 && 4 * c + e < d) {
+                red[0][wave][4 * c + e] = s1;
+                red[1][wave][4 * c + e] = s2;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(&acc_cnt[p], (unsigned long long)((cnt[0] + cnt[1]) + (cnt[2] + cnt[3])));
+    const unsigned nslots = 2 * gridDim.x;
+    const unsigned slot = (half ? gridDim.x : 0u) + blockIdx.x;
+    if (threadIdx.x < d) {
+        const unsigned i = threadIdx.x;
+        const size_t o = ((size_t)p * nslots + slot) * kGbD + i;
+        msum[o] = (red[0][0][i] + red[0][1][i]) + (red[0][2][i] + red[0][3][i]);
+        msum2[o] = (red[1][0][i] + red[1][1][i]) + (red[1][2][i] + red[1][3][i]);
+    }
+}
+
 // Accept test of one half-step for all problems (amd-gcn-mcmc-stretch.cl:
 // 66-78 with z^(d_p - 1) by repeated multiplication), and with accu != 0
 // the accounting: accepted walkers per problem (acc_cnt), and per 256-walker
@@ -412,54 +581,15 @@ gb_accept_kernel(int half, const unsigned* __restrict__ dims, const unsigned* __
                  float* __restrict__ xs, double* __restrict__ lp, float beta, unsigned K, int accu,
                  unsigned long long* __restrict__ acc_cnt, float* __restrict__ msum,
                  float* __restrict__ msum2) {
-    __shared__ float red[2][4][kGbD];
-    __shared__ unsigned cnt[4];
     const unsigned p = blockIdx.y;
     const unsigned k = blockIdx.x * 256 + threadIdx.x;
     const unsigned d = dims[p];
-    const float* y = Y + ((size_t)p * K + k) * kGbD;
-    const double lpy = gb_logpost(y, d, partial, ubeg[p], ubeg[p + 1], K, k,
-                                  lin + (size_t)p * kGbD, scale, prior_prec);
-    const size_t w = (size_t)p * 2 * K + (half ? K : 0) + k;
-    float* xk = xs + w * kGbD;
-    const float z = zbuf[(size_t)p * K + k];
-    float zp = 1.0f;
-    for (unsigned i = 0; i + 1 < d; ++i) zp *= z;
-    const float dl = (float)(lpy - lp[w]);
-    const float q = __builtin_isfinite(lpy) ? zp * expf(beta * dl) : 0.0f;
-    const bool accepted = u2buf[(size_t)p * K + k] <= q;
-    if (accepted) {
-        for (unsigned i = 0; i < d; ++i) xk[i] = y[i];
-        lp[w] = lpy;
-    }
-    if (!accu) return;
-    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const unsigned long long ball = __ballot(accepted);
-    if (lane == 0) cnt[wave] = (unsigned)__popcll(ball);
-    for (unsigned i = 0; i < d; ++i) {
-        const float x = xk[i];
-        float s1 = x, s2 = x * x;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            s1 += __shfl_xor(s1, off, 64);
-            s2 += __shfl_xor(s2, off, 64);
-        }
-        if (lane == 0) {
-            red[0][wave][i] = s1;
-            red[1][wave][i] = s2;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        atomicAdd(&acc_cnt[p], (unsigned long long)((cnt[0] + cnt[1]) + (cnt[2] + cnt[3])));
-    const unsigned nslots = 2 * gridDim.x;
-    const unsigned slot = (half ? gridDim.x : 0u) + blockIdx.x;
-    if (threadIdx.x < d) {
-        const unsigned i = threadIdx.x;
-        const size_t o = ((size_t)p * nslots + slot) * kGbD + i;
-        msum[o] = (red[0][0][i] + red[0][1][i]) + (red[0][2][i] + red[0][3][i]);
-        msum2[o] = (red[1][0][i] + red[1][1][i]) + (red[1][2][i] + red[1][3][i]);
-    }
+    if (d <= 32)
+        gb_accept_one<32>(half, p, k, d, ubeg, Y, zbuf, u2buf, partial, lin, scale, prior_prec,
+                          xs, lp, beta, K, accu, acc_cnt, msum, msum2);
+    else
+        gb_accept_one<64>(half, p, k, d, ubeg, Y, zbuf, u2buf, partial, lin, scale, prior_prec,
+                          xs, lp, beta, K, accu, acc_cnt, msum, msum2);
 }
 
 // End of an accounted step: per problem and coordinate, the step's

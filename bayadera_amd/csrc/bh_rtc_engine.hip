@@ -738,8 +738,17 @@ __device__ __forceinline__ double bh_glm_logpost(const float (&y)[DIM], unsigned
                                                  const double* __restrict__ partial,
                                                  unsigned nranges, const double* __restrict__ lin,
                                                  double scale, double prior_prec) {
+    // in range order, eight loads in flight at a time
     double s = 0.0;
-    for (unsigned r = 0; r < nranges; ++r) s += partial[(size_t)r * n + k];
+    unsigned r = 0;
+    for (; r + 8 <= nranges; r += 8) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = partial[(size_t)(r + e) * n + k];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s += v[e];
+    }
+    for (; r < nranges; ++r) s += partial[(size_t)r * n + k];
     return bh_glm_logpost_sum(y, s, lin, scale, prior_prec);
 }
 
