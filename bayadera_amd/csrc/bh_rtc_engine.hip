@@ -106,19 +106,18 @@ __device__ __forceinline__ void bh_store_walker(float* p, const float (&v)[DIM])
     }
 }
 
-// The proposal half of the stretch move (amd-gcn-mcmc-stretch.cl:46-65):
+// The draws of the stretch move (amd-gcn-mcmc-stretch.cl:46-57):
 //   counter {k, step, odd_or_even, 0xbeeff00d}, key {seed, 0xdecafbad}
 //   z  = fma((a-2+1/a)*u1, u1, (2(1-1/a))*u1) + 1/a                     (:53-54)
-//   Y  = fma(z, Xk - Xj, Xj)                                           (:64)
 // Partner: FLAT reads the complementary half at element offset
 // j0 = (uint)((u0*K)*DIM) (:57), wrapping modulo K*DIM where the reference
-// reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1).
-// Returns u2, the uniform of the accept test.
+// reads out of bounds; WALKER reads walker min((uint)(u0*K), K-1), i.e.
+// element offset j*DIM.  Sets z and the partner's first element offset
+// (< K*DIM); returns u2, the uniform of the accept test.
 template <int PARTNER>
-__device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned seed,
-                                           unsigned odd_or_even, unsigned step, float a,
-                                           const float* S, const float (&xk)[DIM],
-                                           float (&y)[DIM], float& z) {
+__device__ __forceinline__ float bh_stretch_draw(unsigned k, unsigned K, unsigned seed,
+                                                 unsigned odd_or_even, unsigned step, float a,
+                                                 float& z, unsigned& start) {
     const bh_f32x4 u = bh_u01x4(
         bh_philox4x32_10(bh_u32x4{k, step, odd_or_even, BH_CTR3}, seed, BH_KEY1_STRETCH));
     const float inv_a = 1.0f / a;
@@ -129,6 +128,28 @@ __device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned see
         const unsigned total = K * DIM;
         unsigned e = (unsigned)((u.x * (float)K) * (float)DIM);
         if (e >= total) e -= total;
+        start = e;
+    } else {
+        unsigned j = (unsigned)(u.x * (float)K);
+        j = j < K ? j : K - 1;
+        start = j * DIM;
+    }
+    return u.z;
+}
+
+// The proposal half of the stretch move (amd-gcn-mcmc-stretch.cl:46-65):
+// the draws above, then Y = fma(z, Xk - Xj, Xj) (:64) with the partner's
+// coordinates read from S (FLAT: element i at (start + i) mod K*DIM).
+// Returns u2, the uniform of the accept test.
+template <int PARTNER>
+__device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned seed,
+                                           unsigned odd_or_even, unsigned step, float a,
+                                           const float* S, const float (&xk)[DIM],
+                                           float (&y)[DIM], float& z) {
+    unsigned e;
+    const float u2 = bh_stretch_draw<PARTNER>(k, K, seed, odd_or_even, step, a, z, e);
+    if constexpr (PARTNER == BH_PARTNER_FLAT) {
+        const unsigned total = K * DIM;
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
             const float xj = S[e];
@@ -136,14 +157,12 @@ __device__ __forceinline__ float bh_propose(unsigned k, unsigned K, unsigned see
             e = (e + 1 == total) ? 0u : e + 1;
         }
     } else {
-        unsigned j = (unsigned)(u.x * (float)K);
-        j = j < K ? j : K - 1;
         float sj[DIM];
-        bh_load_walker(S + (size_t)j * DIM, sj);
+        bh_load_walker(S + e, sj);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) y[i] = __builtin_fmaf(z, xk[i] - sj[i], sj[i]);
     }
-    return u.z;
+    return u2;
 }
 
 // z^(DIM-1) by repeated multiplication
@@ -294,6 +313,156 @@ __device__ __forceinline__ void bh_store_walker_nt(float* p, const float (&v)[DI
     }
 }
 
+// Staged half-step for DIM >= 8: a thread's own record and its partner's
+// span DIM/4 16-byte pieces (DIM floats when DIM % 4 != 0), and reading them
+// thread by thread makes every load instruction of a wave touch up to 64
+// separate lines (4 to 16 times the requests of a contiguous read).  Here
+// the block's WGS records, contiguous in X, are read and written as
+// consecutive pieces by consecutive threads, and each partner's pieces by
+// consecutive threads too; an LDS tile transposes between that order and
+// one record per thread.  Rows of the tile are an odd number of 16-byte
+// units (odd number of floats for DIM % 4 != 0) apart, so threads reading
+// consecutive rows hit distinct banks.  Arithmetic and results are those of
+// the thread-per-record path.
+constexpr int bh_stage_stride() {
+    return DIM % 4 == 0 ? ((DIM / 4) % 2 ? DIM : DIM + 4) : (DIM % 2 ? DIM : DIM + 1);
+}
+constexpr int kBhStageStride = bh_stage_stride();
+constexpr bool kBhStage = DIM >= 8 && WGS * kBhStageStride * 4 <= 49152;
+
+template <bool ACCU, int PARTNER>
+__device__ __forceinline__ void bh_stretch_half_staged(unsigned seed, unsigned odd_or_even,
+                                                       unsigned data_len, unsigned params_len,
+                                                       const float* __restrict__ params,
+                                                       const float* __restrict__ S,
+                                                       float* __restrict__ X,
+                                                       float* __restrict__ logfn_X, float a,
+                                                       float beta, unsigned step, unsigned K,
+                                                       unsigned* __restrict__ accept,
+                                                       float* __restrict__ means) {
+    typedef float bh_f4 __attribute__((ext_vector_type(4)));
+    constexpr int ST = kBhStageStride;
+    constexpr bool V4 = DIM % 4 == 0;                     // records as 16-byte pieces
+    constexpr bool SV4 = V4 && PARTNER == BH_PARTNER_WALKER;  // partners too
+    constexpr int NP = V4 ? DIM / 4 : DIM;                 // pieces per record
+    __shared__ __attribute__((aligned(16))) float tile[WGS * ST];
+    __shared__ unsigned pstart[WGS];
+    __shared__ unsigned long long acc_mask[WGS / 64];
+    const unsigned t = threadIdx.x;
+    const unsigned k = blockIdx.x * WGS + t;
+    float* xb = X + (size_t)blockIdx.x * WGS * DIM;
+    // own records: piece g = i*WGS + t is piece g % NP of record g / NP
+    float xs[NP * (V4 ? 4 : 1)];
+    if constexpr (V4) {
+        const bh_f4* q = reinterpret_cast<const bh_f4*>(__builtin_assume_aligned(xb, 16));
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const bh_f4 v = __builtin_nontemporal_load(q + i * WGS + t);
+            xs[4 * i] = v.x;
+            xs[4 * i + 1] = v.y;
+            xs[4 * i + 2] = v.z;
+            xs[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) xs[i] = __builtin_nontemporal_load(xb + i * WGS + t);
+    }
+    float lx = __builtin_nontemporal_load(logfn_X + k);
+    float z;
+    unsigned e;
+    const float u2 = bh_stretch_draw<PARTNER>(k, K, seed, odd_or_even, step, a, z, e);
+    pstart[t] = e;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const unsigned g = i * WGS + t, w = g / NP, c = g % NP;
+        if constexpr (V4) {
+            *reinterpret_cast<bh_f4*>(&tile[w * ST + 4 * c]) =
+                bh_f4{xs[4 * i], xs[4 * i + 1], xs[4 * i + 2], xs[4 * i + 3]};
+        } else {
+            tile[w * ST + c] = xs[i];
+        }
+    }
+    __syncthreads();
+    // partner pieces, consecutive threads on consecutive pieces of a partner
+    constexpr int NPS = SV4 ? DIM / 4 : DIM;
+    float ss[DIM];
+    if constexpr (SV4) {
+        const bh_f4* q = reinterpret_cast<const bh_f4*>(__builtin_assume_aligned(S, 16));
+#pragma unroll
+        for (int i = 0; i < NPS; ++i) {
+            const unsigned g = i * WGS + t, w = g / NPS, c = g % NPS;
+            const bh_f4 v = q[pstart[w] / 4 + c];
+            ss[4 * i] = v.x;
+            ss[4 * i + 1] = v.y;
+            ss[4 * i + 2] = v.z;
+            ss[4 * i + 3] = v.w;
+        }
+    } else {
+        const unsigned total = K * DIM;
+#pragma unroll
+        for (int i = 0; i < NPS; ++i) {
+            const unsigned g = i * WGS + t, w = g / NPS, c = g % NPS;
+            unsigned idx = pstart[w] + c;
+            if (PARTNER == BH_PARTNER_FLAT && idx >= total) idx -= total;
+            ss[i] = S[idx];
+        }
+    }
+    float xk[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) xk[i] = tile[t * ST + i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NPS; ++i) {
+        const unsigned g = i * WGS + t, w = g / NPS, c = g % NPS;
+        if constexpr (SV4) {
+            *reinterpret_cast<bh_f4*>(&tile[w * ST + 4 * c]) =
+                bh_f4{ss[4 * i], ss[4 * i + 1], ss[4 * i + 2], ss[4 * i + 3]};
+        } else {
+            tile[w * ST + c] = ss[i];
+        }
+    }
+    __syncthreads();
+    float y[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        const float xj = tile[t * ST + i];
+        y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+    }
+    const float ly = LOGFN(data_len, params_len, params, DIM, y);
+    const float qv = __builtin_isfinite(ly) ? bh_zpow(z) * expf(beta * (ly - lx)) : 0.0f;
+    const bool accepted = u2 <= qv;
+    if (accepted) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            xk[i] = y[i];
+            tile[t * ST + i] = y[i];
+        }
+        __builtin_nontemporal_store(ly, logfn_X + k);
+    }
+    const unsigned long long m = __ballot(accepted);
+    if ((t & 63u) == 0) acc_mask[t >> 6] = m;
+    __syncthreads();
+    // accepted records back to X as consecutive pieces
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const unsigned g = i * WGS + t, w = g / NP, c = g % NP;
+        if ((acc_mask[w >> 6] >> (w & 63u)) & 1ull) {
+            if constexpr (V4) {
+                bh_f4* q = reinterpret_cast<bh_f4*>(__builtin_assume_aligned(xb, 16));
+                __builtin_nontemporal_store(*reinterpret_cast<const bh_f4*>(&tile[w * ST + 4 * c]),
+                                            q + g);
+            } else {
+                __builtin_nontemporal_store(tile[w * ST + c], xb + g);
+            }
+        }
+    }
+    if (ACCU) {
+        __shared__ float red[BH_ACCU_CHUNK * WGS];
+        __shared__ unsigned wave_counts[WGS / 64];
+        bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, red, wave_counts);
+    }
+}
+
 template <bool ACCU, int PARTNER>
 __device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_even,
                                                 unsigned data_len, unsigned params_len,
@@ -303,21 +472,27 @@ __device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_e
                                                 unsigned step, unsigned K,
                                                 unsigned* __restrict__ accept,
                                                 float* __restrict__ means) {
-    const unsigned k = blockIdx.x * WGS + threadIdx.x;
-    float xk[DIM];
-    float* xp = X + (size_t)k * DIM;
-    bh_load_walker_nt(xp, xk);
-    float lx = __builtin_nontemporal_load(logfn_X + k);
-    const bool accepted = bh_stretch_walker<PARTNER>(k, K, seed, odd_or_even, step, a, beta, S,
-                                                     xk, lx, data_len, params_len, params);
-    if (accepted) {
-        bh_store_walker_nt(xp, xk);
-        __builtin_nontemporal_store(lx, logfn_X + k);
-    }
-    if (ACCU) {
-        __shared__ float red[BH_ACCU_CHUNK * WGS];
-        __shared__ unsigned wave_counts[WGS / 64];
-        bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, red, wave_counts);
+    if constexpr (kBhStage) {
+        bh_stretch_half_staged<ACCU, PARTNER>(seed, odd_or_even, data_len, params_len, params, S,
+                                              X, logfn_X, a, beta, step, K, accept, means);
+        return;
+    } else {
+        const unsigned k = blockIdx.x * WGS + threadIdx.x;
+        float xk[DIM];
+        float* xp = X + (size_t)k * DIM;
+        bh_load_walker_nt(xp, xk);
+        float lx = __builtin_nontemporal_load(logfn_X + k);
+        const bool accepted = bh_stretch_walker<PARTNER>(k, K, seed, odd_or_even, step, a, beta, S,
+                                                         xk, lx, data_len, params_len, params);
+        if (accepted) {
+            bh_store_walker_nt(xp, xk);
+            __builtin_nontemporal_store(lx, logfn_X + k);
+        }
+        if (ACCU) {
+            __shared__ float red[BH_ACCU_CHUNK * WGS];
+            __shared__ unsigned wave_counts[WGS / 64];
+            bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, red, wave_counts);
+        }
     }
 }
 

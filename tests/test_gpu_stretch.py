@@ -146,21 +146,29 @@ def test_uniform_run_sampler_exact_vs_oracle(uniform_sf, exec_mode):
     assert np.array_equal(s.state()[0].reshape(-1), ref.xs)
 
 
+def _mvn_params(dim, seed=1001):
+    rng = np.random.default_rng(seed)
+    mu = rng.standard_normal(dim).astype(np.float32)
+    A = rng.standard_normal((dim, dim))
+    P = np.linalg.inv(A @ A.T + np.eye(dim)).astype(np.float32)
+    return np.concatenate([mu, P.reshape(-1)]).astype(np.float32)
+
+
+# DIM 4: one 16-byte piece per record, a record per thread; DIM 9 and 16:
+# records staged through LDS (float pieces, 16-byte pieces)
+@pytest.mark.parametrize("dim,walkers", [(4, 1024), (9, 512), (16, 512)])
 @pytest.mark.parametrize("partner", [_lib.BH_PARTNER_FLAT, _lib.BH_PARTNER_WALKER])
-def test_resident_equals_stepwise_mvn(fac, partner):
-    """4-D correlated Gaussian: the resident kernel and the two-kernels-per-step
-    path produce identical trajectories, accounting and samples."""
-    rng = np.random.default_rng(1001)
-    mu = rng.standard_normal(4).astype(np.float32)
-    A = rng.standard_normal((4, 4))
-    P = np.linalg.inv(A @ A.T + np.eye(4)).astype(np.float32)
-    params = np.concatenate([mu, P.reshape(-1)])
-    sf = fac.mcmc_factory(models.mvn_model(4))
+def test_resident_equals_stepwise_mvn(fac, partner, dim, walkers):
+    """Correlated Gaussian: the resident kernel (one record per thread, LDS
+    resident ensemble) and the two-kernels-per-step path produce identical
+    trajectories, accounting and samples."""
+    params = _mvn_params(dim)
+    sf = fac.mcmc_factory(models.mvn_model(dim))
     out = []
     for mode in (STEPWISE, RESIDENT):
-        s = sf.create_sampler(123, 1024, params).set_exec(mode).set_partner(partner)
+        s = sf.create_sampler(123, walkers, params).set_exec(mode).set_partner(partner)
         s.init(123)
-        s.init_position(123, np.tile([-5.0, 5.0], 4))
+        s.init_position(123, np.tile([-5.0, 5.0], dim))
         s.burn_in(64, 2.0)
         r = s.run_sampler(256, 2.0)
         smp = s.sample(3000)
@@ -169,25 +177,23 @@ def test_resident_equals_stepwise_mvn(fac, partner):
         assert np.array_equal(np.asarray(x), np.asarray(y))
 
 
+@pytest.mark.parametrize("dim", [4, 9, 16])
 @pytest.mark.parametrize("partner", [O.PARTNER_FLAT, O.PARTNER_WALKER])
-def test_mvn_steps_vs_oracle(fac, partner):
-    rng = np.random.default_rng(1001)
-    mu = rng.standard_normal(4).astype(np.float32)
-    A = rng.standard_normal((4, 4))
-    P = np.linalg.inv(A @ A.T + np.eye(4)).astype(np.float32)
-    params = np.concatenate([mu, P.reshape(-1)]).astype(np.float32)
-    sf = fac.mcmc_factory(models.mvn_model(4))
-    s = sf.create_sampler(5, 2048, params).set_partner(partner)
+def test_mvn_steps_vs_oracle(fac, partner, dim):
+    params = _mvn_params(dim)
+    sf = fac.mcmc_factory(models.mvn_model(dim))
+    s = sf.create_sampler(5, 2048, params).set_partner(partner).set_exec(STEPWISE)
     s.init(5)
-    s.init_position(6, np.tile([-5.0, 5.0], 4))
-    ref = O.Stretch(O.MVN_LOGPDF, 4, 2048, params, 5, params_len=20, partner=partner, threads=4)
+    s.init_position(6, np.tile([-5.0, 5.0], dim))
+    ref = O.Stretch(O.MVN_LOGPDF, dim, 2048, params, 5, params_len=dim + dim * dim,
+                    partner=partner, threads=4)
     ref.init(5)
-    ref.init_position(6, np.tile(np.array([[-5.0, 5.0]], np.float32), (4, 1)))
+    ref.init_position(6, np.tile(np.array([[-5.0, 5.0]], np.float32), (dim, 1)))
     assert np.array_equal(s.state()[0].reshape(-1), ref.xs)
     s.burn_in(5, 2.0)
     ref.burn_in(5, 2.0)
-    x = s.state()[0].reshape(-1, 4)
-    r = ref.xs.reshape(-1, 4)
+    x = s.state()[0].reshape(-1, dim)
+    r = ref.xs.reshape(-1, dim)
     same = np.all(x == r, axis=1).mean()
     assert same > 0.995, same
     close = np.abs(x - r) <= 1e-5 * np.maximum(1.0, np.abs(r))
