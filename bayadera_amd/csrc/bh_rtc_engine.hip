@@ -201,6 +201,18 @@ __device__ __forceinline__ void bh_accu(const float (&xk)[DIM], bool accepted, u
     }
 }
 
+// The accounting's LDS staging, one area for every instantiation of a
+// half-step in a kernel.
+struct BhAccuLds {
+    float red[BH_ACCU_CHUNK * WGS];
+    unsigned wave_counts[WGS / 64];
+};
+
+__device__ __forceinline__ BhAccuLds& bh_accu_lds() {
+    __shared__ BhAccuLds st;
+    return st;
+}
+
 #endif  // LOGFN || BH_GLM
 
 #ifdef LOGFN
@@ -330,6 +342,20 @@ constexpr int bh_stage_stride() {
 constexpr int kBhStageStride = bh_stage_stride();
 constexpr bool kBhStage = DIM >= 8 && WGS * kBhStageStride * 4 <= 49152;
 
+// One LDS staging area for every instantiation of the staged half-step in a
+// kernel (a __shared__ array declared in each instantiation would be
+// allocated once per instantiation, halving the blocks that fit on a CU).
+struct BhStageLds {
+    float tile[kBhStage ? WGS * kBhStageStride : 1];
+    unsigned pstart[WGS];
+    unsigned long long acc_mask[WGS / 64];
+};
+
+__device__ __forceinline__ BhStageLds& bh_stage_lds() {
+    __shared__ __attribute__((aligned(16))) BhStageLds st;
+    return st;
+}
+
 template <bool ACCU, int PARTNER>
 __device__ __forceinline__ void bh_stretch_half_staged(unsigned seed, unsigned odd_or_even,
                                                        unsigned data_len, unsigned params_len,
@@ -345,9 +371,10 @@ __device__ __forceinline__ void bh_stretch_half_staged(unsigned seed, unsigned o
     constexpr bool V4 = DIM % 4 == 0;                     // records as 16-byte pieces
     constexpr bool SV4 = V4 && PARTNER == BH_PARTNER_WALKER;  // partners too
     constexpr int NP = V4 ? DIM / 4 : DIM;                 // pieces per record
-    __shared__ __attribute__((aligned(16))) float tile[WGS * ST];
-    __shared__ unsigned pstart[WGS];
-    __shared__ unsigned long long acc_mask[WGS / 64];
+    BhStageLds& lds = bh_stage_lds();
+    float* tile = lds.tile;
+    unsigned* pstart = lds.pstart;
+    unsigned long long* acc_mask = lds.acc_mask;
     const unsigned t = threadIdx.x;
     const unsigned k = blockIdx.x * WGS + t;
     float* xb = X + (size_t)blockIdx.x * WGS * DIM;
@@ -457,9 +484,9 @@ __device__ __forceinline__ void bh_stretch_half_staged(unsigned seed, unsigned o
         }
     }
     if (ACCU) {
-        __shared__ float red[BH_ACCU_CHUNK * WGS];
-        __shared__ unsigned wave_counts[WGS / 64];
-        bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, red, wave_counts);
+        BhAccuLds& al = bh_accu_lds();
+        bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, al.red,
+                al.wave_counts);
     }
 }
 
@@ -489,9 +516,9 @@ __device__ __forceinline__ void bh_stretch_half(unsigned seed, unsigned odd_or_e
             __builtin_nontemporal_store(lx, logfn_X + k);
         }
         if (ACCU) {
-            __shared__ float red[BH_ACCU_CHUNK * WGS];
-            __shared__ unsigned wave_counts[WGS / 64];
-            bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, red, wave_counts);
+            BhAccuLds& al = bh_accu_lds();
+            bh_accu(xk, accepted, blockIdx.x, gridDim.x, step, accept, means, al.red,
+                    al.wave_counts);
         }
     }
 }

@@ -368,6 +368,22 @@ inline REAL mvn_logpdf(const uint data_len, const uint params_len, const REAL* p
 }
 """
 
+DIAG_GAUSS_SOURCE = """
+// Independent Gaussian coordinates up to a constant:
+// -sum_i w_i (x_i - mu_i)^2 / 2, params = [mu (dim), w (dim)], w_i = 1/sigma_i^2.
+inline REAL diag_gaussian_logpdf(const uint data_len, const uint params_len, const REAL* params,
+                                 const uint dim, const REAL* x) {
+    const REAL* mu = params;
+    const REAL* w = params + dim;
+    REAL q = 0.0f;
+    for (uint i = 0; i < dim; i++) {
+        const REAL d = x[i] - mu[i];
+        q = fma(w[i] * d, d, q);
+    }
+    return -0.5f * q;
+}
+"""
+
 LINREG_SOURCE = """
 // Gaussian linear regression, x = [beta (dim-1), log sigma];
 // data rows [x_i (dim-1), y_i], so rows = data_len / dim.
@@ -694,3 +710,10 @@ def hbb_model(max_groups: int = 50) -> DistributionModel:
 def mvn_model(dim: int) -> DistributionModel:
     return distribution_model(MVN_SOURCE, name="mvn", dimension=dim, params_size=dim + dim * dim,
                               limits=[[-FLT_MAX, FLT_MAX]] * dim)
+
+
+def diag_gaussian_model(dim: int) -> DistributionModel:
+    """Independent Gaussian coordinates (params: dim means, dim precisions):
+    a model whose density costs 2 DIM flops, for measuring the move itself."""
+    return distribution_model(DIAG_GAUSS_SOURCE, name="diag_gaussian", dimension=dim,
+                              params_size=2 * dim, limits=[[-FLT_MAX, FLT_MAX]] * dim)

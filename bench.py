@@ -29,10 +29,10 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
   mean, sd and 256-bin histogram of them.  One "step" is one whole job.
 
 Plain stretch move roofline (both workloads): the move-bare! kernel of the
-correlated Gaussian model at 2^24 walkers and DIM 4 and 16 (256 MiB and
-1 GiB of positions, above the 256 MB Infinity Cache), its launches timed
-with HIP events; algorithmic bytes per walker-step 8*DIM + 4 +
-alpha*(4*DIM + 4).
+correlated Gaussian model at 2^24 walkers and DIM 4 and 16, and of
+independent Gaussian coordinates at DIM 32 (256 MiB, 1 GiB and 2 GiB of
+positions, above the 256 MB Infinity Cache), its launches timed with HIP
+events; algorithmic bytes per walker-step 8*DIM + 4 + alpha*(4*DIM + 4).
 
 CPU baseline: the CPU restatement of the reference kernels (oracle/) on
 the host cores, OpenMP over walkers, on a bounded sample of the workload.
@@ -64,6 +64,10 @@ RUN_STEPS = 4096
 SAMPLE_COLS = 4096 * 1024
 ROOF_WALKERS = 1 << 24
 ROOF_STEPS = 20
+# (model, DIM): the correlated Gaussian at DIM 4 and 16, independent
+# coordinates at DIM 32 (a density of 2 DIM flops; the correlated one costs
+# DIM^2 fma, 1024 at DIM 32, and would time the model rather than the move)
+ROOF_DIMS = (("mvn", 4), ("mvn", 16), ("diag", 32))
 
 # logistic-regression workload
 C3_N, C3_D, C3_W = 1_000_000, 64, 16384
@@ -149,15 +153,29 @@ def traffic_of(name):
     return None
 
 
-def move_roofline(fac, dim, rates=None):
+def diag_problem(dim, seed):
+    rng = np.random.default_rng(seed)
+    mu = rng.standard_normal(dim)
+    w = 1.0 / rng.uniform(0.5, 2.0, dim) ** 2
+    params = np.concatenate([mu, w]).astype(np.float32)
+    limits = np.tile(np.array([-5.0, 5.0], np.float32), dim)
+    return params, limits
+
+
+def move_roofline(fac, dim, rates=None, model="mvn"):
     """HBM roofline of the move-bare! kernel at 2^24 walkers of a dim-D
-    correlated Gaussian, and its bound from random line gathers: every
-    walker reads one partner record from a random line of the other half,
-    so a launch needs K line gathers (at the device's measured random-gather
-    rate) plus its streamed bytes at the HBM peak."""
+    correlated Gaussian (model "mvn") or independent Gaussian coordinates
+    ("diag"), and its bound from random line gathers: every walker reads one
+    partner record from a random line of the other half, so a launch needs K
+    line gathers (at the device's measured random-gather rate) plus its
+    streamed bytes at the HBM peak."""
     from bayadera_amd import _lib, models
-    _, _, params, limits = mvn_problem(dim, 1001 + dim)
-    sf = fac.mcmc_factory(models.mvn_model(dim))
+    if model == "mvn":
+        _, _, params, limits = mvn_problem(dim, 1001 + dim)
+        sf = fac.mcmc_factory(models.mvn_model(dim))
+    else:
+        params, limits = diag_problem(dim, 2001 + dim)
+        sf = fac.mcmc_factory(models.diag_gaussian_model(dim))
     s = sf.create_sampler(77, ROOF_WALKERS, params)
     s.set_exec(_lib.BH_EXEC_STEPWISE).set_partner(_lib.BH_PARTNER_WALKER)
     s.init(77)
@@ -182,10 +200,13 @@ def move_roofline(fac, dim, rates=None):
                   "gathers_per_s_measured": round(rates["gathers_per_s"], 0),
                   "streamed_bytes_per_launch": int(streamed),
                   "bound_ms": round(bound_ms, 4), "frac_of_bound": round(bound_ms / avg_ms, 4)}
-    return {"kernel": f"bh_stretch_move_bare (DIM={dim}, K=2^23 walkers per launch)",
+    mdesc = "correlated Gaussian" if model == "mvn" else "independent Gaussian coordinates"
+    tname = (f"stretch_move_bare_dim{dim}_2p24_traffic.json" if model == "mvn"
+             else f"stretch_move_bare_{model}_dim{dim}_2p24_traffic.json")
+    return {"kernel": f"bh_stretch_move_bare (DIM={dim}, {mdesc}, K=2^23 walkers per launch)",
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_of(f"stretch_move_bare_dim{dim}_2p24_traffic.json"),
+            "traffic": traffic_of(tname),
             "avg_kernel_ms": round(avg_ms, 4), "launches_timed": kt["launches"],
             "alpha": round(alpha, 4),
             "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker),
@@ -296,7 +317,8 @@ def main_c1(args):
     if not args.no_roofline:
         rates = fac.device_rates()
         line["roofline"] = move_roofline(fac, DIM, rates)
-        line["plain_move_roofline_dim16"] = move_roofline(fac, 16, rates)
+        line["plain_move_roofline_more"] = [move_roofline(fac, d, rates, m)
+                                            for m, d in ROOF_DIMS[1:]]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(params, limits)
     print(json.dumps(line), flush=True)
@@ -918,7 +940,7 @@ def main_c3(args):
         "setup_s": round(t_setup, 1),
     }
     if not args.no_roofline:
-        line["plain_move_roofline"] = [move_roofline(fac, 4, rates), move_roofline(fac, 16, rates)]
+        line["plain_move_roofline"] = [move_roofline(fac, d, rates, m) for m, d in ROOF_DIMS]
     if not args.no_cpu_baseline:
         cb = c3_cpu_baseline(Xd, y, limits)
         ess_per_ws = out[mname]["ess_min"] / (C3_W * (burn + run))
@@ -976,7 +998,7 @@ def main():
         from bayadera_amd import engine
         fac = engine.HipBayaderaFactory(0, 256)
         rates = fac.device_rates()
-        r = [move_roofline(fac, 4, rates), move_roofline(fac, 16, rates)]
+        r = [move_roofline(fac, d, rates, m) for m, d in ROOF_DIMS]
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)

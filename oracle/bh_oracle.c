@@ -167,6 +167,7 @@ enum {
     BHO_BETA_BINOMIAL_POST = 6,    /* dist/posterior.cl:1-6 with binomial_loglik + beta_logpdf */
     BHO_MVN_LOGPDF = 7,            /* correlated Gaussian -(x-mu)'P(x-mu)/2 (4-D benchmark model) */
     BHO_GAUSSIAN_LOGLIK = 8,       /* dist/gaussian.cl:29-42 */
+    BHO_DIAG_GAUSS_LOGPDF = 9,     /* independent Gaussian coordinates -sum w_i (x_i-mu_i)^2/2 (benchmark model) */
 };
 
 static const float LOG_SQRT_2PI_F = 0.9189385332046727f;
@@ -220,6 +221,17 @@ BHO_EXPORT float bho_model_logfn(int model, uint32_t data_len, uint32_t params_l
             float row = 0.0f;
             for (uint32_t c = 0; c < dim; ++c) row = fmaf(P[r * dim + c], x[c] - mu[c], row);
             q = fmaf(x[r] - mu[r], row, q);
+        }
+        return -0.5f * q;
+    }
+    case BHO_DIAG_GAUSS_LOGPDF: {
+        /* params = [mu (dim), w (dim)], w_i = 1 / sigma_i^2 */
+        const float* mu = params;
+        const float* w = params + dim;
+        float q = 0.0f;
+        for (uint32_t i = 0; i < dim; ++i) {
+            const float d = x[i] - mu[i];
+            q = fmaf(w[i] * d, d, q);
         }
         return -0.5f * q;
     }

@@ -200,6 +200,33 @@ def test_mvn_steps_vs_oracle(fac, partner, dim):
     assert np.all(close[np.all(x == r, axis=1)])
 
 
+# DIM 32: records of eight 16-byte pieces; DIM 33: 33 float pieces
+@pytest.mark.parametrize("dim", [32, 33])
+@pytest.mark.parametrize("partner", [O.PARTNER_FLAT, O.PARTNER_WALKER])
+def test_diag_gaussian_steps_vs_oracle(fac, partner, dim):
+    rng = np.random.default_rng(dim)
+    mu = rng.standard_normal(dim).astype(np.float32)
+    w = rng.uniform(0.25, 4.0, dim).astype(np.float32)
+    params = np.concatenate([mu, w])
+    sf = fac.mcmc_factory(models.diag_gaussian_model(dim))
+    s = sf.create_sampler(9, 2048, params).set_partner(partner).set_exec(STEPWISE)
+    s.init(9)
+    s.init_position(10, np.tile([-3.0, 3.0], dim))
+    ref = O.Stretch(O.DIAG_GAUSS_LOGPDF, dim, 2048, params, 9, params_len=2 * dim,
+                    partner=partner, threads=4)
+    ref.init(9)
+    ref.init_position(10, np.tile(np.array([[-3.0, 3.0]], np.float32), (dim, 1)))
+    s.burn_in(6, 2.0)
+    ref.burn_in(6, 2.0)
+    x = s.state()[0].reshape(-1, dim)
+    r = ref.xs.reshape(-1, dim)
+    same = np.all(x == r, axis=1).mean()
+    assert same > 0.995, same
+    r_run = s.run_sampler(64, 2.0)
+    ref_run = ref.run_sampler(64, 2.0)
+    assert abs(r_run["acceptance-rate"] - ref_run["acceptance_rate"]) < 0.01
+
+
 def test_anneal_vs_oracle(gaussian_sf):
     sched = lambda i: 10.0 - i  # noqa: E731  (mcmc.clj minus-n)
     s = gaussian_sf.create_sampler(11, 1024, [0.0, 1.0])

@@ -129,3 +129,23 @@ def test_mvn_closed_form_posterior_walker_mode():
     sd = np.sqrt(np.diag(Sigma))
     assert np.all(np.abs(draws.mean(0) - mu) < 0.25 * sd)
     assert np.all(np.abs(draws.std(0) / sd - 1) < 0.15)
+
+
+def test_diag_gaussian_model_closed_form():
+    """Independent Gaussian coordinates: log density 0 at the mean, -1/2 one
+    standard deviation away along any axis, and -sum (x-mu)^2/(2 sigma^2)
+    at random points."""
+    dim = 32
+    rng = np.random.default_rng(7)
+    mu = rng.standard_normal(dim).astype(np.float32)
+    sd = rng.uniform(0.5, 2.0, dim).astype(np.float32)
+    w = (1.0 / (sd.astype(np.float64) ** 2)).astype(np.float32)
+    params = np.concatenate([mu, w])
+    assert O.logfn_many(O.DIAG_GAUSS_LOGPDF, params, mu, dim)[0] == 0.0
+    axes = np.tile(mu, (dim, 1)) + np.diag(sd)
+    lp = O.logfn_many(O.DIAG_GAUSS_LOGPDF, params, axes, dim)
+    assert np.allclose(lp, -0.5, rtol=1e-5)
+    x = (mu + 3 * rng.standard_normal((100, dim))).astype(np.float32)
+    want = -0.5 * np.sum(w.astype(np.float64) * (x.astype(np.float64) - mu) ** 2, axis=1)
+    got = O.logfn_many(O.DIAG_GAUSS_LOGPDF, params, x, dim)
+    assert np.allclose(got, want, rtol=1e-5)
