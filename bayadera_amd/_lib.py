@@ -22,6 +22,8 @@ BH_ERR_STATE = -5
 
 BH_PARTNER_FLAT = 0
 BH_PARTNER_WALKER = 1
+BH_BATCH_STATE_LDS = 0
+BH_BATCH_STATE_GLOBAL = 1
 BH_EXEC_AUTO = 0
 BH_EXEC_STEPWISE = 1
 BH_EXEC_RESIDENT = 2
@@ -104,6 +106,8 @@ SIGNATURES = {
     "bh_batch_burn_in": [_P, _U32, _U64, _F],
     "bh_batch_run": [_P, _U32, _U64, _F, _P, _P, _P],
     "bh_batch_autotune": [_P, _U32, _P, _P, _P],
+    "bh_batch_set_state": [_P, _I],
+    "bh_batch_class_states": [_P, _P, _P],
     "bh_batch_run_series": [_P, _U32, _U64, _F, _P, _P, _P, _P],
     "bh_glm_batch_create": [_P, _I, _U32, _P, _P, _P, _P, _P, _F, _U32, _PP],
     "bh_glm_batch_destroy": [_P],

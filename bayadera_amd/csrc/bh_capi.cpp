@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <array>
 #include <vector>
 
 #include "../../include/bayadera_hip.h"
@@ -315,13 +316,16 @@ struct bh_batch {
     bh_ctx* ctx = nullptr;
     hipStream_t stream = nullptr;
     Module mod;
-    hipFunction_t run = nullptr, init = nullptr, logfn = nullptr;
+    hipFunction_t run = nullptr, run_g = nullptr, init = nullptr, logfn = nullptr;
     uint32_t max_dim = 0, problems = 0, W = 0, nb = 0, dim_hi = 0;
+    int lds_max = 0;
     // launch classes: problems order[begin .. begin + count) with dimensions
     // up to dim_hi, each launched with the LDS its largest problem needs
     struct Class {
         uint32_t begin, count, dim_hi;
+        int gstate;  // BH_BATCH_STATE_LDS or BH_BATCH_STATE_GLOBAL
     };
+    int gstate = BH_BATCH_STATE_LDS;  // placement of new classes
     std::vector<Class> classes;
     std::vector<uint32_t> dims_h;
     DevBuf order;
@@ -1515,12 +1519,14 @@ int bh_stretch_glm_guard_stats(bh_stretch* s, uint64_t* listed, uint64_t* overfl
 }
 
 // ------------------------------------------------------------------ batch
-static size_t batch_lds(const bh_batch* b, uint32_t dim_hi, uint32_t nb) {
-    return (size_t)2 * b->max_dim * sizeof(double) + (size_t)8 * nb * sizeof(float) +
-           ((size_t)b->W * dim_hi + b->W + (size_t)nb * dim_hi) * sizeof(float);
+// LDS of one work-group of bh_batch_run: the accounting (2 max_dim doubles,
+// 2 nb floats), the proposals (nb x dim_hi floats) and, with the state in
+// LDS, the positions and log densities (W x dim_hi + W floats).
+static size_t batch_lds(const bh_batch* b, uint32_t dim_hi, uint32_t nb, int gstate) {
+    size_t floats = (size_t)2 * nb + (size_t)nb * dim_hi;
+    if (gstate == BH_BATCH_STATE_LDS) floats += (size_t)b->W * dim_hi + b->W;
+    return (size_t)2 * b->max_dim * sizeof(double) + floats * sizeof(float);
 }
-
-static size_t batch_lds(const bh_batch* b) { return batch_lds(b, b->dim_hi, b->nb); }
 
 // Launch classes from dimension bounds (ascending; the last >= every dim):
 // problems sorted by dimension, one class per bound that holds any.
@@ -1535,7 +1541,7 @@ static int batch_set_classes(bh_batch* b, const std::vector<uint32_t>& bounds) {
         const uint32_t begin = i;
         uint32_t hi = 0;
         while (i < b->problems && b->dims_h[ord[i]] <= bound) hi = std::max(hi, b->dims_h[ord[i++]]);
-        if (i > begin) b->classes.push_back({begin, i - begin, hi});
+        if (i > begin) b->classes.push_back({begin, i - begin, hi, b->gstate});
     }
     if (i != b->problems) return fail(BH_ERR_ARG, "class bounds do not cover every dimension");
     BH_HIP(hipMemcpyAsync(b->order.p, ord.data(), ord.size() * 4, hipMemcpyHostToDevice,
@@ -1577,9 +1583,15 @@ int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uin
     }
     int lds_max = 0;
     BH_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-    if (batch_lds(b.get()) > (size_t)lds_max)
+    // the state goes to LDS when it fits there, else it stays in global
+    // memory (the proposals alone must fit)
+    if (batch_lds(b.get(), b->dim_hi, b->nb, BH_BATCH_STATE_LDS) > (size_t)lds_max)
+        b->gstate = BH_BATCH_STATE_GLOBAL;
+    if (batch_lds(b.get(), b->dim_hi, b->nb, BH_BATCH_STATE_GLOBAL) > (size_t)lds_max)
         return fail(BH_ERR_RANGE, "%u walkers x %u dims need %zu bytes of LDS per problem (%d available)",
-                    walkers, b->dim_hi, batch_lds(b.get()), lds_max);
+                    walkers, b->dim_hi,
+                    batch_lds(b.get(), b->dim_hi, b->nb, BH_BATCH_STATE_GLOBAL), lds_max);
+    b->lds_max = lds_max;
     std::vector<char> code;
     BH_TRY(rtc_compile(ctx->arch, source,
                        {"-DWGS=" + std::to_string(ctx->wgs), "-DDIM=" + std::to_string(max_dim),
@@ -1587,6 +1599,7 @@ int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uin
                        code, log, log_size));
     BH_TRY(b->mod.load(code));
     BH_TRY(b->mod.fn("bh_batch_run", &b->run));
+    BH_TRY(b->mod.fn("bh_batch_run_g", &b->run_g));
     BH_TRY(b->mod.fn("bh_batch_init", &b->init));
     BH_TRY(b->mod.fn("bh_batch_logfn", &b->logfn));
     BH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -1663,7 +1676,8 @@ int bh_batch_init_position(bh_batch* b, uint32_t seed, const float* limits) {
 }
 
 static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mode,
-                       const unsigned long long* soff = nullptr, float* series = nullptr) {
+                       const unsigned long long* soff = nullptr, float* series = nullptr,
+                       std::vector<hipEvent_t>* class_events = nullptr) {
     BH_HIP(hipSetDevice(b->ctx->device));
     const unsigned* dims = b->dims.as<unsigned>();
     const unsigned long long* poff = b->poff.as<unsigned long long>();
@@ -1683,15 +1697,19 @@ static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mod
         unsigned chunk = (unsigned)std::min<uint64_t>(n - done, 1u << 20);
         float* ser = series;  // recorded runs are a single launch (n < 2^20)
         const unsigned* ord = b->order.as<unsigned>();
-        for (const auto& cl : b->classes) {
+        for (size_t c = 0; c < b->classes.size(); ++c) {
+            const auto& cl = b->classes[c];
             unsigned base = cl.begin;
             void* args[] = {&seed0, &dims, &poff, &dlen, &plen, &params, &xs, &lf, &W, &a,
                             &step0, &chunk, &partner, &mode, &acc, &sums, &soff, &ser, &ord,
                             &base};
-            BH_HIP(hipModuleLaunchKernel(b->run, cl.count, 1, 1, b->nb, 1, 1,
-                                         (unsigned)batch_lds(b, cl.dim_hi, b->nb), b->stream,
-                                         args, nullptr));
+            if (class_events) BH_HIP(hipEventRecord((*class_events)[c], b->stream));
+            BH_HIP(hipModuleLaunchKernel(cl.gstate == BH_BATCH_STATE_GLOBAL ? b->run_g : b->run,
+                                         cl.count, 1, 1, b->nb, 1, 1,
+                                         (unsigned)batch_lds(b, cl.dim_hi, b->nb, cl.gstate),
+                                         b->stream, args, nullptr));
         }
+        if (class_events) BH_HIP(hipEventRecord(class_events->back(), b->stream));
         b->bare_counter += chunk;
         done += chunk;
     }
@@ -1703,14 +1721,24 @@ static int batch_steps(bh_batch* b, uint32_t seed0, uint64_t n, float a, int mod
     return BH_OK;
 }
 
+int bh_batch_set_state(bh_batch* b, int where) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(where == BH_BATCH_STATE_LDS || where == BH_BATCH_STATE_GLOBAL,
+           "unknown state placement %d", where);
+    for (const auto& cl : b->classes)
+        if (batch_lds(b, cl.dim_hi, b->nb, where) > (size_t)b->lds_max)
+            return fail(BH_ERR_RANGE, "%u walkers x %u dims do not fit in %d bytes of LDS", b->W,
+                        cl.dim_hi, b->lds_max);
+    b->gstate = where;
+    for (auto& cl : b->classes) cl.gstate = where;
+    return BH_OK;
+}
+
 int bh_batch_autotune(bh_batch* b, uint32_t steps, uint32_t* block_out, uint32_t* classes_out,
                       double* ms_out) {
     BH_ARG(b, "NULL batch");
     BH_ARG(steps >= 1 && steps <= 10000, "bad step count");
     BH_HIP(hipSetDevice(b->ctx->device));
-    int lds_max = 0;
-    BH_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                 b->ctx->device));
     // the state every candidate starts from, restored afterwards
     const size_t xbytes = (size_t)b->problems * b->W * b->max_dim * sizeof(float);
     const size_t lbytes = (size_t)b->problems * b->W * sizeof(float);
@@ -1730,32 +1758,81 @@ int bh_batch_autotune(bh_batch* b, uint32_t steps, uint32_t* block_out, uint32_t
         bs.erase(std::unique(bs.begin(), bs.end()), bs.end());
         bound_sets.push_back(bs);
     }
+    // one event before each class launch and one after the last
+    struct Events {
+        std::vector<hipEvent_t> ev;
+        ~Events() {
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        }
+    } events;
+    for (int i = 0; i < 5; ++i) {
+        hipEvent_t e;
+        BH_HIP(hipEventCreate(&e));
+        events.ev.push_back(e);
+    }
     const uint32_t K = b->W / 2;
+    const int gstate0 = b->gstate;
     double best = -1.0;
     uint32_t best_nb = b->nb;
     std::vector<uint32_t> best_bounds = {b->dim_hi};
+    std::vector<int> best_states;
     for (uint32_t nb = 64; nb <= 1024 && nb <= std::max(64u, K); nb *= 2) {
         for (const auto& bs : bound_sets) {
             b->nb = nb;
             BH_TRY(batch_set_classes(b, bs));
-            bool fits = true;
-            for (const auto& cl : b->classes)
-                fits = fits && batch_lds(b, cl.dim_hi, nb) <= (size_t)lds_max;
-            if (!fits) continue;
-            BH_HIP(hipMemcpyAsync(b->xs.p, xs0.p, xbytes, hipMemcpyDeviceToDevice, b->stream));
-            BH_HIP(hipMemcpyAsync(b->logfn_buf.p, lf0.p, lbytes, hipMemcpyDeviceToDevice,
-                                  b->stream));
-            b->bare_counter = counter0;
-            BH_TRY(batch_steps(b, 0x5eedu, steps, 2.0f, 0));
-            if (best < 0.0 || b->last_ms < best) {
-                best = b->last_ms;
+            const size_t nc = b->classes.size();
+            // per class, the time of each placement that fits (-1: does not)
+            std::vector<std::array<double, 2>> cms(nc, {-1.0, -1.0});
+            for (int where = BH_BATCH_STATE_LDS; where <= BH_BATCH_STATE_GLOBAL; ++where) {
+                bool any = false;
+                for (auto& cl : b->classes) {
+                    cl.gstate = batch_lds(b, cl.dim_hi, nb, where) <= (size_t)b->lds_max
+                                    ? where
+                                    : BH_BATCH_STATE_GLOBAL;
+                    any = any || cl.gstate == where;
+                }
+                if (!any) continue;
+                if (batch_lds(b, b->classes.back().dim_hi, nb, BH_BATCH_STATE_GLOBAL) >
+                    (size_t)b->lds_max)
+                    break;
+                BH_HIP(hipMemcpyAsync(b->xs.p, xs0.p, xbytes, hipMemcpyDeviceToDevice,
+                                      b->stream));
+                BH_HIP(hipMemcpyAsync(b->logfn_buf.p, lf0.p, lbytes, hipMemcpyDeviceToDevice,
+                                      b->stream));
+                b->bare_counter = counter0;
+                std::vector<hipEvent_t> ce(events.ev.begin(), events.ev.begin() + nc + 1);
+                BH_TRY(batch_steps(b, 0x5eedu, steps, 2.0f, 0, nullptr, nullptr, &ce));
+                for (size_t c = 0; c < nc; ++c) {
+                    if (b->classes[c].gstate != where) continue;
+                    float ms = 0.0f;
+                    BH_HIP(hipEventElapsedTime(&ms, ce[c], ce[c + 1]));
+                    cms[c][where] = ms;
+                }
+            }
+            double total = 0.0;
+            std::vector<int> states(nc);
+            bool ok = true;
+            for (size_t c = 0; c < nc; ++c) {
+                const double l = cms[c][0], g = cms[c][1];
+                if (l < 0.0 && g < 0.0) ok = false;
+                states[c] = (g >= 0.0 && (l < 0.0 || g < l)) ? BH_BATCH_STATE_GLOBAL
+                                                            : BH_BATCH_STATE_LDS;
+                total += states[c] == BH_BATCH_STATE_GLOBAL ? g : l;
+            }
+            if (!ok) continue;
+            if (best < 0.0 || total < best) {
+                best = total;
                 best_nb = nb;
                 best_bounds = bs;
+                best_states = states;
             }
         }
     }
     b->nb = best_nb;
+    b->gstate = gstate0;
     BH_TRY(batch_set_classes(b, best_bounds));
+    for (size_t c = 0; c < best_states.size() && c < b->classes.size(); ++c)
+        b->classes[c].gstate = best_states[c];
     BH_HIP(hipMemcpyAsync(b->xs.p, xs0.p, xbytes, hipMemcpyDeviceToDevice, b->stream));
     BH_HIP(hipMemcpyAsync(b->logfn_buf.p, lf0.p, lbytes, hipMemcpyDeviceToDevice, b->stream));
     b->bare_counter = counter0;
@@ -1763,6 +1840,14 @@ int bh_batch_autotune(bh_batch* b, uint32_t steps, uint32_t* block_out, uint32_t
     if (block_out) *block_out = b->nb;
     if (classes_out) *classes_out = (uint32_t)b->classes.size();
     if (ms_out) *ms_out = best;
+    return BH_OK;
+}
+
+int bh_batch_class_states(const bh_batch* b, uint32_t* count, int* states) {
+    BH_ARG(b && count, "NULL argument");
+    if (states)
+        for (size_t c = 0; c < b->classes.size() && c < *count; ++c) states[c] = b->classes[c].gstate;
+    *count = (uint32_t)b->classes.size();
     return BH_OK;
 }
 

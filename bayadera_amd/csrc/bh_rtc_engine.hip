@@ -652,29 +652,37 @@ bh_stretch_resident(unsigned seed_odd, unsigned seed_even, unsigned c_odd, unsig
 
 #ifdef BH_BATCH
 // Many independent posteriors of one model source in one launch: one
-// work-group per problem, its W walkers resident in LDS for all steps.  A
-// problem b with dimension dims[b] <= DIM and params params + poff[b] runs
-// exactly the single-sampler algorithm (move-bare!, constants 3333/4444)
-// with bare seed seed0 + 2b: the same streams, moves and accept tests as a
-// sampler created for that problem alone.  Positions are packed with the
-// problem's own dimension; a proposal is built in the thread's LDS slot,
-// where the model reads it.  Mode 1 also accumulates per problem, in double
-// and in a fixed order, the sums over steps of the ensemble sums of x and
-// x^2 (each step's ensemble sums are float tree sums over the walkers); with
-// series non-null it also writes each step's ensemble mean (that float sum
-// over W), problem b's n x dims[b] block starting at series + soff[b].
-// Work-group i of a launch runs problem order[base + i] (problems are
-// launched in classes of similar dimension, each with the LDS it needs).
-// LDS: W*dim + W + nb*dim floats + 2*DIM doubles + 8*nb floats.
-extern "C" __global__ void __launch_bounds__(1024)
-bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
-             const unsigned long long* __restrict__ poff, const unsigned* __restrict__ dlen,
-             const unsigned* __restrict__ plen, const float* __restrict__ params,
-             float* __restrict__ xs, float* __restrict__ logfn, unsigned W, float a,
-             unsigned step0, unsigned nsteps, int partner, int mode,
-             unsigned long long* __restrict__ accept, double* __restrict__ sums,
-             const unsigned long long* __restrict__ soff, float* __restrict__ series,
-             const unsigned* __restrict__ order, unsigned base) {
+// work-group per problem, its W walkers advanced for all steps of the call
+// inside the work-group.  A problem b with dimension dims[b] <= DIM and
+// params params + poff[b] runs exactly the single-sampler algorithm
+// (move-bare!, constants 3333/4444) with bare seed seed0 + 2b: the same
+// streams, moves and accept tests as a sampler created for that problem
+// alone.  Positions are packed with the problem's own dimension; a proposal
+// is built in the thread's LDS slot, where the model reads it.
+// State placement (gstate): 0 copies the problem's positions and log
+// densities into LDS for the call (partner reads from LDS; the LDS a
+// problem needs bounds how many share a CU), 1 works on them in place in
+// the problem's global slot (L2-resident; many more problems per CU).
+// Mode 1 also accumulates per problem, in double, the sums over steps of the
+// ensemble sums of x and x^2; each step's ensemble sum of a coordinate is a
+// float sum in a fixed order (threads own (coordinate, walker residue)
+// pairs, P = nb / dim residues: a serial sum over walkers p, p + P, ..., then
+// the P partial sums in order).  With series non-null it also writes each
+// step's ensemble mean (that sum over W), problem b's n x dims[b] block
+// starting at series + soff[b].  Work-group i of a launch runs problem
+// order[base + i] (problems are launched in classes of similar dimension,
+// each with the LDS it needs).
+// LDS: 2*DIM doubles + 2*nb floats + nb*dim floats (+ W*dim + W floats for
+// gstate 0).
+template <bool GSTATE>
+__device__ __forceinline__ void bh_batch_impl(
+    unsigned seed0, const unsigned* __restrict__ dims, const unsigned long long* __restrict__ poff,
+    const unsigned* __restrict__ dlen, const unsigned* __restrict__ plen,
+    const float* __restrict__ params, float* __restrict__ xs, float* __restrict__ logfn,
+    unsigned W, float a, unsigned step0, unsigned nsteps, int partner, int mode,
+    unsigned long long* __restrict__ accept, double* __restrict__ sums,
+    const unsigned long long* __restrict__ soff, float* __restrict__ series,
+    const unsigned* __restrict__ order, unsigned base) {
     extern __shared__ double lds_d[];
     const unsigned b = order[base + blockIdx.x];
     const unsigned dim = dims[b];
@@ -686,12 +694,21 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
     float* gl = logfn + (size_t)b * W;
     double* acc1 = lds_d;                  // DIM
     double* acc2 = acc1 + DIM;             // DIM
-    float* red = reinterpret_cast<float*>(acc2 + DIM);  // 8 * nb
-    float* sx = red + 8 * nb;              // W * dim
-    float* sl = sx + (size_t)W * dim;      // W
-    float* yb = sl + W;                    // nb * dim
-    for (unsigned i = t; i < W * dim; i += nb) sx[i] = gx[i];
-    for (unsigned i = t; i < W; i += nb) sl[i] = gl[i];
+    float* red = reinterpret_cast<float*>(acc2 + DIM);  // 2 * nb
+    float* sx;                             // W * dim
+    float* sl;                             // W
+    float* yb;                             // nb * dim
+    if constexpr (GSTATE) {
+        sx = gx;
+        sl = gl;
+        yb = red + 2 * nb;
+    } else {
+        sx = red + 2 * nb;
+        sl = sx + (size_t)W * dim;
+        yb = sl + W;
+        for (unsigned i = t; i < W * dim; i += nb) sx[i] = gx[i];
+        for (unsigned i = t; i < W; i += nb) sl[i] = gl[i];
+    }
     for (unsigned i = t; i < 2 * DIM; i += nb) acc1[i] = 0.0;
     __syncthreads();
     const unsigned seed = seed0 + 2u * b;
@@ -700,6 +717,7 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
     const float ca = a - 2.0f + inv_a;
     const float cb = 2.0f * (1.0f - inv_a);
     float* y = yb + (size_t)t * dim;
+    const unsigned P = dim <= nb ? nb / dim : 1u;
     for (unsigned st = 0; st < nsteps; ++st) {
         const unsigned step = step0 + st;
         for (int h = 0; h < 2; ++h) {
@@ -712,22 +730,37 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
                     BH_KEY1_STRETCH));
                 const float z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
                 const float* xk = X + (size_t)k * dim;
+                // partner element i at S[(e + i) mod K*dim] (FLAT) or
+                // S[j*dim + i] (WALKER); eight coordinates' loads in flight
+                const unsigned total = K * dim;
+                unsigned e;
                 if (partner == BH_PARTNER_FLAT) {
-                    const unsigned total = K * dim;
-                    unsigned e = (unsigned)((u.x * (float)K) * (float)dim);
+                    e = (unsigned)((u.x * (float)K) * (float)dim);
                     if (e >= total) e -= total;
-                    for (unsigned i = 0; i < dim; ++i) {
-                        const float xj = S[e];
-                        y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
-                        e = (e + 1 == total) ? 0u : e + 1;
-                    }
                 } else {
                     unsigned j = (unsigned)(u.x * (float)K);
                     j = j < K ? j : K - 1;
-                    for (unsigned i = 0; i < dim; ++i) {
-                        const float xj = S[(size_t)j * dim + i];
-                        y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+                    e = j * dim;  // e + i < total for i < dim
+                }
+                unsigned i = 0;
+                for (; i + 8 <= dim; i += 8) {
+                    float xa[8], sa[8];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        unsigned ec = e + c;
+                        if (ec >= total) ec -= total;
+                        xa[c] = xk[i + c];
+                        sa[c] = S[ec];
                     }
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) y[i + c] = __builtin_fmaf(z, xa[c] - sa[c], sa[c]);
+                    e += 8;
+                    if (e >= total) e -= total;
+                }
+                for (; i < dim; ++i) {
+                    const float xj = S[e];
+                    y[i] = __builtin_fmaf(z, xk[i] - xj, xj);
+                    e = (e + 1 == total) ? 0u : e + 1;
                 }
                 const float ly = LOGFN(data_len, params_len, prm, dim, y);
                 float zp = 1.0f;
@@ -742,40 +775,62 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
             __syncthreads();
         }
         if (mode == 1) {
-            for (unsigned d0 = 0; d0 < dim; d0 += 4) {
-                float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                for (unsigned w = t; w < W; w += nb) {
+            if (dim <= nb) {
+                if (t < P * dim) {
+                    const unsigned d = t % dim, p = t / dim;
+                    float s1 = 0.0f, s2 = 0.0f;
+                    unsigned w = p;
+                    // walkers p, p + P, ... in order, eight loads in flight
+                    for (; w + 7 * P < W; w += 8 * P) {
+                        float xv[8];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        if (d0 + c < dim) {
-                            const float x = sx[(size_t)w * dim + d0 + c];
-                            v[c] += x;
-                            v[4 + c] = __builtin_fmaf(x, x, v[4 + c]);
+                        for (int c = 0; c < 8; ++c) xv[c] = sx[(size_t)(w + c * P) * dim + d];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            s1 += xv[c];
+                            s2 = __builtin_fmaf(xv[c], xv[c], s2);
                         }
                     }
-                }
-#pragma unroll
-                for (int c = 0; c < 8; ++c) red[c * nb + t] = v[c];
-                __syncthreads();
-                for (unsigned r = nb / 2; r >= 1; r >>= 1) {
-                    if (t < r) {
-#pragma unroll
-                        for (int c = 0; c < 8; ++c) red[c * nb + t] += red[c * nb + t + r];
+                    for (; w < W; w += P) {
+                        const float x = sx[(size_t)w * dim + d];
+                        s1 += x;
+                        s2 = __builtin_fmaf(x, x, s2);
                     }
-                    __syncthreads();
-                }
-                if (t < 4 && d0 + t < dim) {
-                    acc1[d0 + t] += (double)red[t * nb];
-                    acc2[d0 + t] += (double)red[(4 + t) * nb];
-                    if (series)
-                        series[soff[b] + (size_t)st * dim + d0 + t] = red[t * nb] / (float)W;
+                    red[t] = s1;
+                    red[nb + t] = s2;
                 }
                 __syncthreads();
+                if (t < dim) {
+                    float s1 = 0.0f, s2 = 0.0f;
+                    for (unsigned p = 0; p < P; ++p) {
+                        s1 += red[p * dim + t];
+                        s2 += red[nb + p * dim + t];
+                    }
+                    acc1[t] += (double)s1;
+                    acc2[t] += (double)s2;
+                    if (series) series[soff[b] + (size_t)st * dim + t] = s1 / (float)W;
+                }
+                // red is written again only after the next step's two
+                // half-step barriers
+            } else {
+                for (unsigned d = t; d < dim; d += nb) {
+                    float s1 = 0.0f, s2 = 0.0f;
+                    for (unsigned w = 0; w < W; ++w) {
+                        const float x = sx[(size_t)w * dim + d];
+                        s1 += x;
+                        s2 = __builtin_fmaf(x, x, s2);
+                    }
+                    acc1[d] += (double)s1;
+                    acc2[d] += (double)s2;
+                    if (series) series[soff[b] + (size_t)st * dim + d] = s1 / (float)W;
+                }
             }
         }
     }
-    for (unsigned i = t; i < W * dim; i += nb) gx[i] = sx[i];
-    for (unsigned i = t; i < W; i += nb) gl[i] = sl[i];
+    if constexpr (!GSTATE) {
+        for (unsigned i = t; i < W * dim; i += nb) gx[i] = sx[i];
+        for (unsigned i = t; i < W; i += nb) gl[i] = sl[i];
+    }
     // accept count: block reduce in the (now free) staging area
     __syncthreads();
     unsigned long long* ra = reinterpret_cast<unsigned long long*>(red);
@@ -793,6 +848,24 @@ bh_batch_run(unsigned seed0, const unsigned* __restrict__ dims,
         }
     }
 }
+
+// State in LDS (bh_batch_run) or in place in global memory (bh_batch_run_g):
+// two kernels, so that each has the registers of one placement.
+#define BH_BATCH_RUN_KERNEL(NAME, GSTATE)                                                        \
+    extern "C" __global__ void __launch_bounds__(1024) NAME(                                     \
+        unsigned seed0, const unsigned* __restrict__ dims,                                        \
+        const unsigned long long* __restrict__ poff, const unsigned* __restrict__ dlen,           \
+        const unsigned* __restrict__ plen, const float* __restrict__ params,                      \
+        float* __restrict__ xs, float* __restrict__ logfn, unsigned W, float a, unsigned step0,   \
+        unsigned nsteps, int partner, int mode, unsigned long long* __restrict__ accept,          \
+        double* __restrict__ sums, const unsigned long long* __restrict__ soff,                   \
+        float* __restrict__ series, const unsigned* __restrict__ order, unsigned base) {          \
+        bh_batch_impl<GSTATE>(seed0, dims, poff, dlen, plen, params, xs, logfn, W, a, step0,      \
+                              nsteps, partner, mode, accept, sums, soff, series, order, base);    \
+    }
+BH_BATCH_RUN_KERNEL(bh_batch_run, false)
+BH_BATCH_RUN_KERNEL(bh_batch_run_g, true)
+#undef BH_BATCH_RUN_KERNEL
 
 // Initial positions and log-densities of every problem: init_walkers'
 // stream (key {seed0 + b, 0xdecafaaa}) over the problem's W*dim values,

@@ -407,13 +407,28 @@ class HipBatch(_Handle):
         call("bh_batch_burn_in", self._h, int(seed0) & 0xFFFFFFFF, int(n), float(a))
         return self
 
+    def set_state(self, where: int):
+        """Walkers of every launch class in LDS (_lib.BH_BATCH_STATE_LDS) or
+        in place in global memory (_lib.BH_BATCH_STATE_GLOBAL) during a call."""
+        call("bh_batch_set_state", self._h, int(where))
+        return self
+
+    def class_states(self) -> list:
+        n = ctypes.c_uint32(0)
+        call("bh_batch_class_states", self._h, ctypes.byref(n), None)
+        st = (ctypes.c_int * max(1, n.value))()
+        call("bh_batch_class_states", self._h, ctypes.byref(n), st)
+        return [int(st[i]) for i in range(n.value)]
+
     def autotune(self, steps: int = 20) -> dict:
         """Times the candidate launch shapes from the current state and keeps
         the fastest (the state is restored)."""
         nb, nc, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
         call("bh_batch_autotune", self._h, int(steps), ctypes.byref(nb), ctypes.byref(nc),
              ctypes.byref(ms))
-        return {"threads_per_problem": nb.value, "launch_classes": nc.value, "ms": ms.value}
+        return {"threads_per_problem": nb.value, "launch_classes": nc.value, "ms": ms.value,
+                "class_states": ["global" if s == _lib.BH_BATCH_STATE_GLOBAL else "lds"
+                                 for s in self.class_states()]}
 
     def run(self, seed0: int, n: int, a: float, series=None) -> dict:
         """n accounted steps.  ``series``: optional (device or host) float

@@ -153,6 +153,34 @@ def traffic_of(name):
     return None
 
 
+def hbm_copy_gbps(mib=1024, reps=10):
+    """HBM rate of a plain device copy (torch's copy kernel, mib MiB each
+    way), read + write bytes over the copy time: the measured denominator
+    beside the published 8 TB/s."""
+    import torch
+    n = mib << 18
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def device_rates(fac):
+    rates = fac.device_rates()
+    rates["hbm_copy_gbps"] = hbm_copy_gbps()
+    return rates
+
+
 def diag_problem(dim, seed):
     rng = np.random.default_rng(seed)
     mu = rng.standard_normal(dim)
@@ -210,6 +238,9 @@ def move_roofline(fac, dim, rates=None, model="mvn"):
             "avg_kernel_ms": round(avg_ms, 4), "launches_timed": kt["launches"],
             "alpha": round(alpha, 4),
             "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker),
+            "hbm_copy_measured_gbps": round(rates["hbm_copy_gbps"], 1) if rates else None,
+            "frac_of_measured_copy": (round(achieved / rates["hbm_copy_gbps"], 4)
+                                      if rates else None),
             "gather_bound": gather}
 
 
@@ -315,7 +346,7 @@ def main_c1(args):
                      for k in ("t_burn", "t_run", "t_sample")},
     }
     if not args.no_roofline:
-        rates = fac.device_rates()
+        rates = device_rates(fac)
         line["roofline"] = move_roofline(fac, DIM, rates)
         line["plain_move_roofline_more"] = [move_roofline(fac, d, rates, m)
                                             for m, d in ROOF_DIMS[1:]]
@@ -903,7 +934,7 @@ def main_c3(args):
     # msplit times, and one v_exp_f32 per contraction value plus one
     # v_log_f32 per sixteen in the epilogue (products and epilogue issue from
     # the same SIMDs and largely serialize)
-    rates = fac.device_rates()
+    rates = device_rates(fac)
     issued = flops * msplit / (kt["avg_ms"] * 1e-3)
     trans = C3_N * (C3_W // 2) * (1.0 + 1.0 / 16.0)
     trans_rate = trans / (kt["avg_ms"] * 1e-3)
@@ -997,7 +1028,7 @@ def main():
         import torch
         from bayadera_amd import engine
         fac = engine.HipBayaderaFactory(0, 256)
-        rates = fac.device_rates()
+        rates = device_rates(fac)
         r = [move_roofline(fac, d, rates, m) for m, d in ROOF_DIMS]
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

@@ -282,9 +282,12 @@ int bh_stretch_kernel_timing(bh_stretch* s, double* avg_ms, uint32_t* launches);
 /* hyperparameters (data_len = the rest, as create-sampler splits them).     */
 /* Every problem has `walkers` walkers (a multiple of 128) and runs the     */
 /* single-sampler algorithm with bare seed seed0 + 2b; one launch runs all   */
-/* problems, one work-group each, walkers resident in LDS.                   */
+/* problems, one work-group each, for all steps of a call, its walkers in   */
+/* LDS or in place in global memory (bh_batch_set_state).                   */
 /* ---------------------------------------------------------------------- */
 typedef struct bh_batch bh_batch;
+#define BH_BATCH_STATE_LDS 0    /* positions copied into LDS for the call */
+#define BH_BATCH_STATE_GLOBAL 1 /* positions worked on in place (L2-resident) */
 int bh_batch_create(bh_ctx* ctx, const char* source, const char* logfn_name, uint32_t max_dim,
                     uint32_t problems, const uint32_t* dims, const uint64_t* offsets,
                     const uint32_t* hyper_len, const float* params, uint32_t walkers,
@@ -295,15 +298,25 @@ int bh_batch_set_partner(bh_batch* b, int partner_mode);
  * problems x max_dim x 2 (rows beyond a problem's dim are ignored). */
 int bh_batch_init_position(bh_batch* b, uint32_t seed, const float* limits);
 int bh_batch_burn_in(bh_batch* b, uint32_t seed0, uint64_t n, float a);
+/* Where every launch class keeps its walkers during a call
+ * (BH_BATCH_STATE_LDS or BH_BATCH_STATE_GLOBAL); BH_ERR_RANGE when a class
+ * does not fit the LDS.  Created batches keep them in LDS when that fits.
+ * Positions do not depend on it.  No reference counterpart. */
+int bh_batch_set_state(bh_batch* b, int where);
 /* Chooses the launch shape of the batch by timing `steps` bare steps of each
  * candidate from the current state (restored afterwards): threads per
- * problem (walkers per work-group pass; 64 .. 1024) and launch classes
+ * problem (walkers per work-group pass; 64 .. 1024), launch classes
  * (problems split by dimension into 1-4 launches, each with the LDS its
- * largest problem needs).  Positions do not depend on the shape; the
- * accounting sums' rounding does (their tree has one leaf per thread).  No
- * reference counterpart. */
+ * largest problem needs) and, per class, the state placement that runs it
+ * faster.  Positions do not depend on the shape; the accounting sums'
+ * rounding does (each thread sums its own walker residue).  No reference
+ * counterpart. */
 int bh_batch_autotune(bh_batch* b, uint32_t steps, uint32_t* threads_per_problem,
                       uint32_t* launch_classes, double* best_ms);
+/* The state placement of each launch class, smallest dimensions first: on
+ * entry *count is the capacity of states (may be 0 with states NULL), on
+ * return the number of classes.  No reference counterpart. */
+int bh_batch_class_states(const bh_batch* b, uint32_t* count, int* states);
 /* n steps with accounting: per problem the acceptance rate and, over the n
  * steps and all walkers, the posterior mean and sd of each coordinate
  * (problems x max_dim; NULL to skip). */

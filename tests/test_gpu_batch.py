@@ -29,11 +29,13 @@ def _mvn_problems(B, seed):
     return dims, params
 
 
+@pytest.mark.parametrize("where", [_lib.BH_BATCH_STATE_LDS, _lib.BH_BATCH_STATE_GLOBAL])
 @pytest.mark.parametrize("partner", [_lib.BH_PARTNER_FLAT, _lib.BH_PARTNER_WALKER])
-def test_batch_problems_equal_single_oracle_samplers(fac, partner):
+def test_batch_problems_equal_single_oracle_samplers(fac, partner, where):
     B, W, MAXD = 12, 256, 6
     dims, params = _mvn_problems(B, 4)
     bt = fac.batch(models.mvn_model(MAXD), params, dims, [0] * B, W).set_partner(partner)
+    bt.set_state(where)
     lim = np.zeros((B, MAXD, 2), np.float32)
     lim[:, :, 0], lim[:, :, 1] = -4.0, 4.0
     bt.init_position(100, lim)
@@ -121,3 +123,54 @@ def test_batch_autotune_keeps_trajectories_and_restores_state(fac):
         bt.release()
     assert all(np.array_equal(x, y) for x, y in zip(runs[0][0], runs[1][0]))
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def _hbb_problems(B, seed, GMAX=50):
+    rng = np.random.default_rng(seed)
+    dims, params = [], []
+    lim = np.zeros((B, GMAX + 2, 2), np.float32)
+    for b in range(B):
+        G = int(rng.integers(5, GMAX + 1))
+        n = np.round(np.exp(rng.uniform(np.log(10), np.log(1e5), G)))
+        th = rng.beta(2, 5, G)
+        y = rng.binomial(n.astype(np.int64), th).astype(np.float64)
+        dims.append(G + 2)
+        params.append(np.concatenate([n, y, [10.0]]).astype(np.float32))
+        t = np.log((y + 0.5) / (n - y + 0.5))
+        lim[b, 0] = [0.0, 1.5]
+        lim[b, 1] = [0.5, 2.0]
+        lim[b, 2:G + 2, 0] = t - 0.05
+        lim[b, 2:G + 2, 1] = t + 0.05
+    return dims, params, lim
+
+
+def test_batch_state_placements_agree_and_series_is_the_ensemble_mean(fac):
+    """Walkers in LDS and in place in global memory: identical positions,
+    acceptance, accounting and series (the same arithmetic in the same
+    order); each step's recorded mean is the ensemble mean of that step's
+    positions (checked at the last step)."""
+    import torch
+    B, W, GMAX = 24, 256, 50
+    dims, params, lim = _hbb_problems(B, 77, GMAX)
+    out = []
+    for where in (_lib.BH_BATCH_STATE_LDS, _lib.BH_BATCH_STATE_GLOBAL):
+        bt = fac.batch(models.hbb_model(GMAX), params, dims, [1] * B, W).set_state(where)
+        assert bt.class_states() == [where]
+        bt.init_position(3, lim)
+        bt.burn_in(5, 20, 2.0)
+        n = 30
+        series = torch.zeros(n * int(np.sum(dims)), dtype=torch.float32, device="cuda")
+        res = bt.run(6, n, 2.0, series=series)
+        xs, lf = bt.state()
+        blocks = [np.asarray(blk.cpu()) for blk in bt.series_blocks(series, n)]
+        out.append((xs, lf, res, blocks))
+        bt.release()
+    (x0, l0, r0, s0), (x1, l1, r1, s1) = out
+    assert all(np.array_equal(a, b) for a, b in zip(x0, x1))
+    assert np.array_equal(l0, l1)
+    for k in ("acceptance-rate", "mean", "sd"):
+        assert np.array_equal(np.asarray(r0[k]), np.asarray(r1[k])), k
+    assert all(np.array_equal(a, b) for a, b in zip(s0, s1))
+    for b in range(B):
+        want = x0[b].astype(np.float64).mean(0)
+        assert np.allclose(s0[b][-1], want, rtol=1e-5, atol=1e-5), b
