@@ -26,9 +26,11 @@ typedef unsigned long ulong;
 #endif
 
 // native_* are the hardware approximations in OpenCL C; the gfx950
-// equivalents are v_exp_f32 / v_log_f32 behind __expf / __logf.
+// equivalents are v_exp_f32 / v_log_f32: __expf is exp2(x log2 e) on
+// v_exp_f32, and native_log is log2 on v_log_f32 times ln 2 (HIP's __logf
+// is the full-precision logf sequence).
 #define native_exp(x) __expf(x)
-#define native_log(x) __logf(x)
+#define native_log(x) (__builtin_amdgcn_logf(x) * 0.693147182f)
 #define native_sqrt(x) __builtin_sqrtf(x)
 #define native_recip(x) (1.0f / (x))
 #define native_divide(x, y) ((x) / (y))
