@@ -517,6 +517,54 @@ def c4_problems(B=C4_B, seed=1004):
     return dims, params, lim
 
 
+# MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction every 4 cycles per
+# SIMD at the 2.4 GHz peak engine clock
+VALU_WAVE_INSTS_PEAK = 1024 * 2.4e9 / 4
+
+
+def c4_roofline(walker_steps_per_s, burn_ms_per_step):
+    """VALU-issue roofline of bh_batch_run_g (the batch's burn-in launch):
+    VALU instructions per walker-step from the committed counter profile
+    (SQ_INSTS_VALU of the 100-step burn-in launch) times the burn-in's
+    walker-steps/s, against one wave64 instruction per 4 cycles per SIMD."""
+    path = os.path.join(ROOT, "profiles", "batch_hbb_4096_valu_pmc.json")
+    if not os.path.exists(path):
+        return None
+    prof = json.load(open(path))
+    per_ws = prof["valu_wave_insts_per_walker_step"]
+    ws_burn = C4_B * C4_W / (burn_ms_per_step * 1e-3)
+    achieved = per_ws * ws_burn
+    return {"kernel": "bh_batch_run_g (4096 problems, one work-group each, burn-in launch)",
+            "bound": "valu", "achieved": round(achieved * 64 / 1e12, 2),
+            "peak": round(VALU_WAVE_INSTS_PEAK * 64 / 1e12, 2), "unit": "T VALU lane-instructions/s",
+            "frac": round(achieved / VALU_WAVE_INSTS_PEAK, 4), "traffic": None,
+            "valu_wave_insts_per_walker_step": per_ws,
+            "burn_in_ms_per_step": round(burn_ms_per_step, 4),
+            "source": "profiles/batch_hbb_4096_valu_pmc.json"}
+
+
+def c4_cpu_baseline(dims, params, lim, budget_s=12.0):
+    """Walker-steps/s of the oracle's single sampler (OpenMP over walkers)
+    on the first problems of the batch, one after another, burn-in steps
+    from the same start as the GPU run, within a bounded time."""
+    import oracle as O
+    threads = _threads()
+    steps, done, b = 50, 0, 0
+    t0 = time.perf_counter()
+    while b < len(dims) and (b == 0 or time.perf_counter() - t0 < budget_s):
+        G = dims[b] - 2
+        s = O.Stretch(O.HBB_LOGPDF, dims[b], C4_W, params[b], 5 + 2 * b, wgs=128,
+                      data_len=2 * G, params_len=1, threads=threads)
+        s.init(5 + 2 * b)
+        s.init_position(3 + b, lim[b, :dims[b]])
+        s.burn_in(steps, A)
+        done += C4_W * steps
+        b += 1
+    t = time.perf_counter() - t0
+    return {"walker_steps_per_s": done / t, "threads": threads,
+            "sample": f"{b} problems x {steps} steps x {C4_W} walkers (init + burn-in) in {t:.1f} s"}
+
+
 def main_c4(args):
     import torch
     from bayadera_amd import engine, models
@@ -584,6 +632,16 @@ def main_c4(args):
         "acceptance_rate_median": round(float(np.median(res["acceptance-rate"])), 4),
         "acceptance_rate_min": round(float(np.min(res["acceptance-rate"])), 4),
     }
+    ro = c4_roofline(walker_steps / wall, burn_ms / burn)
+    if ro:
+        line["roofline"] = ro
+    if not args.no_cpu_baseline:
+        cb = c4_cpu_baseline(dims, params, lim)
+        ess_per_ws = float(ess[ok].sum()) / walker_steps
+        line["cpu_baseline"] = {"value": round(cb["walker_steps_per_s"] * ess_per_ws, 1),
+                                "unit": "ESS/s", "cores": cb["threads"], "kind": "port",
+                                "sample": cb["sample"] + "; ESS per walker-step of the GPU run",
+                                "walker_steps_per_s": round(cb["walker_steps_per_s"], 1)}
     print(json.dumps(line), flush=True)
 
 

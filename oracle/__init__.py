@@ -50,6 +50,7 @@ BETA_BINOMIAL_POST = 6
 MVN_LOGPDF = 7
 GAUSSIAN_LOGLIK = 8
 DIAG_GAUSS_LOGPDF = 9
+HBB_LOGPDF = 10
 
 PARTNER_FLAT = 0
 PARTNER_WALKER = 1

@@ -168,6 +168,7 @@ enum {
     BHO_MVN_LOGPDF = 7,            /* correlated Gaussian -(x-mu)'P(x-mu)/2 (4-D benchmark model) */
     BHO_GAUSSIAN_LOGLIK = 8,       /* dist/gaussian.cl:29-42 */
     BHO_DIAG_GAUSS_LOGPDF = 9,     /* independent Gaussian coordinates -sum w_i (x_i-mu_i)^2/2 (benchmark model) */
+    BHO_HBB_LOGPDF = 10,           /* hierarchical beta-binomial on logits (batched benchmark model) */
 };
 
 static const float LOG_SQRT_2PI_F = 0.9189385332046727f;
@@ -234,6 +235,26 @@ BHO_EXPORT float bho_model_logfn(int model, uint32_t data_len, uint32_t params_l
             q = fmaf(w[i] * d, d, q);
         }
         return -0.5f * q;
+    }
+    case BHO_HBB_LOGPDF: {
+        /* x = [log alpha, log beta, t_1 .. t_G], theta_g = 1/(1+exp(-t_g));
+         * params = [n_1 .. n_G, y_1 .. y_G, s0] (data_len = 2G):
+         * -G lnB(a, b) - sum_g [(y_g + a) sp(-t_g) + (n_g - y_g + b) sp(t_g)]
+         * - (la^2 + lb^2) / (2 s0^2), sp(t) = max(t, 0) + log(1 + exp(-|t|)) */
+        const uint32_t G = dim - 2;
+        const float* n = params;
+        const float* y = params + G;
+        const float s0 = params[data_len];
+        const float la = x[0], lb = x[1];
+        const float a = expf(la), b = expf(lb);
+        float acc = -(float)G * (lgammaf(a) + lgammaf(b) - lgammaf(a + b));
+        for (uint32_t g = 0; g < G; ++g) {
+            const float t = x[2 + g];
+            const float spm = fmaxf(-t, 0.0f) + logf(1.0f + expf(-fabsf(t)));
+            const float spp = fmaxf(t, 0.0f) + logf(1.0f + expf(-fabsf(t)));
+            acc -= (y[g] + a) * spm + (n[g] - y[g] + b) * spp;
+        }
+        return acc - 0.5f * (la * la + lb * lb) / (s0 * s0);
     }
     case BHO_GAUSSIAN_LOGLIK: {
         /* data = params[0 .. data_len), x = (mu, sigma) */

@@ -174,3 +174,34 @@ def test_batch_state_placements_agree_and_series_is_the_ensemble_mean(fac):
     for b in range(B):
         want = x0[b].astype(np.float64).mean(0)
         assert np.allclose(s0[b][-1], want, rtol=1e-5, atol=1e-5), b
+
+
+@pytest.mark.parametrize("where", [_lib.BH_BATCH_STATE_LDS, _lib.BH_BATCH_STATE_GLOBAL])
+def test_batch_hbb_problems_equal_single_oracle_samplers(fac, where):
+    """Hierarchical beta-binomials (DIM 7-52) in one batch: each problem's
+    walkers after burn-in equal the oracle's single sampler with the same
+    seeds (model evaluations differ in the last bits between the devices'
+    log, exp and lgamma, so a few accept tests may go the other way)."""
+    B, W, GMAX = 6, 256, 50
+    dims, params, lim = _hbb_problems(B, 5, GMAX)
+    bt = fac.batch(models.hbb_model(GMAX), params, dims, [1] * B, W).set_state(where)
+    bt.init_position(40, lim)
+    xs0, lf0 = bt.state()
+    bt.burn_in(60, 4, 2.0)
+    xs, _ = bt.state()
+    for b in range(B):
+        ref = O.Stretch(O.HBB_LOGPDF, dims[b], W, params[b], 60 + 2 * b, wgs=128,
+                        data_len=2 * (dims[b] - 2), params_len=1, threads=4)
+        ref.init(60 + 2 * b)
+        ref.init_position(40 + b, lim[b, :dims[b]])
+        # identical starting walkers; the model's log density within a few
+        # float32 ulps of the oracle's
+        assert np.array_equal(xs0[b], ref.xs.reshape(W, dims[b]))
+        assert np.allclose(lf0[b], ref.logfn, rtol=2e-6, atol=0), b
+        ref.burn_in(4, 2.0)
+        x = xs[b]
+        r = ref.xs.reshape(W, dims[b])
+        same = np.all(x == r, axis=1).mean()
+        assert same > 0.97, (b, dims[b], same)
+        close = np.abs(x - r) <= 1e-5 * np.maximum(1.0, np.abs(r))
+        assert np.all(close[np.all(x == r, axis=1)])

@@ -149,3 +149,35 @@ def test_diag_gaussian_model_closed_form():
     want = -0.5 * np.sum(w.astype(np.float64) * (x.astype(np.float64) - mu) ** 2, axis=1)
     got = O.logfn_many(O.DIAG_GAUSS_LOGPDF, params, x, dim)
     assert np.allclose(got, want, rtol=1e-5)
+
+
+def test_hbb_model_against_its_definition():
+    """Hierarchical beta-binomial on logits: the oracle's density equals,
+    up to a constant, binomial likelihoods x Beta(a, b) priors on theta_g x
+    the logit Jacobian theta (1 - theta) x N(0, s0^2) priors on log a and
+    log b, built from scipy's distributions in float64 (differences between
+    points, so the dropped constants cancel)."""
+    from scipy import special, stats
+    rng = np.random.default_rng(11)
+    G, s0 = 12, 10.0
+    n = np.round(np.exp(rng.uniform(np.log(10), np.log(1e4), G)))
+    y = rng.binomial(n.astype(np.int64), rng.beta(2, 5, G)).astype(np.float64)
+    params = np.concatenate([n, y, [s0]]).astype(np.float32)
+
+    def ref(x):
+        la, lb, t = x[0], x[1], x[2:]
+        a, b = np.exp(la), np.exp(lb)
+        theta = special.expit(t)
+        lp = stats.binom.logpmf(y, n, theta).sum()
+        lp += stats.beta.logpdf(theta, a, b).sum()
+        lp += np.sum(np.log(theta) + np.log1p(-theta))
+        return lp + stats.norm.logpdf(la, 0, s0) + stats.norm.logpdf(lb, 0, s0)
+
+    t0 = np.log((y + 0.5) / (n - y + 0.5))
+    xs = np.array([np.concatenate([[rng.uniform(0, 1.5), rng.uniform(0.5, 2.0)],
+                                   t0 + 0.2 * rng.standard_normal(G)]) for _ in range(20)],
+                  np.float32)
+    got = O.logfn_many(O.HBB_LOGPDF, params, xs, G + 2, data_len=2 * G,
+                       params_len=1).astype(np.float64)
+    want = np.array([ref(x.astype(np.float64)) for x in xs])
+    assert np.allclose(got - got[0], want - want[0], rtol=1e-5, atol=0.01)
