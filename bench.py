@@ -15,8 +15,9 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
     fp64    the fp64 likelihood kernel (coupled windows; the whole job with
             --c3-fp64-full).
   ESS_d = Var_d(final ensemble) / sigma_d^2 with sigma_d from the engine's
-  acor of the run-sampler means; value = min_d ESS_d per second of the
-  MFMA job (burn-in + run, inputs resident in HBM).  Posterior-mean error
+  acor of the run-sampler means (Sokal's windowed estimate for a dimension
+  where acor gives no value); value = min_d ESS_d per second of the MFMA
+  job (burn-in + run, inputs resident in HBM).  Posterior-mean error
   = |mean_d - mean_fp64,d| / sigma_fp64,d on these coupled runs.
   Roofline: the MFMA likelihood kernel, 2 N d K = 1.05e12 algorithmic flops
   per launch, its launches timed with HIP events on the sampler's stream
@@ -98,13 +99,17 @@ def ess_from(draws_var, sigma):
     return draws_var.astype(np.float64) / np.square(sigma.astype(np.float64))
 
 
-def ess_min(ess):
-    """min over dimensions of the finite ESS values and the count of
-    dimensions where the reference's acor gave no finite sigma (its window
-    recursion can end on a negative autocovariance sum for short series)."""
-    ess = np.asarray(ess, np.float64)
-    ok = np.isfinite(ess)
-    return (float(ess[ok].min()) if ok.any() else float("nan")), int((~ok).sum())
+def ess_acor_or_sokal(ess_acor, var, series):
+    """Per-dimension ESS: the reference's acor where it gives a value,
+    Sokal's windowed estimate (var / (tau_sokal var(series) / n)) where it
+    does not.  Returns (min over all dimensions, dims without an acor value,
+    Sokal's per-dimension ESS)."""
+    ess_acor = np.asarray(ess_acor, np.float64)
+    m = np.asarray(series, np.float64)
+    ess_s = np.asarray(var, np.float64) / (sokal_tau(m) * m.var(0) / m.shape[0])
+    ok = np.isfinite(ess_acor)
+    comb = np.where(ok, ess_acor, ess_s)
+    return float(np.min(comb)), int((~ok).sum()), ess_s
 
 
 def sokal_tau(series, c=5.0):
@@ -432,7 +437,7 @@ def main_c2(args):
     res, var, wall, run_ms, kt, xs, lpv = c2_job(sf, params, limits, burn, run, _lib.BH_GLM_FP32)
     ac = res["autocorrelation"]
     ess = ess_from(var, ac.sigma)
-    ess_lo, nan_dims = ess_min(ess)
+    ess_lo, nan_dims, _ = ess_acor_or_sokal(ess, var, res["means"])
     xs = xs.reshape(-1, C2_D + 1)
     lp64 = c2_loglik64(rows, xs[:2048])
     lp_err = np.abs(lpv[:2048].astype(np.float64) - lp64) / np.abs(lp64)
@@ -952,10 +957,11 @@ def main_c3(args):
     out = {}
     for name, prec, g in legs:
         r = res[name]
-        ess, nan_dims = ess_min(r["ess"])
         m = np.asarray(r["means"], np.float64)
+        # min over dims of the acor ESS, Sokal's where acor gives none
+        ess, nan_dims, ess_sd = ess_acor_or_sokal(r["ess"], np.asarray(r["sd"]) ** 2, m)
         tau_s = sokal_tau(m)
-        ess_s = float(np.min(r["sd"] ** 2 / (tau_s * m.var(0) / m.shape[0])))
+        ess_s = float(np.min(ess_sd))
         o = {"ess_per_s": round(ess / r["wall_s"], 2),
              "acor_nan_dims": nan_dims,
              "ess_sokal_min": round(ess_s, 1),
@@ -979,7 +985,7 @@ def main_c3(args):
             o["trajectory_equals_fp64"] = bool(np.array_equal(r["xs"], ref["xs"]))
         out[name] = o
     if ref is not None:
-        ess, _ = ess_min(ref["ess"])
+        ess, _, _ = ess_acor_or_sokal(ref["ess"], np.asarray(ref["sd"]) ** 2, ref["means"])
         out["fp64"] = {"ess_per_s": round(ess / ref["wall_s"], 2), "ess_min": round(ess, 1),
                        "wall_s": round(ref["wall_s"], 3),
                        "ms_per_step_run": round(ref["run_gpu_ms"] / run, 3),
