@@ -309,7 +309,8 @@ struct bh_stretch_factory {
     hipFunction_t logfn = nullptr, bare = nullptr, accu = nullptr, resident = nullptr;
     std::unique_ptr<GlmData> glm;
     hipFunction_t glm_propose = nullptr, glm_split = nullptr, glm_logpost = nullptr,
-                  glm_accept = nullptr, glm_screen = nullptr, glm_resolve = nullptr;
+                  glm_accept = nullptr, glm_screen = nullptr, glm_resolve = nullptr,
+                  glm_accept_split = nullptr;
 };
 
 struct bh_batch {
@@ -357,7 +358,7 @@ struct bh_stretch {
     DevBuf lp, gY, gz, gu2, gsplit, gpartial;
     // guarded accept test of the reduced precisions (guard 0: off)
     double glm_guard = 0.0;
-    DevBuf gflag, glist, gcount, gtotals, gpart64, gpts64;
+    DevBuf gflag, glist, gcount, gtotals, gpart64, gpts64, glpy;
     // event pairs around the dominant kernel's launches (the stretch move
     // kernels, or the GLM likelihood contraction), up to the pool's size
     bool ktiming = false;
@@ -599,8 +600,9 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         double guard = s->glm_guard * std::max(1.0f, beta);
         const double* lpc = lp;
         double* pts64 = s->gpts64.as<double>();
+        double* lpy = s->glpy.as<double>();
         void* sa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &lpc, &beta, &K, &guard,
-                      &flag, &list, &gcount, &lmax, &Xc, &pts64};
+                      &flag, &list, &gcount, &lmax, &Xc, &pts64, &lpy};
         BH_TRY(launch(f->glm_screen, K / s->wgs, s->wgs, 0, s->stream, sa));
         double* part64 = s->gpart64.as<double>();
         BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, pts64, gcount, lmax,
@@ -614,6 +616,16 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         void* ra[] = {&Yc, &Xc, &zc, &u2c, &p64, &nr64, &stride, &lin64, &pp, &beta, &listc,
                       &countc, &lmax, &flag, &totals};
         BH_TRY(launch(f->glm_resolve, kGuardResolveBlocks, s->wgs, 0, s->stream, ra));
+        // every walker decided: the accept step in (group, coordinate chunk)
+        // work-groups
+        const unsigned chunk = std::min(s->dim, 8u);
+        const unsigned nchunks = (s->dim + chunk - 1) / chunk;
+        const double* lpyc = lpy;
+        const unsigned* flagc = flag;
+        void* xa[] = {&Yc, &lpyc, &X, &lp, &accu, &acc, &means, &mstep, &flagc};
+        BH_HIP(hipModuleLaunchKernel(f->glm_accept_split, K / s->wgs, nchunks, 1, s->wgs, 1, 1, 0,
+                                     s->stream, xa, nullptr));
+        return BH_OK;
     }
     const unsigned* flagc = flag;
     void* aa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &X, &lp, &beta, &K, &accu,
@@ -639,6 +651,7 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
         BH_TRY(s->gz.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gu2.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gflag.alloc((size_t)k_half * sizeof(unsigned)));
+        BH_TRY(s->glpy.alloc((size_t)k_half * sizeof(double)));
         BH_TRY(s->glist.alloc((size_t)kGuardListMax * sizeof(unsigned)));
         BH_TRY(s->gcount.alloc(sizeof(unsigned)));
         BH_TRY(s->gtotals.alloc(2 * sizeof(unsigned long long)));
@@ -1255,6 +1268,7 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
     BH_TRY(f->mod.fn("bh_glm_logpost", &f->glm_logpost));
     BH_TRY(f->mod.fn("bh_glm_accept", &f->glm_accept));
     BH_TRY(f->mod.fn("bh_glm_screen", &f->glm_screen));
+    BH_TRY(f->mod.fn("bh_glm_accept_split", &f->glm_accept_split));
     BH_TRY(f->mod.fn("bh_glm_resolve", &f->glm_resolve));
     auto g = std::make_unique<GlmData>();
     g->kind = kind;
@@ -1311,6 +1325,7 @@ int bh_glm_gaussian_factory_create(bh_ctx* ctx, const float* X, const float* y, 
     BH_TRY(f->mod.fn("bh_glm_logpost", &f->glm_logpost));
     BH_TRY(f->mod.fn("bh_glm_accept", &f->glm_accept));
     BH_TRY(f->mod.fn("bh_glm_screen", &f->glm_screen));
+    BH_TRY(f->mod.fn("bh_glm_accept_split", &f->glm_accept_split));
     BH_TRY(f->mod.fn("bh_glm_resolve", &f->glm_resolve));
     auto g = std::make_unique<GlmData>();
     g->kind = BH_GLM_GAUSSIAN;

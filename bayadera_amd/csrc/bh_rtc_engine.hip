@@ -1078,12 +1078,13 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
               const double* __restrict__ lp, float beta, unsigned K, double guard,
               unsigned* __restrict__ flag, unsigned* __restrict__ list,
               unsigned* __restrict__ count, unsigned lmax, const float* __restrict__ Xpos,
-              double* __restrict__ pts64) {
+              double* __restrict__ pts64, double* __restrict__ lpy_out) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
     const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
+    lpy_out[k] = lpy;
     unsigned f = 0;
     if (__builtin_isfinite(lpy)) {
         const double m = log((double)u2buf[k]) - (double)(DIM - 1) * log((double)zbuf[k]) -
@@ -1103,6 +1104,9 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
             }
         }
     }
+    // walkers not listed are decided here, on this precision's values
+    // (1 reject, 2 accept; bh_glm_resolve decides the listed ones)
+    if (f == 0) f = bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta) ? 2u : 1u;
     flag[k] = f;
 }
 
@@ -1180,6 +1184,54 @@ bh_glm_accept(const float* __restrict__ Y, const float* __restrict__ zbuf,
         __shared__ float red[BH_ACCU_CHUNK * WGS];
         __shared__ unsigned wave_counts[WGS / 64];
         bh_accu(xk, accepted, blockIdx.x, gridDim.x, mstep, accept, means, red, wave_counts);
+    }
+}
+// The guarded path's accept step, with every decision already in flag (1
+// reject, 2 accept) and the proposals' log posteriors in lpy: work-group
+// (g, c) handles group g's WGS walkers and coordinates
+// [c*BH_ACCU_CHUNK, (c+1)*BH_ACCU_CHUNK): it copies accepted proposals of
+// those coordinates into X and adds their group sums to the means (the
+// same tree per coordinate as bh_accu); chunk 0 also stores the accepted
+// log posteriors and the group's accept count.
+extern "C" __global__ void __launch_bounds__(WGS)
+bh_glm_accept_split(const float* __restrict__ Y, const double* __restrict__ lpy,
+                    float* __restrict__ X, double* __restrict__ lp, int accu,
+                    unsigned* __restrict__ accept, float* __restrict__ means, unsigned mstep,
+                    const unsigned* __restrict__ flag) {
+    const unsigned g = blockIdx.x, c = blockIdx.y;
+    const unsigned k = g * WGS + threadIdx.x;
+    const bool accepted = flag[k] == 2u;
+    const unsigned d0 = c * BH_ACCU_CHUNK;
+    float v[BH_ACCU_CHUNK];
+#pragma unroll
+    for (int i = 0; i < BH_ACCU_CHUNK; ++i) {
+        const unsigned d = d0 + i;
+        v[i] = 0.0f;
+        if (d < DIM) {
+            if (accepted) {
+                v[i] = Y[(size_t)k * DIM + d];
+                X[(size_t)k * DIM + d] = v[i];
+            } else if (accu) {
+                v[i] = X[(size_t)k * DIM + d];
+            }
+        }
+    }
+    if (c == 0 && accepted) lp[k] = lpy[k];
+    if (accu) {
+        __shared__ float red[BH_ACCU_CHUNK * WGS];
+        __shared__ unsigned wave_counts[WGS / 64];
+        const unsigned groups = gridDim.x;
+        const unsigned cnt = bh_slice_count(accepted, wave_counts);
+        const bool leader = threadIdx.x == 0;
+        if (c == 0 && leader) accept[g] += cnt;
+        float sums[BH_ACCU_CHUNK];
+        bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums);
+        if (leader) {
+            float* m = means + (size_t)groups * DIM * mstep + g;
+#pragma unroll
+            for (int i = 0; i < BH_ACCU_CHUNK; ++i)
+                if (d0 + i < DIM) m[(size_t)(d0 + i) * groups] += sums[i];
+        }
     }
 }
 #endif  // BH_GLM
