@@ -19,56 +19,89 @@
 #define BH_PARTNER_FLAT 0
 #define BH_PARTNER_WALKER 1
 
+// The last six levels of the work-group tree inside one wave, for lane 0:
+// level i adds lane t + i to lane t (i = 32, 16, 8, 4, 2, 1), the pairing
+// of the reference's tree; v_permlane32_swap / v_permlane16_swap bring
+// lanes t + 32 / t + 16 to lanes t < 32 / t < 16 (the builtins' second
+// result), DPP row_shl:i lanes t + i within each 16-lane row.
+__device__ __forceinline__ float bh_wave_tree_sum(float x) {
+    const unsigned u = __float_as_uint(x);
+    x = x + __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[1]);
+    const unsigned u16 = __float_as_uint(x);
+    x = x + __uint_as_float(__builtin_amdgcn_permlane16_swap(u16, u16, false, false)[1]);
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x108, 0xf, 0xf, false));
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x104, 0xf, 0xf, false));
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x102, 0xf, 0xf, false));
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x101, 0xf, 0xf, false));
+    return x;
+}
+
 // Sum v over each aligned slice of WGS threads with the reference's
 // work-group tree (amd-gcn-mcmc-stretch.cl:5-29 for power-of-two WGS: level
-// i adds item l+i into item l, i = WGS/2 ... 1).  The levels i >= 64 cross
-// wavefronts and run in LDS; levels 32 ... 1 run inside the slice's first
-// wave with cross-lane moves.  Float addition is not associative, so the
-// tree shape is kept exactly: the result is bit-identical to the reference's.
-// Values are staged in buf[d*blockDim + t]; the total of dim d for the slice
-// is returned in out[d] by the slice's lane 0.
+// i adds item l+i into item l, i = WGS/2 ... 1).  Float addition is not
+// associative, so the tree shape is kept exactly: the result is
+// bit-identical to the reference's.  Values are staged in
+// buf[d*blockDim + t]; after one barrier lane l < 64 of the slice reads
+// items l + 64j (j < WGS/64) and applies the levels i >= 64 to them in the
+// tree's pairing (item l + 64j gets item l + 64(j + i/64)), then the slice's
+// first wave applies levels 32 ... 1 with cross-lane moves.  The total of
+// dim d is returned in out[d] by the slice's lane 0.  With count non-null
+// the slice's set flags are counted too (ballot counts staged with the
+// values; order-independent integers), returned in *count by lane 0.
+// tail_sync: a barrier at the end, before the caller writes buf or
+// wave_counts again (not needed when a barrier or the kernel's end comes
+// first).
 template <int ND>
-__device__ __forceinline__ void bh_slice_tree_sums(const float (&v)[ND], float* buf,
-                                                   float (&out)[ND]) {
+__device__ __forceinline__ void bh_slice_tree(const float (&v)[ND], float* buf, float (&out)[ND],
+                                              bool flag, unsigned* wave_counts,
+                                              unsigned* count, bool tail_sync) {
+    static_assert(WGS >= 64 && (WGS & (WGS - 1)) == 0, "WGS: a power of two >= 64");
+    constexpr unsigned R = WGS / 64;
     const unsigned t = threadIdx.x;
     const unsigned nb = blockDim.x;
     const unsigned l = t & (WGS - 1);
+    if (count) {
+        const unsigned long long m = __ballot(flag);
+        if ((t & 63u) == 0) wave_counts[t >> 6] = (unsigned)__popcll(m);
+    }
 #pragma unroll
     for (int d = 0; d < ND; ++d) buf[d * nb + t] = v[d];
     __syncthreads();
-#pragma unroll
-    for (unsigned i = WGS / 2; i >= 64; i >>= 1) {
-        if (l < i) {
-#pragma unroll
-            for (int d = 0; d < ND; ++d) buf[d * nb + t] = buf[d * nb + t] + buf[d * nb + t + i];
-        }
-        __syncthreads();
-    }
     if (l < 64) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            float x = buf[d * nb + t];
+            float x[R];
 #pragma unroll
-            for (int i = 32; i >= 1; i >>= 1) x += __shfl_down(x, i, 64);
-            out[d] = x;
+            for (unsigned j = 0; j < R; ++j) x[j] = buf[d * nb + t + 64 * j];
+#pragma unroll
+            for (unsigned h = R / 2; h >= 1; h >>= 1)
+#pragma unroll
+                for (unsigned j = 0; j < h; ++j) x[j] = x[j] + x[j + h];
+            out[d] = bh_wave_tree_sum(x[0]);
+        }
+        if (count && l == 0) {
+            unsigned total = 0;
+            const unsigned w0 = t >> 6;
+            for (unsigned w = 0; w < R; ++w) total += wave_counts[w0 + w];
+            *count = total;
         }
     }
-    __syncthreads();  // buf is reused by the caller
+    if (tail_sync) __syncthreads();
 }
 
-// Number of set flags over each slice of WGS threads (order-independent).
-__device__ __forceinline__ unsigned bh_slice_count(bool flag, unsigned* wave_counts) {
-    const unsigned t = threadIdx.x;
-    const unsigned long long m = __ballot(flag);
-    if ((t & 63u) == 0) wave_counts[t >> 6] = (unsigned)__popcll(m);
-    __syncthreads();
-    unsigned total = 0;
-    if ((t & (WGS - 1)) == 0) {
-        const unsigned w0 = t >> 6;
-        for (unsigned w = 0; w < WGS / 64; ++w) total += wave_counts[w0 + w];
-    }
-    __syncthreads();
-    return total;
+template <int ND>
+__device__ __forceinline__ void bh_slice_tree_sums(const float (&v)[ND], float* buf,
+                                                   float (&out)[ND], bool tail_sync = true) {
+    bh_slice_tree<ND>(v, buf, out, false, nullptr, nullptr, tail_sync);
+}
+
+template <int ND>
+__device__ __forceinline__ void bh_slice_count_tree_sums(bool flag, const float (&v)[ND],
+                                                         float* buf, unsigned* wave_counts,
+                                                         float (&out)[ND], unsigned& count,
+                                                         bool tail_sync = true) {
+    count = 0;
+    bh_slice_tree<ND>(v, buf, out, flag, wave_counts, &count, tail_sync);
 }
 
 #if defined(LOGFN) || defined(BH_GLM)
@@ -183,16 +216,23 @@ __device__ __forceinline__ float bh_zpow(float z) {
 __device__ __forceinline__ void bh_accu(const float (&xk)[DIM], bool accepted, unsigned g,
                                         unsigned groups, unsigned mstep, unsigned* accept,
                                         float* means, float* red, unsigned* wave_counts) {
-    const unsigned cnt = bh_slice_count(accepted, wave_counts);
     const bool leader = (threadIdx.x & (WGS - 1)) == 0;
-    if (leader) accept[g] += cnt;
     float* m = means + (size_t)groups * DIM * mstep + g;
 #pragma unroll
     for (int d0 = 0; d0 < DIM; d0 += BH_ACCU_CHUNK) {
         float v[BH_ACCU_CHUNK], sums[BH_ACCU_CHUNK];
 #pragma unroll
         for (int c = 0; c < BH_ACCU_CHUNK; ++c) v[c] = (d0 + c < DIM) ? xk[d0 + c] : 0.0f;
-        bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums);
+        // every caller ends its kernel here: no barrier after the last chunk
+        const bool more = d0 + BH_ACCU_CHUNK < DIM;
+        if (d0 == 0) {
+            unsigned cnt;
+            bh_slice_count_tree_sums<BH_ACCU_CHUNK>(accepted, v, red, wave_counts, sums, cnt,
+                                                    more);
+            if (leader) accept[g] += cnt;
+        } else {
+            bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums, more);
+        }
         if (leader) {
 #pragma unroll
             for (int c = 0; c < BH_ACCU_CHUNK; ++c)
@@ -264,17 +304,25 @@ bh_logfn(unsigned data_len, unsigned params_len, const float* __restrict__ param
 __device__ __forceinline__ void bh_accu_resident(const float (&xk)[DIM], bool accepted, unsigned g,
                                                  unsigned groups, unsigned mstep, int half,
                                                  unsigned* acc_lds, float* odd_lds, float* means,
-                                                 float* red, unsigned* wave_counts) {
-    const unsigned cnt = bh_slice_count(accepted, wave_counts);
+                                                 float* red, unsigned* wave_counts,
+                                                 bool last_pass) {
     const bool leader = (threadIdx.x & (WGS - 1)) == 0;
-    if (leader) acc_lds[g] += cnt;
     float* m = means + (size_t)groups * DIM * mstep + g;
 #pragma unroll
     for (int d0 = 0; d0 < DIM; d0 += BH_ACCU_CHUNK) {
         float v[BH_ACCU_CHUNK], sums[BH_ACCU_CHUNK];
 #pragma unroll
         for (int c = 0; c < BH_ACCU_CHUNK; ++c) v[c] = (d0 + c < DIM) ? xk[d0 + c] : 0.0f;
-        bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums);
+        // the half-step's own barrier follows its last pass over walkers
+        const bool sync = d0 + BH_ACCU_CHUNK < DIM || !last_pass;
+        if (d0 == 0) {
+            unsigned cnt;
+            bh_slice_count_tree_sums<BH_ACCU_CHUNK>(accepted, v, red, wave_counts, sums, cnt,
+                                                    sync);
+            if (leader) acc_lds[g] += cnt;
+        } else {
+            bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums, sync);
+        }
         if (leader) {
 #pragma unroll
             for (int c = 0; c < BH_ACCU_CHUNK; ++c) {
@@ -610,7 +658,7 @@ __device__ __forceinline__ void bh_resident_impl(
                 }
                 if (mode == 1)
                     bh_accu_resident(xk, acc, k / WGS, groups, step, h, acc_lds, odd_lds, means,
-                                     red, wave_counts);
+                                     red, wave_counts, k0 + nb >= K);
             }
             __syncthreads();
         }
@@ -1221,11 +1269,11 @@ bh_glm_accept_split(const float* __restrict__ Y, const double* __restrict__ lpy,
         __shared__ float red[BH_ACCU_CHUNK * WGS];
         __shared__ unsigned wave_counts[WGS / 64];
         const unsigned groups = gridDim.x;
-        const unsigned cnt = bh_slice_count(accepted, wave_counts);
         const bool leader = threadIdx.x == 0;
-        if (c == 0 && leader) accept[g] += cnt;
         float sums[BH_ACCU_CHUNK];
-        bh_slice_tree_sums<BH_ACCU_CHUNK>(v, red, sums);
+        unsigned cnt;
+        bh_slice_count_tree_sums<BH_ACCU_CHUNK>(accepted, v, red, wave_counts, sums, cnt, false);
+        if (c == 0 && leader) accept[g] += cnt;
         if (leader) {
             float* m = means + (size_t)groups * DIM * mstep + g;
 #pragma unroll
