@@ -474,6 +474,22 @@ gb_logpost_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dim
 // d <= D), in 16-byte accesses over the first D slots of its proposal and
 // position (the slots from d on are zeros in both, and so are the linear
 // coefficients there).
+// Sum over the wave for lane 0 in the butterfly's pairing (lane t adds
+// lane t + i, i = 32, 16, 8, 4, 2, 1): v_permlane32_swap / v_permlane16_swap
+// bring lanes t + 32 / t + 16 to the low lanes (the builtins' second result),
+// DPP row_shl:i lane t + i within each 16-lane row.
+__device__ __forceinline__ float gb_wave_sum_lane0(float x) {
+    const unsigned u = __float_as_uint(x);
+    x = x + __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[1]);
+    const unsigned u16 = __float_as_uint(x);
+    x = x + __uint_as_float(__builtin_amdgcn_permlane16_swap(u16, u16, false, false)[1]);
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x108, 0xf, 0xf, false));
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x104, 0xf, 0xf, false));
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x102, 0xf, 0xf, false));
+    x = x + __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x101, 0xf, 0xf, false));
+    return x;
+}
+
 template <int D>
 __device__ __forceinline__ void gb_accept_one(int half, unsigned p, unsigned k, unsigned d,
                                               const unsigned* __restrict__ ubeg,
@@ -542,12 +558,8 @@ __device__ __forceinline__ void gb_accept_one(int half, unsigned p, unsigned k, 
         const float a[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            float s1 = a[e], s2 = a[e] * a[e];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                s1 += __shfl_xor(s1, off, 64);
-                s2 += __shfl_xor(s2, off, 64);
-            }
+            const float s1 = gb_wave_sum_lane0(a[e]);
+            const float s2 = gb_wave_sum_lane0(a[e] * a[e]);
             if (lane == 0 && 4 * c + e < d) {
                 red[0][wave][4 * c + e] = s1;
                 red[1][wave][4 * c + e] = s2;
