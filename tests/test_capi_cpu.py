@@ -109,6 +109,9 @@ def test_bad_wgs_rejected():
     (["-DWGS=256", "-DDIM=17", "-DBH_GLM=1", "-DBH_GLM_KIND=2"], ""),
     (["-DWGS=256", "-DDIM=52", "-DBH_BATCH=1", "-DLOGFN=hbb_logpdf"], models.HBB_SOURCE),
     (["-DWGS=256", "-DDIM=6", "-DBH_BATCH=1", "-DLOGFN=mvn_logpdf"], models.MVN_SOURCE),
+    # the move kernels' LDS-staged records: 16-byte pieces, float pieces
+    (["-DWGS=256", "-DDIM=16", "-DLOGFN=mvn_logpdf"], models.MVN_SOURCE),
+    (["-DWGS=256", "-DDIM=33", "-DLOGFN=diag_gaussian_logpdf"], models.DIAG_GAUSS_SOURCE),
 ])
 def test_engine_variants_compile(tmp_path, defines, source):
     """The GLM and batched variants of the run-time-compiled engine source
@@ -129,3 +132,10 @@ def test_engine_variants_compile(tmp_path, defines, source):
                           "-std=c++17", "-ffp-contract=on", "-I", csrc, *defines, "-c", str(src),
                           "-o", str(tmp_path / "engine.o")], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr[-3000:]
+
+
+def test_batch_state_calls_reject_null():
+    lib = _lib.load()
+    assert lib.bh_batch_set_state(None, _lib.BH_BATCH_STATE_GLOBAL) == _lib.BH_ERR_ARG
+    n = ctypes.c_uint32(0)
+    assert lib.bh_batch_class_states(None, ctypes.byref(n), None) == _lib.BH_ERR_ARG
