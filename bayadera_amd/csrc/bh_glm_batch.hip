@@ -221,45 +221,88 @@ gb_points_kernel(const float* __restrict__ xs, const unsigned* __restrict__ dims
         gb_points_one<64>(x, t, d, P, p, n, k, ysc, ysplit, nsplit, f16);
 }
 
-// The proposal of walker k of problem p (dimension d <= D): reads of its
-// position and its partner's, and writes of Y, in 16-byte pieces of the
-// first D slots.
+// The proposals of a block's 256 walkers from their partners (indices jl,
+// stretch factors zl), one 16-byte piece per thread: D/4 consecutive threads
+// take one walker's record, so the block reads and writes its records, and
+// each partner's, as consecutive pieces: Y = fma(z, X - S, S) (zero beyond
+// d), then the scaled point and its split as gb_store_point makes them.
 template <int D>
-__device__ __forceinline__ void gb_propose_one(const float* __restrict__ xk,
-                                               const float* __restrict__ sj, float z, unsigned d,
-                                               float* __restrict__ yo, const float* t, unsigned P,
-                                               unsigned p, unsigned K, unsigned k, float* ysc,
-                                               void* ysplit, int nsplit, int f16) {
-    float y[D];
-#pragma unroll
-    for (int c = 0; c < D / 4; ++c) {
-        const float4 xv = reinterpret_cast<const float4*>(xk)[c];
-        const float4 sv = reinterpret_cast<const float4*>(sj)[c];
+__device__ __forceinline__ void gb_propose_pieces(const float* __restrict__ X,
+                                                  const float* __restrict__ S,
+                                                  const unsigned* jl, const float* zl,
+                                                  unsigned d, unsigned p, unsigned K, unsigned kb,
+                                                  float* __restrict__ Y,
+                                                  const float* __restrict__ t, unsigned P,
+                                                  float* __restrict__ ysc,
+                                                  void* __restrict__ ysplit, int nsplit,
+                                                  int f16) {
+    typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    constexpr unsigned T = D / 4, WP = 256 / T;
+    const unsigned c = threadIdx.x % T;
+    const float4 tv = reinterpret_cast<const float4*>(t)[c];
+    const float ta[4] = {tv.x, tv.y, tv.z, tv.w};
+    for (unsigned it = 0; it < T; ++it) {
+        const unsigned w = it * WP + threadIdx.x / T;
+        const unsigned k = kb + w;
+        const float4 xv = reinterpret_cast<const float4*>(X + (size_t)k * kGbD)[c];
+        const float4 sv = reinterpret_cast<const float4*>(S + (size_t)jl[w] * kGbD)[c];
+        const float z = zl[w];
         const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
         const float sa[4] = {sv.x, sv.y, sv.z, sv.w};
+        float y[4], r[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const unsigned i = 4 * c + e;
-            y[i] = i < d ? __builtin_fmaf(z, xa[e] - sa[e], sa[e]) : 0.0f;
+            y[e] = i < d ? __builtin_fmaf(z, xa[e] - sa[e], sa[e]) : 0.0f;
+            r[e] = i < d ? y[e] * ta[e] : 0.0f;
+        }
+        const size_t row = ((size_t)p * K + k) * kGbD;
+        reinterpret_cast<float4*>(Y + row)[c] = make_float4(y[0], y[1], y[2], y[3]);
+        reinterpret_cast<float4*>(ysc + row)[c] = make_float4(r[0], r[1], r[2], r[3]);
+        if (f16) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r[e] *= kGbF16Up;
+        }
+        for (int sp = 0; sp < nsplit; ++sp) {
+            const size_t at = (((size_t)sp * P + p) * K + k) * kGbD;
+            if (f16) {
+                f16x4 h;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    h[e] = (_Float16)r[e];
+                    r[e] -= (float)h[e];
+                }
+                reinterpret_cast<f16x4*>(static_cast<_Float16*>(ysplit) + at)[c] = h;
+            } else {
+                bf16x4 h;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    h[e] = (__bf16)r[e];
+                    r[e] -= (float)h[e];
+                }
+                reinterpret_cast<bf16x4*>(static_cast<__bf16*>(ysplit) + at)[c] = h;
+            }
         }
     }
-#pragma unroll
-    for (int c = 0; c < D / 4; ++c)
-        reinterpret_cast<float4*>(yo)[c] = make_float4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
-    gb_store_point<D>(y, t, d, P, p, K, k, ysc, ysplit, nsplit, f16);
 }
 
 // Proposals of one half-step for all problems (the move of
 // amd-gcn-mcmc-stretch.cl:31-65 with whole-walker partners): problem p uses
-// seed seed0 + 2p (+1 for the second half) and counter word `oe`.
+// seed seed0 + 2p (+1 for the second half) and counter word `oe`.  Each
+// thread draws one walker's uniforms; the records are then built piece by
+// piece (gb_propose_pieces).
 __global__ void __launch_bounds__(256)
 gb_propose_kernel(unsigned seed0, int half, unsigned oe, unsigned step, float a,
                   const unsigned* __restrict__ dims, const float* __restrict__ tscale,
                   const float* __restrict__ xs, unsigned P, unsigned K, float* __restrict__ Y,
                   float* __restrict__ zbuf, float* __restrict__ u2buf, float* __restrict__ ysc,
                   void* __restrict__ ysplit, int nsplit, int f16) {
+    __shared__ unsigned jl[256];
+    __shared__ float zl[256];
     const unsigned p = blockIdx.y;
-    const unsigned k = blockIdx.x * 256 + threadIdx.x;
+    const unsigned kb = blockIdx.x * 256;
+    const unsigned k = kb + threadIdx.x;
     const unsigned d = dims[p];
     const float* X = xs + ((size_t)p * 2 * K + (half ? K : 0)) * kGbD;
     const float* S = xs + ((size_t)p * 2 * K + (half ? 0 : K)) * kGbD;
@@ -272,16 +315,16 @@ gb_propose_kernel(unsigned seed0, int half, unsigned oe, unsigned step, float a,
     const float z = __builtin_fmaf(ca * u.y, u.y, cb * u.y) + inv_a;
     unsigned j = (unsigned)(u.x * (float)K);
     j = j < K ? j : K - 1;
-    const float* xk = X + (size_t)k * kGbD;
-    const float* sj = S + (size_t)j * kGbD;
-    float* yo = Y + ((size_t)p * K + k) * kGbD;
-    const float* t = tscale + (size_t)p * kGbD;
-    if (d <= 32)
-        gb_propose_one<32>(xk, sj, z, d, yo, t, P, p, K, k, ysc, ysplit, nsplit, f16);
-    else
-        gb_propose_one<64>(xk, sj, z, d, yo, t, P, p, K, k, ysc, ysplit, nsplit, f16);
+    jl[threadIdx.x] = j;
+    zl[threadIdx.x] = z;
     zbuf[(size_t)p * K + k] = z;
     u2buf[(size_t)p * K + k] = u.z;
+    __syncthreads();
+    const float* t = tscale + (size_t)p * kGbD;
+    if (d <= 32)
+        gb_propose_pieces<32>(X, S, jl, zl, d, p, K, kb, Y, t, P, ysc, ysplit, nsplit, f16);
+    else
+        gb_propose_pieces<64>(X, S, jl, zl, d, p, K, kb, Y, t, P, ysc, ysplit, nsplit, f16);
 }
 
 // Block -> (unit, 256-point group), XCD-aware as glm_tile: the consecutive

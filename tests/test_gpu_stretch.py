@@ -227,6 +227,27 @@ def test_diag_gaussian_steps_vs_oracle(fac, partner, dim):
     assert abs(r_run["acceptance-rate"] - ref_run["acceptance_rate"]) < 0.01
 
 
+def test_staged_move_large_ensemble_vs_oracle(fac):
+    """The staged move kernels over 2^18 walkers (512 work-groups per half)
+    of the 16-D correlated Gaussian: the same moves as the oracle for all
+    but the few walkers whose accept test sits on the last bit of exp."""
+    dim, W = 16, 1 << 18
+    params = _mvn_params(dim, 7)
+    sf = fac.mcmc_factory(models.mvn_model(dim))
+    s = sf.create_sampler(21, W, params).set_partner(_lib.BH_PARTNER_WALKER).set_exec(STEPWISE)
+    s.init(21)
+    s.init_position(22, np.tile([-5.0, 5.0], dim))
+    ref = O.Stretch(O.MVN_LOGPDF, dim, W, params, 21, params_len=dim + dim * dim,
+                    partner=O.PARTNER_WALKER, threads=8)
+    ref.init(21)
+    ref.init_position(22, np.tile(np.array([[-5.0, 5.0]], np.float32), (dim, 1)))
+    s.burn_in(3, 2.0)
+    ref.burn_in(3, 2.0)
+    x = s.state()[0].reshape(-1, dim)
+    r = ref.xs.reshape(-1, dim)
+    assert np.all(x == r, axis=1).mean() > 0.999
+
+
 def test_anneal_vs_oracle(gaussian_sf):
     sched = lambda i: 10.0 - i  # noqa: E731  (mcmc.clj minus-n)
     s = gaussian_sf.create_sampler(11, 1024, [0.0, 1.0])
