@@ -692,6 +692,34 @@ def c5_problems(P=C5_P, seed=1005):
     return m, lim
 
 
+def c5_cpu_baseline(m, lim, budget_s=12.0):
+    """Walker-steps/s of the oracle's fp64 GLM sampler on the first problems'
+    data (dequantized rows), half-steps of 1024 walkers in turn, within a
+    bounded time (positions only: the initial log posteriors are not
+    computed, the work per half-step is the same)."""
+    import oracle as O
+    threads = _threads()
+    K = C5_W // 2
+    work, p, t = 0.0, 0, 0.0
+    while p < m.problems and (p == 0 or t < budget_s):
+        Xd = np.ascontiguousarray(m.dequantized(p), np.float32)
+        d = Xd.shape[1]
+        g = O.GlmStretch(Xd, m.y[p], C3_PRIOR_SD, C5_W, 9 + 2 * p, partner=O.PARTNER_WALKER,
+                         threads=threads)
+        g.xs[:] = O.init_walkers(9 + p, lim[p, :d], d, C5_W)
+        t0 = time.perf_counter()
+        g._half(9 + 2 * p, 3333, 0, True, A, 1.0)
+        t += time.perf_counter() - t0
+        work += float(K) * Xd.shape[0] * d
+        p += 1
+    # a walker-step of problem p costs ~N_p d_p: the sampled rate in
+    # (row x coordinate) walker-steps per second over the batch's mean N_p d_p
+    mean_cost = float(np.mean(m.rows.astype(np.float64) * m.dims.astype(np.float64)))
+    return {"walker_steps_per_s": work / t / mean_cost, "threads": threads,
+            "sample": f"one half-step of {K} walkers for each of the first {p} problems "
+                      f"({t:.1f} s), scaled by the batch's mean N_p d_p"}
+
+
 def c5_run(fac, m, lim, prec, burn, run, series=None):
     bt = fac.glm_batch(m, C5_W).set_precision(prec)
     bt.init_position(9, lim)
@@ -795,6 +823,13 @@ def main_c5(args):
                      "algorithmic_flops_per_half_step": flops},
         "setup_s": round(t_setup, 1),
     }
+    if not args.no_cpu_baseline:
+        cb = c5_cpu_baseline(m, lim)
+        ess_per_ws = float(ess[ok].sum()) / (m.problems * C5_W * (burn + run))
+        line["cpu_baseline"] = {"value": round(cb["walker_steps_per_s"] * ess_per_ws, 2),
+                                "unit": "ESS/s", "cores": cb["threads"], "kind": "port",
+                                "sample": cb["sample"] + "; ESS per walker-step of the GPU run",
+                                "walker_steps_per_s": round(cb["walker_steps_per_s"], 1)}
     if args.c5_fp32:
         bt32, res32, wall32, _, run32 = c5_run(fac, m, lim, _lib.BH_GLM_FP32, burn, run)
         bt32.release()
@@ -900,7 +935,8 @@ def c3_cpu_baseline(Xd, y, limits, budget_s=20.0):
         g._half(5, 3333, n_half, True, A, 1.0)
         n_half += 1
     t = time.perf_counter() - t0
-    return {"walker_steps_per_s": (W // 2) * n_half / 2 / t, "threads": threads,
+    # each half-step moves W / 2 walkers: W / 2 walker-steps
+    return {"walker_steps_per_s": (W // 2) * n_half / t, "threads": threads,
             "sample": f"{n_half} half-step(s) of {W // 2} walkers over the full "
                       f"{C3_N}x{C3_D} data in {t:.1f} s"}
 
