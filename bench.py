@@ -424,6 +424,28 @@ def c2_job(sf, params, limits, burn, run, prec=None):
     return res, var, wall, run_ms, kt, xs, lpv
 
 
+def c2_cpu_baseline(params, limits, budget_s=12.0):
+    """Walker-steps/s of the oracle's sampler with the linear-regression
+    posterior as a user model (one walker per OpenMP iteration looping over
+    the rows, float as the reference's kernel), whole steps of 1024 walkers
+    within a bounded time."""
+    import oracle as O
+    threads = _threads()
+    W = 1024
+    s = O.Stretch(O.LINREG_POST, C2_D + 1, W, params, 123, data_len=C2_N * (C2_D + 1),
+                  params_len=2, partner=O.PARTNER_WALKER, threads=threads)
+    s.init(123)
+    s.init_position(124, np.asarray(limits, np.float32).reshape(C2_D + 1, 2))
+    t0 = time.perf_counter()
+    n = 0
+    while n == 0 or (time.perf_counter() - t0 < budget_s and n < 200):
+        s.burn_in(1, A)
+        n += 1
+    t = time.perf_counter() - t0
+    return {"walker_steps_per_s": W * n / t, "threads": threads,
+            "sample": f"{n} steps of {W} walkers, the posterior as a user model, in {t:.1f} s"}
+
+
 def main_c2(args):
     from bayadera_amd import _lib, engine, models
     rows, params, bh, limits = c2_problem()
@@ -487,6 +509,13 @@ def main_c2(args):
         line["user_model_path"] = {"steps": n, "ms_per_step": round(run_u / n, 3),
                                    "walker_steps_per_s": round(C2_W * n / (run_u * 1e-3), 1),
                                    "kernel_avg_ms": round(kt_u["avg_ms"], 4)}
+    if not args.no_cpu_baseline:
+        cb = c2_cpu_baseline(params, limits)
+        ess_per_ws = ess_lo / (C2_W * (burn + run))
+        line["cpu_baseline"] = {"value": round(cb["walker_steps_per_s"] * ess_per_ws, 2),
+                                "unit": "ESS/s", "cores": cb["threads"], "kind": "port",
+                                "sample": cb["sample"] + "; ESS per walker-step of the GPU run",
+                                "walker_steps_per_s": round(cb["walker_steps_per_s"], 1)}
     print(json.dumps(line), flush=True)
 
 

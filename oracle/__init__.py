@@ -51,6 +51,7 @@ MVN_LOGPDF = 7
 GAUSSIAN_LOGLIK = 8
 DIAG_GAUSS_LOGPDF = 9
 HBB_LOGPDF = 10
+LINREG_POST = 11
 
 PARTNER_FLAT = 0
 PARTNER_WALKER = 1

@@ -169,6 +169,7 @@ enum {
     BHO_GAUSSIAN_LOGLIK = 8,       /* dist/gaussian.cl:29-42 */
     BHO_DIAG_GAUSS_LOGPDF = 9,     /* independent Gaussian coordinates -sum w_i (x_i-mu_i)^2/2 (benchmark model) */
     BHO_HBB_LOGPDF = 10,           /* hierarchical beta-binomial on logits (batched benchmark model) */
+    BHO_LINREG_POST = 11,          /* Gaussian linear regression with normal priors (benchmark model) */
 };
 
 static const float LOG_SQRT_2PI_F = 0.9189385332046727f;
@@ -255,6 +256,35 @@ BHO_EXPORT float bho_model_logfn(int model, uint32_t data_len, uint32_t params_l
             acc -= (y[g] + a) * spm + (n[g] - y[g] + b) * spp;
         }
         return acc - 0.5f * (la * la + lb * lb) / (s0 * s0);
+    }
+    case BHO_LINREG_POST: {
+        /* x = [beta (dim-1), log sigma]; data rows [x_i (dim-1), y_i]
+         * (data_len / dim rows); hyperparameters [s_beta, s_sigma]:
+         * -1/2 sum ((y_i - x_i.beta) / sigma)^2 - rows (log sigma + log sqrt(2 pi))
+         * - 1/2 (|beta / s_beta|^2 + (log sigma / s_sigma)^2) */
+        const uint32_t d = dim - 1;
+        const uint32_t rows = data_len / dim;
+        const float log_sigma = x[d];
+        const float inv_sigma = expf(-log_sigma);
+        float ss = 0.0f;
+        for (uint32_t i = 0; i < rows; ++i) {
+            const float* row = params + (size_t)i * dim;
+            float eta = 0.0f;
+            for (uint32_t j = 0; j < d; ++j) eta = fmaf(row[j], x[j], eta);
+            const float r = (row[d] - eta) * inv_sigma;
+            ss = fmaf(r, r, ss);
+        }
+        const float ll = -0.5f * ss - (float)rows * (log_sigma + LOG_SQRT_2PI_F);
+        const float* hyper = params + data_len;
+        const float ib = 1.0f / hyper[0];
+        const float is = 1.0f / hyper[1];
+        float q = 0.0f;
+        for (uint32_t j = 0; j < d; ++j) {
+            const float z = x[j] * ib;
+            q = fmaf(z, z, q);
+        }
+        const float zs = x[d] * is;
+        return ll + -0.5f * fmaf(zs, zs, q);
     }
     case BHO_GAUSSIAN_LOGLIK: {
         /* data = params[0 .. data_len), x = (mu, sigma) */

@@ -335,3 +335,37 @@ def test_direct_samplers_vs_oracle_and_goldens(fac, goldens, name, kind):
     assert np.allclose(x, r, rtol=2e-6, atol=2e-6 * scale)
     assert np.allclose(x[:4], g["first4"], rtol=2e-6, atol=0)
     assert np.allclose(x[9996:], g["at9996"], rtol=2e-6, atol=0)
+
+
+def test_linreg_user_model_vs_oracle(fac):
+    """The linear-regression posterior as a user model in the reference's
+    dialect (the plugin path, 17 dimensions, one thread per walker over the
+    rows): the same moves as the oracle's restatement, up to accept tests
+    that sit on the last bit of the model's exp."""
+    rng = np.random.default_rng(31)
+    N, d = 3000, 16
+    X = rng.standard_normal((N, d))
+    y = X @ rng.standard_normal(d) + 0.5 * rng.standard_normal(N)
+    rows = np.concatenate([X, y[:, None]], axis=1).astype(np.float32)
+    params = np.concatenate([rows.reshape(-1), [10.0, 1.0]]).astype(np.float32)
+    lik = models.likelihood_model(models.LINREG_SOURCE, name="linreg")
+    prior = models.distribution_model(models.LINREG_SOURCE, name="linreg_prior",
+                                      dimension=d + 1, params_size=2)
+    sf = fac.mcmc_factory(models.posterior_model(prior, "linreg_post", lik))
+    W = 1024
+    limits = np.tile(np.array([[-2.0, 2.0]], np.float32), (d + 1, 1))
+    s = sf.create_sampler(41, W, params).set_partner(_lib.BH_PARTNER_WALKER)
+    s.init(41)
+    s.init_position(42, limits.reshape(-1))
+    ref = O.Stretch(O.LINREG_POST, d + 1, W, params, 41, data_len=N * (d + 1), params_len=2,
+                    partner=O.PARTNER_WALKER, threads=4)
+    ref.init(41)
+    ref.init_position(42, limits)
+    x0, lf0 = s.state()[0].reshape(-1, d + 1), s.state()[1]
+    assert np.array_equal(x0.reshape(-1), ref.xs)
+    assert np.allclose(lf0, ref.logfn, rtol=2e-5)
+    s.burn_in(4, 2.0)
+    ref.burn_in(4, 2.0)
+    x = s.state()[0].reshape(-1, d + 1)
+    r = ref.xs.reshape(-1, d + 1)
+    assert np.all(x == r, axis=1).mean() > 0.99

@@ -181,3 +181,30 @@ def test_hbb_model_against_its_definition():
                        params_len=1).astype(np.float64)
     want = np.array([ref(x.astype(np.float64)) for x in xs])
     assert np.allclose(got - got[0], want - want[0], rtol=1e-5, atol=0.01)
+
+
+def test_linreg_posterior_against_its_definition():
+    """Gaussian linear regression with normal priors: the oracle's density
+    equals, up to a constant, scipy's normal log-densities of the residuals
+    (sd sigma) plus N(0, s_beta^2) on each coefficient and N(0, s_sigma^2)
+    on log sigma, in float64 (differences between points)."""
+    from scipy import stats
+    rng = np.random.default_rng(12)
+    N, d, sb, ssig = 400, 6, 10.0, 1.0
+    X = rng.standard_normal((N, d))
+    y = X @ rng.standard_normal(d) + 0.5 * rng.standard_normal(N)
+    rows = np.concatenate([X, y[:, None]], axis=1).astype(np.float32)
+    params = np.concatenate([rows.reshape(-1), [sb, ssig]]).astype(np.float32)
+    Xf, yf = rows[:, :d].astype(np.float64), rows[:, d].astype(np.float64)
+
+    def ref(x):
+        beta, ls = x[:d], x[d]
+        lp = stats.norm.logpdf(yf, Xf @ beta, np.exp(ls)).sum()
+        return lp + stats.norm.logpdf(beta, 0, sb).sum() + stats.norm.logpdf(ls, 0, ssig)
+
+    xs = np.concatenate([rng.standard_normal((20, d)), rng.uniform(-1.5, 0.5, (20, 1))],
+                        axis=1).astype(np.float32)
+    got = O.logfn_many(O.LINREG_POST, params, xs, d + 1, data_len=N * (d + 1),
+                       params_len=2).astype(np.float64)
+    want = np.array([ref(x.astype(np.float64)) for x in xs])
+    assert np.allclose(got - got[0], want - want[0], rtol=1e-5, atol=0.05)
