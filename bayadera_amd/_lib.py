@@ -68,6 +68,7 @@ SIGNATURES = {
     "bh_dataset_variance": [_P, _P, _U32, _U64, _U32, _P],
     "bh_dataset_histogram": [_P, _P, _U32, _U64, _U32, _P, _P, _P],
     "bh_acor": [_P, _P, _U32, _U32, _P, _P, _P, _P],
+    "bh_acor_batch": [_P, _P, _U32, _P, _U32, _P, _P, _P, _P],
     "bh_stretch_factory_create": [_P, _S, _S, _U32, _U32, _PP, _P, _SZ],
     "bh_stretch_factory_destroy": [_P],
     "bh_glm_stretch_factory_create": [_P, _I, _P, _P, _U64, _U32, _F, _PP, _P, _SZ],

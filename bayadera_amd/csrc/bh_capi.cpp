@@ -1201,6 +1201,64 @@ int bh_dataset_histogram(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n,
     return sync(ctx->stream);
 }
 
+int bh_acor_batch(bh_ctx* ctx, const float* series, uint32_t problems, const uint32_t* dims,
+                  uint32_t n, float* tau, float* mean, float* sigma, uint32_t* lag) {
+    BH_ARG(ctx && series && dims, "NULL argument");
+    BH_ARG(problems > 0 && n > 0, "empty series");
+    const uint32_t wgs = ctx->wgs;
+    const uint32_t lg = acor_lag(n, wgs);
+    if (lag) *lag = lg;
+    if ((uint64_t)lg * 5 > n)
+        return fail(BH_ERR_RANGE,
+                    "The autocorrelation time is too long relative to the variance. Number of "
+                    "steps (%u) must not be less than %u.",
+                    n, lg * 5);
+    // one column per (problem, coordinate): problem p's n x dims[p] block
+    // starts after those of problems 0 .. p-1
+    std::vector<unsigned long long> off;
+    std::vector<unsigned> stride;
+    uint64_t base = 0;
+    uint32_t per_dim = 0;
+    for (uint32_t p = 0; p < problems; ++p) {
+        BH_ARG(dims[p] > 0, "problem %u: empty series", p);
+        for (uint32_t d = 0; d < dims[p]; ++d) {
+            off.push_back(base + d);
+            stride.push_back(dims[p]);
+        }
+        base += (uint64_t)n * dims[p];
+        const uint32_t wgsm = std::min(std::min(dims[p], 16u), wgs);
+        const uint32_t lnm = wgs / wgsm;
+        per_dim = std::max(per_dim, 2 * ((n + lnm - 1) / lnm + (n + wgs - 1) / wgs + 8));
+    }
+    const uint32_t cols = (uint32_t)off.size();
+    BH_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevBuf copy, scratch, out, doff, dstride;
+    BH_TRY(copy.alloc((size_t)base * sizeof(float)));
+    BH_TRY(scratch.alloc((size_t)per_dim * cols * sizeof(float)));
+    BH_TRY(out.alloc((size_t)3 * cols * sizeof(float)));
+    BH_TRY(doff.alloc(off.size() * sizeof(unsigned long long)));
+    BH_TRY(dstride.alloc(stride.size() * sizeof(unsigned)));
+    BH_HIP(hipMemcpyAsync(copy.p, series, (size_t)base * sizeof(float), hipMemcpyDefault, st));
+    BH_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * sizeof(unsigned long long),
+                          hipMemcpyHostToDevice, st));
+    BH_HIP(hipMemcpyAsync(dstride.p, stride.data(), stride.size() * sizeof(unsigned),
+                          hipMemcpyHostToDevice, st));
+    float* o = out.as<float>();
+    BH_HIP(bh::launch_acor_batch(st, copy.as<float>(), doff.as<unsigned long long>(),
+                                 dstride.as<unsigned>(), cols, n, lg, wgs, scratch.as<float>(),
+                                 per_dim, o, o + cols, o + 2 * cols));
+    std::vector<float> h(3 * (size_t)cols);
+    BH_HIP(hipMemcpyAsync(h.data(), o, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    BH_TRY(sync(st));
+    for (uint32_t b = 0; b < cols; ++b) {
+        if (tau) tau[b] = h[b];
+        if (mean) mean[b] = h[cols + b];
+        if (sigma) sigma[b] = h[2 * (size_t)cols + b];
+    }
+    return BH_OK;
+}
+
 int bh_acor(bh_ctx* ctx, const float* series, uint32_t dim, uint32_t n, float* tau, float* mean,
             float* sigma, uint32_t* lag) {
     BH_ARG(ctx && series, "NULL argument");

@@ -391,17 +391,18 @@ sum_f64_kernel(const double* __restrict__ v, unsigned n, double* __restrict__ ou
 // series: n x dim (series[t*dim + d]), a device copy the kernel centres and
 // pair-sums in place.  scratch: >= 2*(ceil(n/wgs)+1) floats per dimension.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, float inv_n,
-            unsigned local_n_mean, float* __restrict__ scratch, unsigned scratch_per_dim,
-            float* __restrict__ tau, float* __restrict__ mean, float* __restrict__ sigma) {
+// One column: element t at col[t*cs]; its results go to tau[0], mean[0],
+// sigma[0]; c0acc: scratch_per_dim floats of scratch.
+__device__ __forceinline__ void acor_column(float* __restrict__ col, unsigned cs, unsigned n,
+                                            unsigned lag, float inv_n, unsigned local_n_mean,
+                                            float* __restrict__ c0acc, unsigned scratch_per_dim,
+                                            float* __restrict__ tau, float* __restrict__ mean,
+                                            float* __restrict__ sigma) {
     const unsigned wgs = blockDim.x;
-    const unsigned d = blockIdx.x;
     const unsigned l = threadIdx.x;
     const unsigned win_mult = 5, min_lag = 10;
     __shared__ float lacc[256];
     __shared__ float lm[512];
-    float* c0acc = scratch + (size_t)d * scratch_per_dim;
     float* dacc = c0acc + scratch_per_dim / 2;
 
     // mean: enq-reduce! with local (min(16,dim), wgs/that) along n, scal! 1/n
@@ -414,7 +415,7 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
                 const unsigned start = g * local_n_mean;
                 const unsigned len = (c - start < local_n_mean) ? c - start : local_n_mean;
                 float v = 0.0f;
-                if (l < len) v = first ? series[(size_t)(start + l) * dim + d] : c0acc[start + l];
+                if (l < len) v = first ? col[(size_t)(start + l) * cs] : c0acc[start + l];
                 const float s = group_tree(v, len, lacc);
                 if (l == 0) c0acc[g] = s;
                 __syncthreads();
@@ -426,7 +427,7 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
     }
     const float mu = c0acc[0] * inv_n;
     __syncthreads();
-    for (unsigned t = l; t < n; t += wgs) series[(size_t)t * dim + d] -= mu;
+    for (unsigned t = l; t < n; t += wgs) col[(size_t)t * cs] -= mu;
     __syncthreads();
 
     // one acor_1d level (acor.cl:26-86) over gsize points at stride
@@ -444,8 +445,8 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
                 const unsigned gid = start + l;
                 const bool compute = gid + lg < gsize;
                 const bool load_lag = (l < lg) && has_next && (gid + lsize < gsize);
-                lm[l] = compute ? series[(size_t)gid * stride + d] : 0.0f;
-                lm[l + lsize] = load_lag ? series[(size_t)(gid + lsize) * stride + d] : 0.0f;
+                lm[l] = compute ? col[(size_t)gid * stride] : 0.0f;
+                lm[l + lsize] = load_lag ? col[(size_t)(gid + lsize) * stride] : 0.0f;
             }
             __syncthreads();
             float xv = 0.0f, xacc = 0.0f;
@@ -482,7 +483,7 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
         dd = group_tree(b, lsize, lacc);
     };
 
-    unsigned groups = level(dim, lag, n, wgs);
+    unsigned groups = level(cs, lag, n, wgs);
     float c0, dd;
     reduce(groups, c0, dd);
     unsigned stride = 1, cur_lag = lag, prev_n = n;
@@ -495,9 +496,9 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
         if (min_lag < lag2 && (float)lag2 < tv * (float)win_mult) {
             const unsigned n2 = prev_n / 2;
             const unsigned w2 = wgs < n2 ? wgs : n2;
-            const size_t step = (size_t)stride * dim;
+            const size_t step = (size_t)stride * cs;
             __syncthreads();
-            for (unsigned g = l; g < n2; g += wgs) series[d + 2 * g * step] += series[d + (2 * (size_t)g + 1) * step];
+            for (unsigned g = l; g < n2; g += wgs) col[2 * g * step] += col[(2 * (size_t)g + 1) * step];
             __syncthreads();
             groups = level((unsigned)(2 * step), lag2, n2, w2);
             float c0n, dn;
@@ -512,13 +513,42 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
         } else {
             if (l == 0) {
                 const float scale = (float)stride * (float)(prev_n - lag2);
-                tau[d] = dd * (float)(n - cur_lag) / (scale * start_c0);
-                sigma[d] = sqrtf(dd / (scale * (float)n));
-                mean[d] = mu;
+                tau[0] = dd * (float)(n - cur_lag) / (scale * start_c0);
+                sigma[0] = sqrtf(dd / (scale * (float)n));
+                mean[0] = mu;
             }
             break;
         }
     }
+}
+
+__global__ void __launch_bounds__(256)
+acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, float inv_n,
+            unsigned local_n_mean, float* __restrict__ scratch, unsigned scratch_per_dim,
+            float* __restrict__ tau, float* __restrict__ mean, float* __restrict__ sigma) {
+    const unsigned d = blockIdx.x;
+    acor_column(series + d, dim, n, lag, inv_n, local_n_mean,
+                scratch + (size_t)d * scratch_per_dim, scratch_per_dim, tau + d, mean + d,
+                sigma + d);
+}
+
+// Many series of n steps in one launch, one block per column: block b takes
+// the column starting at series + col_off[b] with stride col_stride[b] (its
+// series' dimension, which also sets the mean's reduction shape as in
+// acor_kernel), results at index b.
+__global__ void __launch_bounds__(256)
+acor_batch_kernel(float* __restrict__ series, const unsigned long long* __restrict__ col_off,
+                  const unsigned* __restrict__ col_stride, unsigned n, unsigned lag, float inv_n,
+                  float* __restrict__ scratch, unsigned scratch_per_dim, float* __restrict__ tau,
+                  float* __restrict__ mean, float* __restrict__ sigma) {
+    const unsigned b = blockIdx.x;
+    const unsigned cs = col_stride[b];
+    const unsigned wgs = blockDim.x;
+    unsigned wgsm = cs < 16 ? cs : 16;
+    if (wgs < wgsm) wgsm = wgs;
+    acor_column(series + col_off[b], cs, n, lag, inv_n, wgs / wgsm,
+                scratch + (size_t)b * scratch_per_dim, scratch_per_dim, tau + b, mean + b,
+                sigma + b);
 }
 
 // --------------------------------------------------------------------------
@@ -628,6 +658,16 @@ hipError_t launch_sum_u32(hipStream_t s, const unsigned* v, unsigned n, unsigned
 
 hipError_t launch_sum_f64(hipStream_t s, const double* v, unsigned n, double* out) {
     sum_f64_kernel<<<1, 256, 0, s>>>(v, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_acor_batch(hipStream_t s, float* series, const unsigned long long* col_off,
+                             const unsigned* col_stride, unsigned cols, unsigned n, unsigned lag,
+                             unsigned wgs, float* scratch, unsigned scratch_per_dim, float* tau,
+                             float* mean, float* sigma) {
+    const float inv_n = (float)(1.0 / (double)n);
+    acor_batch_kernel<<<cols, wgs, 0, s>>>(series, col_off, col_stride, n, lag, inv_n, scratch,
+                                           scratch_per_dim, tau, mean, sigma);
     return hipGetLastError();
 }
 

@@ -27,6 +27,10 @@ hipError_t launch_row_tree_sum(hipStream_t s, const float* a, unsigned rows, uns
                                float scale, float* out, unsigned long long out_stride);
 hipError_t launch_sum_u32(hipStream_t s, const unsigned* v, unsigned n, unsigned long long* out);
 hipError_t launch_sum_f64(hipStream_t s, const double* v, unsigned n, double* out);
+hipError_t launch_acor_batch(hipStream_t s, float* series, const unsigned long long* col_off,
+                             const unsigned* col_stride, unsigned cols, unsigned n, unsigned lag,
+                             unsigned wgs, float* scratch, unsigned scratch_per_dim, float* tau,
+                             float* mean, float* sigma);
 hipError_t launch_acor(hipStream_t s, float* series, unsigned dim, unsigned n, unsigned lag,
                        unsigned wgs, float* scratch, unsigned scratch_per_dim, float* tau,
                        float* mean, float* sigma);

@@ -317,6 +317,24 @@ class HipAcorEngine:
              sigma.ctypes.data, ctypes.byref(lag))
         return Autocorrelation(tau, mean, sigma, n, lag.value)
 
+    def acor_batch(self, series, dims, n: int) -> list:
+        """acor of many series in one launch: series holds series p as an
+        (n, dims[p]) block after those of series 0 .. p-1 (a batch's
+        recorded run); one Autocorrelation per series."""
+        a = _Arr(series)
+        dv = np.ascontiguousarray(dims, np.uint32)
+        total = int(dv.sum())
+        tau = np.zeros(total, np.float32)
+        mean = np.zeros(total, np.float32)
+        sigma = np.zeros(total, np.float32)
+        lag = ctypes.c_uint32()
+        call("bh_acor_batch", self.factory.handle, a.ptr, len(dv), dv.ctypes.data, int(n),
+             tau.ctypes.data, mean.ctypes.data, sigma.ctypes.data, ctypes.byref(lag))
+        offs = np.concatenate([[0], np.cumsum(dv.astype(np.int64))])
+        return [Autocorrelation(tau[offs[p]:offs[p + 1]], mean[offs[p]:offs[p + 1]],
+                                sigma[offs[p]:offs[p + 1]], int(n), lag.value)
+                for p in range(len(dv))]
+
 
 # ---------------------------------------------------------------------------
 class HipStretchFactory(_Handle):

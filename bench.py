@@ -629,8 +629,9 @@ def main_c4(args):
     tau_max = np.zeros(len(dims))
     print("bench: acor of every problem", file=sys.stderr, flush=True)
     acor_ok = np.zeros(len(dims), bool)
+    acs = acor.acor_batch(series, dims, run)  # one launch for all problems
     for b, blk in enumerate(blocks):
-        ac = acor.acor(blk)
+        ac = acs[b]
         var = res["sd"][b, :dims[b]] ** 2
         sig2 = np.square(ac.sigma.astype(np.float64))
         tau = ac.tau.astype(np.float64)
@@ -790,9 +791,10 @@ def main_c5(args):
     sd_bias = np.zeros(m.problems)
     lp_err = np.zeros(m.problems)
     acor_ok = np.zeros(m.problems, bool)
+    acs = acor.acor_batch(series, dims, run)  # one launch for all problems
     for p, blk in enumerate(bt.series_blocks(series, run)):
         d = int(dims[p])
-        ac = acor.acor(blk)
+        ac = acs[p]
         var = res["sd"][p, :d] ** 2
         sig2 = np.square(ac.sigma.astype(np.float64))
         acor_ok[p] = bool(np.all(np.isfinite(sig2)))

@@ -169,6 +169,14 @@ int bh_dataset_histogram(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n,
 /* ---------------------------------------------------------------------- */
 int bh_acor(bh_ctx* ctx, const float* series, uint32_t dim, uint32_t n, float* tau, float* mean,
             float* sigma, uint32_t* lag);
+/* acor of many series of n steps in one launch (no reference counterpart;
+ * each series is what bh_acor computes for it alone, amd_gcn.clj:216-267):
+ * series p is an n x dims[p] block (row-major, step by step) after those of
+ * series 0 .. p-1 (the layout bh_batch_run_series records); tau, mean and
+ * sigma receive dims[0] + .. + dims[problems-1] values in the same order.
+ * BH_ERR_RANGE when n < 5 * lag, as bh_acor. */
+int bh_acor_batch(bh_ctx* ctx, const float* series, uint32_t problems, const uint32_t* dims,
+                  uint32_t n, float* tau, float* mean, float* sigma, uint32_t* lag);
 
 /* ---------------------------------------------------------------------- */
 /* Stretch sampler factory: gcn-stretch-factory (amd_gcn.clj:623-631) with  */

@@ -139,3 +139,9 @@ def test_batch_state_calls_reject_null():
     assert lib.bh_batch_set_state(None, _lib.BH_BATCH_STATE_GLOBAL) == _lib.BH_ERR_ARG
     n = ctypes.c_uint32(0)
     assert lib.bh_batch_class_states(None, ctypes.byref(n), None) == _lib.BH_ERR_ARG
+
+
+def test_acor_batch_rejects_null():
+    lib = _lib.load()
+    dims = (ctypes.c_uint32 * 1)(4)
+    assert lib.bh_acor_batch(None, None, 1, dims, 100, None, None, None, None) == _lib.BH_ERR_ARG

@@ -205,3 +205,33 @@ def test_batch_hbb_problems_equal_single_oracle_samplers(fac, where):
         assert same > 0.97, (b, dims[b], same)
         close = np.abs(x - r) <= 1e-5 * np.maximum(1.0, np.abs(r))
         assert np.all(close[np.all(x == r, axis=1)])
+
+
+def test_acor_batch_equals_acor_per_series(fac):
+    """acor of every problem's recorded mean series in one launch equals the
+    engine's acor of each series alone, bit for bit (DIM 7-52, and 1)."""
+    import torch
+    B, W, GMAX = 8, 256, 50
+    dims, params, lim = _hbb_problems(B, 21, GMAX)
+    bt = fac.batch(models.hbb_model(GMAX), params, dims, [1] * B, W)
+    bt.init_position(3, lim)
+    bt.burn_in(5, 50, 2.0)
+    n = 300
+    series = torch.zeros(n * int(np.sum(dims)), dtype=torch.float32, device="cuda")
+    bt.run(6, n, 2.0, series=series)
+    acor = fac.acor_engine()
+    batch = acor.acor_batch(series, dims, n)
+    for b, blk in enumerate(bt.series_blocks(series, n)):
+        one = acor.acor(blk.contiguous())
+        for f in ("tau", "mean", "sigma"):
+            assert np.array_equal(getattr(one, f), getattr(batch[b], f), equal_nan=True), (b, f)
+        assert one.lag == batch[b].lag
+    bt.release()
+    # one-dimensional series too
+    x = np.cumsum(np.random.default_rng(4).standard_normal((400, 3)), axis=0).astype(np.float32)
+    joined = np.concatenate([x[:, :1].reshape(-1), x[:, 1:].reshape(-1)])
+    res = acor.acor_batch(joined, [1, 2], 400)
+    for got, cols in zip(res, (x[:, :1], x[:, 1:])):
+        want = acor.acor(np.ascontiguousarray(cols))
+        assert np.array_equal(want.tau, got.tau, equal_nan=True)
+        assert np.array_equal(want.sigma, got.sigma, equal_nan=True)
