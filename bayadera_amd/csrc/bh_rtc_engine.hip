@@ -616,7 +616,7 @@ __device__ __forceinline__ void bh_resident_impl(
     float* __restrict__ logfn, unsigned K, float a, float beta, const float* __restrict__ betas,
     unsigned step0, unsigned nsteps, int mode, unsigned* __restrict__ accept,
     float* __restrict__ means, float* __restrict__ sample_out, unsigned long long sample_floats) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     const unsigned W = 2 * K;
     const unsigned nb = blockDim.x;
     const unsigned t = threadIdx.x;
@@ -665,7 +665,15 @@ __device__ __forceinline__ void bh_resident_impl(
         if (sample_out && written < sample_floats) {
             const unsigned long long left = sample_floats - written;
             const unsigned chunk = left < (unsigned long long)W * DIM ? (unsigned)left : W * DIM;
-            for (unsigned i = t; i < chunk; i += nb) sample_out[written + i] = sx[i];
+            float* dst = sample_out + written;
+            if (((unsigned long long)dst & 15ull) == 0 && chunk % 4 == 0) {
+                // 16-byte pieces (the LDS ensemble starts 16-byte aligned)
+                const float4* src4 = reinterpret_cast<const float4*>(sx);
+                float4* dst4 = reinterpret_cast<float4*>(dst);
+                for (unsigned i = t; i < chunk / 4; i += nb) dst4[i] = src4[i];
+            } else {
+                for (unsigned i = t; i < chunk; i += nb) dst[i] = sx[i];
+            }
             written += chunk;
         }
     }
