@@ -1146,8 +1146,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
-                    help="only the move-bare! roofline runs (DIM 4 and 16), plus a 256 MiB "
-                         "device copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
+                    help="only the move-bare! roofline runs (correlated Gaussian at DIM 4 "
+                         "and 16, independent coordinates at DIM 32), plus a 256 MiB device "
+                         "copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
     if 0 < args.c2_user_steps < 50:
         ap.error("--c2-user-steps: run-sampler! needs at least 50 steps (0 skips the run)")
