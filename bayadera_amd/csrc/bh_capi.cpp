@@ -371,11 +371,20 @@ constexpr size_t kKernelTimerPairs = 1024;
 // rounds of work-groups per logistic likelihood launch
 constexpr unsigned kGlmRounds = 4;
 
-// Guard fallback: walkers re-evaluated in fp64 per half-step at most, and
-// the row ranges of that evaluation.
-constexpr unsigned kGuardListMax = 1024;
-constexpr unsigned kGuardRanges = 1024;
+// Guarded accept test: the list holds every walker of a half (K), so no
+// walker inside the guard is ever decided on reduced-precision values; the
+// fp64 pass over the listed points uses up to kGuardRangesMax row ranges,
+// fewer for very large halves so that its partial sums (ranges x 2K doubles)
+// stay within kGuardPartialMax doubles.
+constexpr unsigned kGuardRangesMax = 1024;
+constexpr size_t kGuardPartialMax = (size_t)1 << 25;  // 256 MB of doubles
 constexpr unsigned kGuardResolveBlocks = 16;
+
+static unsigned guard_ranges(unsigned k_half) {
+    size_t r = kGuardPartialMax / (2 * (size_t)k_half);
+    r = std::min<size_t>(r, kGuardRangesMax);
+    return (unsigned)std::max<size_t>(r, 8);
+}
 
 namespace {
 
@@ -596,7 +605,7 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         // screen, fp64 evaluation of the listed walkers, their decisions
         flag = s->gflag.as<unsigned>();
         unsigned* list = s->glist.as<unsigned>();
-        unsigned lmax = kGuardListMax;
+        unsigned lmax = K;
         double guard = s->glm_guard * std::max(1.0f, beta);
         const double* lpc = lp;
         double* pts64 = s->gpts64.as<double>();
@@ -606,9 +615,9 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         BH_TRY(launch(f->glm_screen, K / s->wgs, s->wgs, 0, s->stream, sa));
         double* part64 = s->gpart64.as<double>();
         BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, pts64, gcount, lmax,
-                                        kGuardRanges, part64));
+                                        guard_ranges(K), part64));
         const double* p64 = part64;
-        unsigned nr64 = kGuardRanges, stride = 2 * kGuardListMax;
+        unsigned nr64 = guard_ranges(K), stride = 2 * K;
         const double* lin64 = glm_lin(g, BH_GLM_FP64);
         const unsigned* listc = list;
         const unsigned* countc = gcount;
@@ -652,11 +661,11 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
         BH_TRY(s->gu2.alloc((size_t)k_half * sizeof(float)));
         BH_TRY(s->gflag.alloc((size_t)k_half * sizeof(unsigned)));
         BH_TRY(s->glpy.alloc((size_t)k_half * sizeof(double)));
-        BH_TRY(s->glist.alloc((size_t)kGuardListMax * sizeof(unsigned)));
+        BH_TRY(s->glist.alloc((size_t)k_half * sizeof(unsigned)));
         BH_TRY(s->gcount.alloc(sizeof(unsigned)));
         BH_TRY(s->gtotals.alloc(2 * sizeof(unsigned long long)));
-        BH_TRY(s->gpart64.alloc((size_t)kGuardRanges * 2 * kGuardListMax * sizeof(double)));
-        BH_TRY(s->gpts64.alloc((size_t)2 * kGuardListMax * dim * sizeof(double)));
+        BH_TRY(s->gpart64.alloc((size_t)guard_ranges(k_half) * 2 * k_half * sizeof(double)));
+        BH_TRY(s->gpts64.alloc((size_t)2 * k_half * dim * sizeof(double)));
         BH_HIP(hipMemsetAsync(s->gtotals.p, 0, 2 * sizeof(unsigned long long), s->stream));
     }
     return BH_OK;

@@ -325,11 +325,11 @@ __device__ float group_tree(float v, unsigned lsize, float* lacc) {
 // sum_reduction_horizontal, amd_gcn.clj:437-438): groups of local_n
 // consecutive values reduce with the tree, the partials reduce again until
 // one is left.  One block per row; work holds a row-sized scratch area.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kMaxGroupThreads)
 row_tree_sum_kernel(const float* __restrict__ a, unsigned count, unsigned local_n,
                     unsigned long long row_stride, float* __restrict__ work, float scale,
                     float* __restrict__ out, unsigned long long out_stride) {
-    __shared__ float lacc[256];
+    __shared__ float lacc[kMaxGroupThreads];
     const unsigned row = blockIdx.x;
     const float* src = a + row * row_stride;
     float* w = work + (size_t)row * ((count + local_n - 1) / local_n + 1);
@@ -401,8 +401,8 @@ __device__ __forceinline__ void acor_column(float* __restrict__ col, unsigned cs
     const unsigned wgs = blockDim.x;
     const unsigned l = threadIdx.x;
     const unsigned win_mult = 5, min_lag = 10;
-    __shared__ float lacc[256];
-    __shared__ float lm[512];
+    __shared__ float lacc[kMaxGroupThreads];
+    __shared__ float lm[2 * kMaxGroupThreads];
     float* dacc = c0acc + scratch_per_dim / 2;
 
     // mean: enq-reduce! with local (min(16,dim), wgs/that) along n, scal! 1/n
@@ -522,7 +522,7 @@ __device__ __forceinline__ void acor_column(float* __restrict__ col, unsigned cs
     }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kMaxGroupThreads)
 acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, float inv_n,
             unsigned local_n_mean, float* __restrict__ scratch, unsigned scratch_per_dim,
             float* __restrict__ tau, float* __restrict__ mean, float* __restrict__ sigma) {
@@ -536,7 +536,7 @@ acor_kernel(float* __restrict__ series, unsigned dim, unsigned n, unsigned lag, 
 // the column starting at series + col_off[b] with stride col_stride[b] (its
 // series' dimension, which also sets the mean's reduction shape as in
 // acor_kernel), results at index b.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kMaxGroupThreads)
 acor_batch_kernel(float* __restrict__ series, const unsigned long long* __restrict__ col_off,
                   const unsigned* __restrict__ col_stride, unsigned n, unsigned lag, float inv_n,
                   float* __restrict__ scratch, unsigned scratch_per_dim, float* __restrict__ tau,
@@ -646,8 +646,10 @@ hipError_t launch_moments(hipStream_t s, const float* a, unsigned m, unsigned lo
 hipError_t launch_row_tree_sum(hipStream_t s, const float* a, unsigned rows, unsigned count,
                                unsigned local_n, unsigned long long row_stride, float* work,
                                float scale, float* out, unsigned long long out_stride) {
-    row_tree_sum_kernel<<<rows, 256, 0, s>>>(a, count, local_n, row_stride, work, scale, out,
-                                             out_stride);
+    // the tree of a group of local_n values needs local_n threads (local_n <= WGS)
+    const unsigned threads = local_n > 256 ? local_n : 256;
+    row_tree_sum_kernel<<<rows, threads, 0, s>>>(a, count, local_n, row_stride, work, scale, out,
+                                                 out_stride);
     return hipGetLastError();
 }
 

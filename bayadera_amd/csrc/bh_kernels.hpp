@@ -5,6 +5,9 @@
 namespace bh {
 
 constexpr unsigned kMaxDatasetBlocks = 2048;  // 8 blocks per CU over 256 CUs
+// largest work-group the context accepts (WGS); the reduction trees of acor and
+// the per-step means are sized for it
+constexpr unsigned kMaxGroupThreads = 1024;
 
 hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits, unsigned dim,
                                unsigned walkers, float* xs);

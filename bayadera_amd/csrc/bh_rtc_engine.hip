@@ -675,6 +675,10 @@ __device__ __forceinline__ void bh_resident_impl(
                 for (unsigned i = t; i < chunk; i += nb) dst[i] = sx[i];
             }
             written += chunk;
+            // the next step's first half writes accepted proposals into sx:
+            // every wave finishes copying this step's ensemble first (the
+            // condition is uniform over the work-group)
+            __syncthreads();
         }
     }
     for (unsigned i = t; i < W * DIM; i += nb) xs[i] = sx[i];
@@ -880,6 +884,9 @@ __device__ __forceinline__ void bh_batch_impl(
                     acc2[d] += (double)s2;
                     if (series) series[soff[b] + (size_t)st * dim + d] = s1 / (float)W;
                 }
+                // the next step's first half writes accepted proposals into
+                // sx, which these sums read
+                __syncthreads();
             }
         }
     }
@@ -1123,8 +1130,9 @@ __device__ __forceinline__ bool bh_glm_decide(double lpy, double lpx, float z, f
 // a monotone step in d = lp(Y) - lp(X); a walker whose log margin
 //   m = log u2 - (DIM-1) log z - beta d
 // is within `guard` of zero could be decided differently by an error of that
-// size in d, so it is listed (up to lmax walkers; the rest count as
-// overflow) for an fp64 evaluation of lp(Y) and lp(X) and decided on those.
+// size in d, so it is listed (up to lmax walkers, the rest counting as
+// overflow; the engine passes lmax = K, so none overflow) for an fp64
+// evaluation of lp(Y) and lp(X) and decided on those.
 // flag[k]: 0 decided here, 3 listed (bh_glm_resolve turns it into 1 = reject
 // or 2 = accept).
 extern "C" __global__ void __launch_bounds__(WGS)

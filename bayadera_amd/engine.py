@@ -50,10 +50,31 @@ class _Arr:
 
 
 def _out(out, shape, dtype=np.float32):
-    if out is not None:
-        return out, _Arr(out, dtype)
-    a = np.zeros(shape, dtype)
-    return a, _Arr(a, dtype)
+    """The caller's ``out`` (written in place by the C side) or a new array.
+    ``out`` must already have the dtype, be contiguous and hold at least the
+    elements the call writes: a conversion would write into a temporary copy
+    and a short buffer would be overrun."""
+    if out is None:
+        a = np.zeros(shape, dtype)
+        return a, _Arr(a, dtype)
+    need = int(np.prod(shape))
+    if hasattr(out, "data_ptr") and hasattr(out, "is_cuda"):
+        import torch
+        want = {np.float32: torch.float32, np.float64: torch.float64,
+                np.uint32: torch.int32}[dtype]
+        ok_dtype, contiguous, size = out.dtype == want, out.is_contiguous(), out.numel()
+    else:
+        if not isinstance(out, np.ndarray):
+            raise ValueError("out must be a numpy array or a torch tensor")
+        ok_dtype = out.dtype == np.dtype(dtype)
+        contiguous, size = out.flags.c_contiguous, out.size
+    if not ok_dtype:
+        raise ValueError(f"out has dtype {out.dtype}, needs {np.dtype(dtype).name}")
+    if not contiguous:
+        raise ValueError("out must be contiguous")
+    if size < need:
+        raise ValueError(f"out holds {size} elements, the call writes {need}")
+    return out, _Arr(out, dtype)
 
 
 @dataclass

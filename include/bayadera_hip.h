@@ -224,11 +224,13 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision);
  * within guard * max(1, beta) of zero is decided on fp64 log posteriors of
  * Y and X instead; while the reduced-precision error of lp(Y) - lp(X) stays
  * below the guard, every decision equals the fp64 sampler's.  guard = 0
- * (the default) turns it off; at most 1024 walkers per half-step are
- * re-evaluated, the rest count as overflow. */
+ * (the default) turns it off.  Every walker inside the guard is
+ * re-evaluated (the list holds a whole half-ensemble), so no decision inside
+ * the guard falls back to the reduced precision. */
 int bh_stretch_set_glm_guard(bh_stretch* s, double guard);
-/* Walkers re-evaluated in fp64 and walkers past the limit, summed over all
- * guarded half-steps of the sampler. */
+/* Walkers re-evaluated in fp64, summed over all guarded half-steps of the
+ * sampler; overflow counts listed walkers left to the reduced precision,
+ * which the whole-half list keeps at 0. */
 int bh_stretch_glm_guard_stats(bh_stretch* s, uint64_t* listed, uint64_t* overflow);
 /* Info.info: {walker-count, iteration-counter} (amd_gcn.clj:321-324) */
 int bh_stretch_info(const bh_stretch* s, uint32_t* walkers, uint64_t* iteration_counter);

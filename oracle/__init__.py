@@ -126,7 +126,9 @@ def lib():
 
 
 def set_variant(flags: int) -> None:
-    """Planted-mistake switch (tests only): bit 0 un-fuses z, bit 1 un-fuses Y."""
+    """Planted-mistake switch (tests only): bit 0 un-fuses z, bit 1 un-fuses Y,
+    bit 2 drops the logistic posterior's prior, bit 3 flips the sign of its
+    linear predictor."""
     lib().bho_set_variant(int(flags))
 
 

@@ -783,10 +783,12 @@ BHO_EXPORT double bho_glm_logpost(const float* Xd, const float* y, uint64_t N, u
     for (uint64_t i = 0; i < N; ++i) {
         double eta = 0.0;
         for (uint32_t j = 0; j < d; ++j) eta = fma((double)Xd[i * d + j], (double)beta[j], eta);
+        if (g_variant & 8) eta = -eta; /* planted mistake (tests only) */
         ll += (double)y[i] * eta - (fmax(eta, 0.0) + log1p(exp(-fabs(eta))));
     }
     double q = 0.0;
     for (uint32_t j = 0; j < d; ++j) q = fma((double)beta[j], (double)beta[j], q);
+    if (g_variant & 4) q = 0.0; /* planted mistake (tests only) */
     return ll - 0.5 * prior_prec * q;
 }
 

@@ -279,3 +279,34 @@ def test_glm_gaussian_sampler_state_at_8192_walkers(fac):
     ref = _linreg_lp64(X, y, xs[idx])
     assert np.allclose(lp[idx], ref, rtol=2e-6)
     s.release()
+
+
+def test_glm_guard_full_shape_equals_fp64(fac):
+    """The logistic workload at its full shape (1,000,000 x 64 rows, 16384
+    walkers): ten guarded fp16x2 steps take exactly the fp64 sampler's
+    positions, nothing overflows the guard's list, and the fp16x2 and fp64
+    log posteriors of a few walkers agree with the oracle's."""
+    N, d, W = 1_000_000, 64, 16384
+    print("full-shape guard: 1M x 64, 16384 walkers, 10 steps", flush=True)
+    Xb, Xd, y = _problem(N, d, 1003)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=5.0))
+    out = {}
+    for prec, guard in ((_lib.BH_GLM_FP64, 0.0), (_lib.BH_GLM_FP16X2, 1e-2)):
+        s = sf.create_sampler(31, W, np.zeros(1, np.float32)).set_glm_precision(prec)
+        s.set_partner(_lib.BH_PARTNER_WALKER)
+        if guard:
+            s.set_glm_guard(guard)
+        s.init(31)
+        s.init_position(32, np.tile(np.array([-0.3, 0.3], np.float32), d))
+        s.burn_in(10, 2.0)
+        out[prec] = (s.state()[0].copy(), s.glm_guard_stats())
+        s.release()
+    x64, _ = out[_lib.BH_GLM_FP64]
+    xg, st = out[_lib.BH_GLM_FP16X2]
+    print("guard stats", st, flush=True)
+    assert st["overflow"] == 0 and st["listed"] > 0
+    assert np.array_equal(xg, x64)
+    pts = np.ascontiguousarray(x64[:4])
+    ref = O.glm_logpost(Xd, y, 5.0, pts)
+    assert np.max(np.abs(sf.log_density(pts, _lib.BH_GLM_FP64) - ref) / np.abs(ref)) < 1e-12
+    assert np.max(np.abs(sf.log_density(pts, _lib.BH_GLM_FP16X2) - ref) / np.abs(ref)) < 2e-6

@@ -8,7 +8,8 @@ import math
 import numpy as np
 import pytest
 
-from bayadera_amd import engine, library
+import oracle as O
+from bayadera_amd import engine, library, mcmc, models
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +107,9 @@ def test_beta_bernoulli(lib):
     assert abs(ev / library.beta_bernoulli_evidence(a, b, N, z) - 1.0) < 1e-3
     post = lib.posterior(lik, prior)(data)
     s = post.sampler(15)
+    # (doto (sampler post-dist) (mix!)), library_test.clj:135
+    mixed = mcmc.mix(s)
+    assert 0.2 <= mixed["acc-rate"] <= 0.5 or mixed["a"] != 2.0
     x = s.sample()
     real = lib.beta(a + z, b + N - z)
     m, v = _moments(lib, x)
@@ -118,3 +122,43 @@ def test_beta_bernoulli(lib):
     assert roughly100(pdf.mean(), rpdf.mean())
     for h in (s, ps, rsamp, le):
         h.release()
+
+
+class _OracleSampler:
+    """The oracle's stretch sampler with the model() the mix! host loop reads."""
+
+    def __init__(self, stretch, dimension):
+        self.s, self.dimension = stretch, dimension
+
+    def model(self):
+        return self
+
+    def anneal(self, schedule, n, a):
+        return self.s.anneal(schedule, n, a)
+
+    def acc_rate(self, a):
+        return self.s.acc_rate(a)
+
+    def burn_in(self, n, a):
+        return self.s.burn_in(n, a)
+
+
+def test_mix_uniform_hip_equals_oracle(lib):
+    """mix! (mcmc.clj:66-101) on the uniform model through the HIP sampler
+    and through the oracle: the uniform model's accept decisions are exact,
+    so every acceptance rate, the tuned a and the final ensemble agree."""
+    W, seed = 4096, 21
+    sf = lib.factory.mcmc_factory(models.uniform_model())
+    s = sf.create_sampler(seed, W, [-1, 2])
+    s.init(seed)
+    s.init_position(seed + 1, [-1, 2])
+    got = mcmc.mix(s)
+    ref = O.Stretch(O.UNIFORM_LOGPDF, 1, W, [-1, 2], seed, params_len=2, threads=4)
+    ref.init(seed)
+    ref.init_position(seed + 1, [-1, 2])
+    want = mcmc.mix(_OracleSampler(ref, 1))
+    assert got == want
+    assert got["a"] != 2.0  # the loop moved a
+    assert np.array_equal(s.state()[0].reshape(-1), ref.xs)
+    s.release()
+    sf.release()

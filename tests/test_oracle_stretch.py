@@ -208,3 +208,108 @@ def test_linreg_posterior_against_its_definition():
                        params_len=2).astype(np.float64)
     want = np.array([ref(x.astype(np.float64)) for x in xs])
     assert np.allclose(got - got[0], want - want[0], rtol=1e-5, atol=0.05)
+
+
+# --------------------------------------------------------------------------
+# The logistic-regression posterior (the checker of the GLM samplers)
+# --------------------------------------------------------------------------
+def _logistic_problem(seed=21, N=600, d=8):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((N, d)) / np.sqrt(d)).astype(np.float32)
+    beta_true = rng.standard_normal(d)
+    y = (rng.uniform(size=N) < 1.0 / (1.0 + np.exp(-(X.astype(np.float64) @ beta_true))))
+    pts = (beta_true + 0.5 * rng.standard_normal((16, d))).astype(np.float32)
+    return X, y.astype(np.float32), pts
+
+
+def _logistic_scipy(X, y, s, pts):
+    """Bernoulli log-likelihood of sigmoid(X beta) plus N(0, s^2) log-priors,
+    from scipy's distributions in float64 (normalising constants included;
+    compare differences between points)."""
+    from scipy import special, stats
+    Xd = X.astype(np.float64)
+    out = []
+    for b in pts.astype(np.float64):
+        out.append(stats.bernoulli.logpmf(y.astype(np.int64), special.expit(Xd @ b)).sum()
+                   + stats.norm.logpdf(b, 0.0, s).sum())
+    return np.array(out)
+
+
+def test_glm_logpost_against_scipy():
+    X, y, pts = _logistic_problem()
+    got = O.glm_logpost(X, y, 5.0, pts)
+    want = _logistic_scipy(X, y, 5.0, pts)
+    assert np.allclose(got - got[0], want - want[0], rtol=0, atol=1e-9)
+
+
+def test_glm_logpost_closed_forms():
+    X, y, pts = _logistic_problem(N=512)
+    # beta = 0: every row contributes -ln 2 and the prior nothing
+    zero = O.glm_logpost(X, y, 5.0, np.zeros((1, X.shape[1]), np.float32))[0]
+    assert zero == pytest.approx(-X.shape[0] * np.log(2.0), rel=1e-14)
+    # label flip: (y, X) -> (1 - y, -X) leaves every row's term unchanged
+    a = O.glm_logpost(X, y, 5.0, pts)
+    b = O.glm_logpost(-X, 1.0 - y, 5.0, pts)
+    assert np.allclose(a, b, rtol=1e-13, atol=0)
+    # zero data: only the prior -|beta|^2 / (2 s^2) remains beside -N ln 2
+    Z = np.zeros_like(X)
+    got = O.glm_logpost(Z, y, 2.0, pts)
+    want = -X.shape[0] * np.log(2.0) - 0.125 * np.sum(pts.astype(np.float64) ** 2, axis=1)
+    assert np.allclose(got, want, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("variant", [4, 8])
+def test_glm_logpost_planted_mistakes_fail(variant):
+    """A dropped prior (bit 2) or a sign-flipped linear predictor (bit 3)
+    planted in the oracle must fail the scipy comparison above."""
+    X, y, pts = _logistic_problem()
+    want = _logistic_scipy(X, y, 5.0, pts)
+    O.set_variant(variant)
+    try:
+        got = O.glm_logpost(X, y, 5.0, pts)
+    finally:
+        O.set_variant(0)
+    assert not np.allclose(got - got[0], want - want[0], rtol=0, atol=1e-3)
+
+
+def test_glm_stretch_zero_data_moves_as_gaussian_stretch():
+    """With every row of X zero the logistic posterior is N(0, s^2 I): the GLM
+    sampler (log posteriors kept in double) takes the moves of the plain
+    sampler on the independent-Gaussian model from the same seeds (its float
+    log densities can flip a decision only at a rounding tie)."""
+    d, W, s, seed = 3, 1024, 1.0, 77
+    Z = np.zeros((16, d), np.float32)
+    y = np.ones(16, np.float32)
+    g = O.GlmStretch(Z, y, s, W, seed, partner=O.PARTNER_WALKER)
+    g.init(seed)
+    lim = np.tile([-2.0, 2.0], d)
+    g.init_position(seed, lim)
+    params = np.concatenate([np.zeros(d), np.full(d, 1.0 / s ** 2)]).astype(np.float32)
+    p = O.Stretch(O.DIAG_GAUSS_LOGPDF, d, W, params, seed, params_len=2 * d,
+                  partner=O.PARTNER_WALKER)
+    p.init(seed)
+    p.init_position(seed, lim)
+    assert np.array_equal(g.xs, p.xs)
+    for _ in range(20):
+        g.move_bare()
+        p.move_bare()
+    same = np.all(g.xs.reshape(W, d) == p.xs.reshape(W, d), axis=1)
+    assert same.mean() >= 0.999
+
+
+def test_glm_stretch_zero_data_moments():
+    """The zero-data GLM posterior N(0, s^2 I): means within 5 MC standard
+    errors of 0 and sds within 5% of s over a long run."""
+    d, W, s, seed = 2, 512, 1.0, 91
+    Z = np.zeros((4, d), np.float32)
+    g = O.GlmStretch(Z, np.zeros(4, np.float32), s, W, seed, partner=O.PARTNER_WALKER,
+                     threads=4)
+    g.init(seed)
+    g.init_position(seed, np.tile([-1.0, 1.0], d))
+    g.burn_in(200, 2.0)
+    draws = np.concatenate([g.sample(W) for _ in range(400)])
+    m, sd = draws.mean(0), draws.std(0)
+    # ~20 steps of autocorrelation: effective draws ~ W * 400 / 20
+    mcse = s / np.sqrt(W * 400 / 20)
+    assert np.all(np.abs(m) < 5 * mcse), (m, mcse)
+    assert np.all(np.abs(sd - s) < 0.05 * s), sd
