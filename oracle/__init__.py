@@ -52,6 +52,7 @@ GAUSSIAN_LOGLIK = 8
 DIAG_GAUSS_LOGPDF = 9
 HBB_LOGPDF = 10
 LINREG_POST = 11
+LOGISTIC_POST = 12
 
 PARTNER_FLAT = 0
 PARTNER_WALKER = 1
@@ -121,8 +122,54 @@ def lib():
             ctypes.c_void_p, _c_f32, _c_f32, ctypes.c_int, _f32p, _f32p, ctypes.c_uint64,
             ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         L.bho_glm_stretch_half.restype = _c_u32
+        _declare_blocked(L)
         _lib = L
     return _lib
+
+
+def _declare_blocked(L):
+    L.bho_logistic_half_blocked.argtypes = [
+        _c_u32, _c_u32, _c_u32, _c_u32, _c_u32, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_void_p, _c_f32, _c_f32, ctypes.c_int, _c_u32, _f32p, ctypes.c_void_p,
+        ctypes.c_int]
+    L.bho_logistic_half_blocked.restype = _c_u32
+
+
+_fast = None
+
+
+def _has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return " avx512f " in line + " "
+    except OSError:
+        pass
+    return False
+
+
+def fast_lib():
+    """The timed CPU-baseline build of the same source (-O3, vector
+    expf/logf; AVX-512 where the host has it).  Only bench.py's cpu_baseline
+    leg times it; parity checks use lib()."""
+    global _fast
+    if _fast is None:
+        name = "libbh_oracle_fast_v4.so" if _has_avx512() else "libbh_oracle_fast_v3.so"
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path) or (
+            os.path.getmtime(path) < os.path.getmtime(os.path.join(_HERE, "bh_oracle.c"))
+        ):
+            build()
+        L = ctypes.CDLL(path)
+        _declare_blocked(L)
+        _fast = (L, name)
+    return _fast[0]
+
+
+def fast_lib_name() -> str:
+    fast_lib()
+    return _fast[1]
 
 
 def set_variant(flags: int) -> None:

@@ -170,6 +170,7 @@ enum {
     BHO_DIAG_GAUSS_LOGPDF = 9,     /* independent Gaussian coordinates -sum w_i (x_i-mu_i)^2/2 (benchmark model) */
     BHO_HBB_LOGPDF = 10,           /* hierarchical beta-binomial on logits (batched benchmark model) */
     BHO_LINREG_POST = 11,          /* Gaussian linear regression with normal priors (benchmark model) */
+    BHO_LOGISTIC_POST = 12,        /* dist/posterior.cl:1-6: logistic loglik + iid normal prior (benchmark model) */
 };
 
 static const float LOG_SQRT_2PI_F = 0.9189385332046727f;
@@ -189,6 +190,38 @@ static inline float binomial_loglik(const float* data, const float* p) {
     /* dist/binomial.cl:5-7, 25-27: k*log(p) + (n-k)*log(1-p) */
     const float n = data[0], k = data[1];
     return fmaf(n - k, logf(1.0f - p[0]), k * logf(p[0]));
+}
+
+/* One data row of the logistic loglik in fp32: softplus(t) with t = -eta
+ * for y = 1 and eta for y = 0, as max(t, 0) + log(1 + exp(-|t|)). */
+static inline float logistic_row_term(float eta, float yv) {
+    const float t = (yv > 0.5f) ? -eta : eta;
+    return fmaxf(t, 0.0f) + logf(1.0f + expf(-fabsf(t)));
+}
+
+/* The logistic posterior as a user model in the reference's plugin form:
+ * posterior.cl:1-6 (LOGLIK + LOGPDF) with LOGLIK a loop over the data rows
+ * [x_i (dim), y_i] (params[0 .. data_len), data_len / (dim + 1) rows) in
+ * fp32, the accumulation shape of gaussian.cl:29-42, and LOGPDF the iid
+ * N(0, s^2) prior (hyperparameter params[data_len] = s) without its
+ * constant. */
+static float logistic_post_f32(uint32_t data_len, const float* params, uint32_t dim,
+                               const float* x) {
+    const uint32_t rows = data_len / (dim + 1);
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < rows; ++i) {
+        const float* row = params + (size_t)i * (dim + 1);
+        float eta = 0.0f;
+        for (uint32_t j = 0; j < dim; ++j) eta = fmaf(row[j], x[j], eta);
+        acc -= logistic_row_term(eta, row[dim]);
+    }
+    const float is = 1.0f / params[data_len];
+    float q = 0.0f;
+    for (uint32_t j = 0; j < dim; ++j) {
+        const float z = x[j] * is;
+        q = fmaf(z, z, q);
+    }
+    return acc + -0.5f * q;
 }
 
 BHO_EXPORT float bho_model_logfn(int model, uint32_t data_len, uint32_t params_len,
@@ -286,6 +319,8 @@ BHO_EXPORT float bho_model_logfn(int model, uint32_t data_len, uint32_t params_l
         const float zs = x[d] * is;
         return ll + -0.5f * fmaf(zs, zs, q);
     }
+    case BHO_LOGISTIC_POST:
+        return logistic_post_f32(data_len, params, dim, x);
     case BHO_GAUSSIAN_LOGLIK: {
         /* data = params[0 .. data_len), x = (mu, sigma) */
         const float mu = x[0], sigma = x[1];
@@ -409,6 +444,93 @@ BHO_EXPORT uint32_t bho_stretch_half(uint32_t seed, uint32_t odd_or_even, uint32
             if (accepted) accepted[k] = (uint8_t)acc;
             total += (uint32_t)acc;
         }
+        free(Y);
+    }
+    (void)threads;
+    return total;
+}
+
+/* bho_stretch_half for the logistic posterior (BHO_LOGISTIC_POST) with the
+ * walkers taken BHO_WBLK at a time: every walker's proposal, log posterior
+ * and accept test are the per-walker ones (the same operations in the same
+ * order), but the row loop runs once per block, each row's values applied
+ * to the block's walkers side by side -- the work-item vectorisation an
+ * OpenCL CPU device applies to the reference's kernel.  This is the timed
+ * CPU baseline of the logistic workload; with -ffp-contract=off it equals
+ * bho_stretch_half bit for bit. */
+#define BHO_WBLK 128
+BHO_EXPORT uint32_t bho_logistic_half_blocked(uint32_t seed, uint32_t odd_or_even, uint32_t step,
+                                              uint32_t K, uint32_t dim, const float* Scompl,
+                                              float* X, float* logfn_X, float a, float beta,
+                                              int partner_mode, uint32_t data_len,
+                                              const float* params, uint8_t* accepted,
+                                              int threads) {
+    uint32_t total = 0;
+    const uint32_t nblk = (K + BHO_WBLK - 1) / BHO_WBLK;
+    const uint32_t rows = data_len / (dim + 1);
+    const float is = 1.0f / params[data_len];
+#ifdef _OPENMP
+    if (threads <= 0) threads = 1;
+#pragma omp parallel num_threads(threads) reduction(+ : total)
+#endif
+    {
+        float* Yt = (float*)malloc(sizeof(float) * (size_t)dim * BHO_WBLK);
+        float* Y = (float*)malloc(sizeof(float) * (dim ? dim : 1));
+        float z[BHO_WBLK], u2[BHO_WBLK], acc[BHO_WBLK], eta[BHO_WBLK];
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int64_t bb = 0; bb < (int64_t)nblk; ++bb) {
+            const uint32_t k0 = (uint32_t)bb * BHO_WBLK;
+            const uint32_t nw = K - k0 < BHO_WBLK ? K - k0 : BHO_WBLK;
+            for (uint32_t w = 0; w < BHO_WBLK; ++w) {
+                if (w < nw) {
+                    u2[w] = propose_one(k0 + w, seed, odd_or_even, step, K, dim, Scompl, X, a,
+                                        partner_mode, Y, &z[w]);
+                    for (uint32_t j = 0; j < dim; ++j) Yt[(size_t)j * BHO_WBLK + w] = Y[j];
+                } else {
+                    for (uint32_t j = 0; j < dim; ++j) Yt[(size_t)j * BHO_WBLK + w] = 0.0f;
+                }
+                acc[w] = 0.0f;
+            }
+            for (uint32_t i = 0; i < rows; ++i) {
+                const float* row = params + (size_t)i * (dim + 1);
+                for (uint32_t w = 0; w < BHO_WBLK; ++w) eta[w] = 0.0f;
+                for (uint32_t j = 0; j < dim; ++j) {
+                    const float xv = row[j];
+                    const float* yj = Yt + (size_t)j * BHO_WBLK;
+#ifdef _OPENMP
+#pragma omp simd
+#endif
+                    for (uint32_t w = 0; w < BHO_WBLK; ++w) eta[w] = fmaf(xv, yj[w], eta[w]);
+                }
+                const float yv = row[dim];
+#ifdef _OPENMP
+#pragma omp simd
+#endif
+                for (uint32_t w = 0; w < BHO_WBLK; ++w) acc[w] -= logistic_row_term(eta[w], yv);
+            }
+            for (uint32_t w = 0; w < nw; ++w) {
+                const uint32_t k = k0 + w;
+                float q = 0.0f;
+                for (uint32_t j = 0; j < dim; ++j) {
+                    const float zz = Yt[(size_t)j * BHO_WBLK + w] * is;
+                    q = fmaf(zz, zz, q);
+                }
+                const float ly = acc[w] + -0.5f * q;
+                const float qa = isfinite(ly)
+                                     ? pown_f(z[w], (int)dim - 1) * expf(beta * (ly - logfn_X[k]))
+                                     : 0.0f;
+                const int acc_k = u2[w] <= qa;
+                if (acc_k) {
+                    for (uint32_t j = 0; j < dim; ++j) X[(size_t)k * dim + j] = Yt[(size_t)j * BHO_WBLK + w];
+                    logfn_X[k] = ly;
+                }
+                if (accepted) accepted[k] = (uint8_t)acc_k;
+                total += (uint32_t)acc_k;
+            }
+        }
+        free(Yt);
         free(Y);
     }
     (void)threads;

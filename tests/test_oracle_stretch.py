@@ -313,3 +313,51 @@ def test_glm_stretch_zero_data_moments():
     mcse = s / np.sqrt(W * 400 / 20)
     assert np.all(np.abs(m) < 5 * mcse), (m, mcse)
     assert np.all(np.abs(sd - s) < 0.05 * s), sd
+
+
+def _logistic_rows(X, y, s):
+    rows = np.concatenate([X, y[:, None]], axis=1).astype(np.float32)
+    return np.concatenate([rows.reshape(-1), [s]]).astype(np.float32)
+
+
+def test_logistic_user_model_against_glm_logpost():
+    """The logistic posterior as a reference-style fp32 user model (row loop
+    inside LOGFN) agrees with the fp64 GLM posterior, up to the prior's
+    constant and fp32 rounding."""
+    X, y, pts = _logistic_problem(N=600, d=8)
+    params = _logistic_rows(X, y, 5.0)
+    got = O.logfn_many(O.LOGISTIC_POST, params, pts, 8, data_len=600 * 9,
+                       params_len=1).astype(np.float64)
+    want = O.glm_logpost(X, y, 5.0, pts)
+    assert np.allclose(got, want, rtol=2e-6, atol=1e-3)
+
+
+def test_logistic_blocked_half_equals_per_walker_half():
+    """The walker-blocked half-step (the timed CPU baseline) takes exactly
+    the per-walker half-step's decisions, positions and log posteriors, with
+    a partial last block."""
+    X, y, _ = _logistic_problem(N=300, d=6)
+    params = _logistic_rows(X, y, 5.0)
+    K, d = 200, 6  # one full block of 128 walkers and a partial one
+    rng = np.random.default_rng(4)
+    xs = (0.3 * rng.standard_normal(2 * K * d)).astype(np.float32)
+    lf = O.logfn_many(O.LOGISTIC_POST, params, xs.reshape(2 * K, d), d, data_len=300 * 7,
+                      params_len=1).astype(np.float32)
+    out = []
+    for blocked in (False, True):
+        x, l = xs.copy(), lf.copy()
+        acc = np.zeros(K, np.uint8)
+        L = O.lib()
+        if blocked:
+            n = L.bho_logistic_half_blocked(9, 3333, 2, K, d, x.ctypes.data + 4 * K * d,
+                                            x.ctypes.data, l.ctypes.data, 2.0, 1.0,
+                                            O.PARTNER_WALKER, 300 * 7, params, acc.ctypes.data,
+                                            3)
+        else:
+            n = L.bho_stretch_half(9, 3333, 2, K, d, x.ctypes.data + 4 * K * d, x.ctypes.data,
+                                   l.ctypes.data, 2.0, 1.0, O.PARTNER_WALKER, O.LOGISTIC_POST,
+                                   300 * 7, 1, params, acc.ctypes.data, 3)
+        out.append((n, x, l, acc))
+    assert out[0][0] == out[1][0] and 0 < out[0][0] < K
+    for a, b in zip(out[0][1:], out[1][1:]):
+        assert np.array_equal(a, b)
