@@ -66,6 +66,7 @@ SIGNATURES = {
     "bh_evidence": [_P, _P, _U32, _P, _U32, _U64, _P],
     "bh_dataset_mean": [_P, _P, _U32, _U64, _U32, _P],
     "bh_dataset_variance": [_P, _P, _U32, _U64, _U32, _P],
+    "bh_dataset_variance_noalloc": [_P, _P, _U32, _U64, _U32, _P],
     "bh_dataset_histogram": [_P, _P, _U32, _U64, _U32, _P, _P, _P],
     "bh_acor": [_P, _P, _U32, _U32, _P, _P, _P, _P],
     "bh_acor_batch": [_P, _P, _U32, _P, _U32, _P, _P, _P, _P],

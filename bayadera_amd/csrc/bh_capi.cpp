@@ -896,6 +896,9 @@ int bh_ctx_create(int device, uint32_t wgs, bh_ctx** out) {
     c->cu = prop.multiProcessorCount;
     std::string arch = prop.gcnArchName;
     c->arch = arch.substr(0, arch.find(':'));
+    // the address of this device's copy of bh_dataset_variance_noalloc's
+    // working memory, looked up once per device with it current
+    BH_HIP(bh::noalloc_scratch_init(device));
     BH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
     return BH_OK;
@@ -1181,6 +1184,27 @@ int bh_dataset_mean(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint
 int bh_dataset_variance(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
                         float* out) {
     return moments_impl(ctx, data, m, n, ld, out, true);
+}
+
+int bh_dataset_variance_noalloc(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n,
+                                uint32_t ld, float* out) {
+    BH_ARG(ctx && data && out, "NULL argument");
+    BH_TRY(check_matrix(m, n, ld));
+    BH_ARG(m <= bh::kNoAllocMaxRows,
+           "%u rows: bh_dataset_variance_noalloc takes at most %u (bh_dataset_variance has no "
+           "limit)",
+           m, bh::kNoAllocMaxRows);
+    BH_ARG(is_device_ptr(data) && is_device_ptr(out),
+           "bh_dataset_variance_noalloc needs data and out in device memory (it stages nothing)");
+    BH_HIP(hipSetDevice(ctx->device));
+    void* scr = nullptr;
+    BH_HIP(bh::noalloc_scratch_init(ctx->device));
+    BH_HIP(bh::noalloc_scratch(ctx->device, &scr));
+    // the same launches as bh_dataset_variance, with the partial sums and the
+    // row means in the device array instead of allocated buffers
+    float* mean = reinterpret_cast<float*>(static_cast<char*>(scr) + bh::kNoAllocPartialBytes);
+    BH_HIP(bh::launch_moments(ctx->stream, data, m, n, ld, mean, out, scr));
+    return sync(ctx->stream);
 }
 
 int bh_dataset_histogram(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,

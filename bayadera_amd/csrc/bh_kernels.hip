@@ -10,10 +10,43 @@
 // read consecutive addresses of a column.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "bh_kernels.hpp"
 #include "bh_philox.hpp"
 
+// The working memory of bh_dataset_variance_noalloc (partial sums and row
+// means), for callers that cannot allocate.  Each GPU holds its own copy at
+// its own address; host code reaches it only through the address that
+// noalloc_scratch_init looks up (the array's name is a host-side stand-in).
+static __device__ __align__(256) unsigned char scratch[bh::kNoAllocScratchBytes];
+
 namespace bh {
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices];
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_noalloc_ptr[device] = p;
+    return hipSuccess;
+}
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    *ptr = g_noalloc_ptr[device];
+    return *ptr ? hipSuccess : hipErrorNotInitialized;
+}
 
 // --------------------------------------------------------------------------
 // init_walkers (amd-gcn-mcmc-stretch.cl:138-159)

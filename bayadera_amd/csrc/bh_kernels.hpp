@@ -9,6 +9,19 @@ constexpr unsigned kMaxDatasetBlocks = 2048;  // 8 blocks per CU over 256 CUs
 // the per-step means are sized for it
 constexpr unsigned kMaxGroupThreads = 1024;
 
+// bh_dataset_variance_noalloc: the rows one call may have, and the device
+// array (one per GPU, held until the process ends) that takes the place of
+// the buffers bh_dataset_variance allocates: kMaxDatasetBlocks x rows double
+// partial sums, then the row means.
+constexpr unsigned kNoAllocMaxRows = 256;
+constexpr size_t kNoAllocPartialBytes = (size_t)kMaxDatasetBlocks * kNoAllocMaxRows * sizeof(double);
+constexpr size_t kNoAllocScratchBytes = kNoAllocPartialBytes + kNoAllocMaxRows * sizeof(float);
+// Looks up the array's address on `device`, which must be the current
+// device; once per device, later calls return at once.
+hipError_t noalloc_scratch_init(int device);
+// The address noalloc_scratch_init found for `device`.
+hipError_t noalloc_scratch(int device, void** ptr);
+
 hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits, unsigned dim,
                                unsigned walkers, float* xs);
 hipError_t launch_uniform_sample(hipStream_t s, unsigned seed, float lo, float hi, unsigned quads,

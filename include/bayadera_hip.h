@@ -157,6 +157,17 @@ int bh_dataset_mean(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint
                     float* out);
 int bh_dataset_variance(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
                         float* out);
+/* bh_dataset_variance without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  data and
+ * out must be device pointers (nothing is staged: BH_ERR_ARG otherwise) and
+ * m <= 256 (BH_ERR_ARG beyond).  The partial sums and row means live in one
+ * static device array per GPU (4 MiB, held until the process ends) instead
+ * of allocated buffers; the result equals bh_dataset_variance's bit for bit.
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU (from several threads or contexts) race. */
+int bh_dataset_variance_noalloc(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n,
+                                uint32_t ld, float* out);
 /* EstimateEngine.histogram: Histogram{limits 2 x m, pdf WGS x m, bin-ranks
  * WGS x m} (protocols.clj:13-18); WGS bins per row (amd-gcn-estimate.cl:1-25). */
 int bh_dataset_histogram(bh_ctx* ctx, const float* data, uint32_t m, uint64_t n, uint32_t ld,
