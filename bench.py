@@ -4,24 +4,36 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
   Bayesian logistic regression, precision form.  X is 1,000,000 x 64 with
   rows N(0,1)/8, stored bf16 (numpy seed 1003); beta* ~ N(0,1); y ~
   Bernoulli(sigmoid(X beta*)); prior beta ~ N(0, 5^2).  16384 walkers (8192
-  per half), whole-walker partners, a = 2, 500 burn-in + 2000 run-sampler
-  steps; walkers start uniform in a +-4 sd box around the posterior mode
-  (Newton steps on the host, not timed).  Runs from the same seeds:
-    MFMA    likelihood contraction on the matrix cores (--c3-mfma: fp16x2 =
-            fp16 copy of X with proposals split in two fp16 terms, or
-            bf16x3 = bf16 X with three bf16 terms; fp32 accumulate, fp64 row
-            sums) with the guarded accept test,
+  per half), whole-walker partners, a = 2; walkers start uniform in a box of
+  half-width sqrt(3) sd around the posterior mode per coordinate (Newton
+  steps and the Laplace sd on the host, not timed).  Legs, from the same
+  seeds:
+    fp16x2  likelihood contraction on the matrix cores, fp16 copy of X with
+            the proposals split in two fp16 terms (the headline, --c3-mfma),
+    bf16x3  the same on bf16 X with three bf16 terms,
+            both with fp32 accumulation, fp64 row sums and the guarded
+            accept test (decisions near the threshold taken on fp64 values),
     fp32    the plain fp32 VALU likelihood kernel,
-    fp64    the fp64 likelihood kernel (coupled windows; the whole job with
+    fp64    the fp64 likelihood kernel (coupled windows; a whole job with
             --c3-fp64-full).
+  Timed region of each leg: a fresh sampler, --warmup untimed move-bare!
+  steps, then exactly --steps accounted move! steps between device
+  synchronisations -> ms_per_step and walker-steps/s.  Efficiency:
   ESS_d = Var_d(final ensemble) / sigma_d^2 with sigma_d from the engine's
-  acor of the run-sampler means (Sokal's windowed estimate for a dimension
-  where acor gives no value); value = min_d ESS_d per second of the MFMA
-  job (burn-in + run, inputs resident in HBM).  Posterior-mean error
-  = |mean_d - mean_fp64,d| / sigma_fp64,d on these coupled runs.
+  acor of the run-sampler means (Sokal's windowed estimate where acor gives
+  no value), ESS per walker-step = min_d ESS_d / (W x run steps), measured
+  on guarded runs of 70000 steps (about 40 tau_max) from three seeds
+  (--c3-ess-calibration, kept under profiles/); one job of the headline leg
+  (500 burn-in + 12000 run-sampler steps) in every run reports its own
+  (shorter, noisier) estimate beside it.  value = ESS per walker-step x the
+  headline leg's walker-steps/s.  Every leg runs the same Markov kernel
+  (the guarded legs' trajectories equal the fp64 sampler's step for step,
+  checked in the coupled windows; the plain fp32 leg differs from it by
+  rounding only), so each leg's ESS/s is the same ESS per walker-step times
+  its own walker-steps/s.
   Roofline: the MFMA likelihood kernel, 2 N d K = 1.05e12 algorithmic flops
-  per launch, its launches timed with HIP events on the sampler's stream
-  during the run phase.
+  per launch, its launches in the timed region timed with HIP events on the
+  sampler's stream.
 
 --workload c1: the reference's CPU-runnable case.  4-D correlated Gaussian
   target, mu ~ N(0,1)^4, Sigma = A A' + I with A 4x4 i.i.d. N(0,1) (numpy
@@ -36,7 +48,10 @@ positions, above the 256 MB Infinity Cache), its launches timed with HIP
 events; algorithmic bytes per walker-step 8*DIM + 4 + alpha*(4*DIM + 4).
 
 CPU baseline: the CPU restatement of the reference kernels (oracle/) on
-the host cores, OpenMP over walkers, on a bounded sample of the workload.
+the host cores, OpenMP over walkers, on a bounded sample of the workload;
+for the logistic workload the reference's own arithmetic (the posterior as
+an fp32 user model, its row loop inside LOGFN), a whole 8192-walker
+half-step at -O3.
 """
 from __future__ import annotations
 
@@ -72,8 +87,18 @@ ROOF_DIMS = (("mvn", 4), ("mvn", 16), ("diag", 32))
 
 # logistic-regression workload
 C3_N, C3_D, C3_W = 1_000_000, 64, 16384
-C3_BURN, C3_RUN = 500, 2000
+# the job that measures ESS per walker-step: tau_max is about 600 steps, so
+# the run is about 20 tau
+C3_BURN, C3_RUN = 500, 12000
 C3_PRIOR_SD = 5.0
+# half-width of the init-position! box per coordinate, in sd of the Laplace
+# approximation (a uniform box of half-width sqrt(3) sd has variance sd^2)
+C3_BOX_SD = 3.0 ** 0.5
+# ESS calibration (--c3-ess-calibration): seeds and run length (about 40
+# tau_max)
+C3_CAL_SEEDS = (123, 124, 125)
+C3_CAL_RUN = 70000
+C3_CAL_FILE = "glm_logistic_1Mx64_ess_calibration.json"
 # guard of the reduced-precision accept test, in log units: walkers whose
 # accept margin is smaller are decided on fp64 log posteriors.  The measured
 # error of lp(Y) - lp(X) at this size has sd 1.0e-3 and max 3.4e-3 (fp32 and
@@ -110,6 +135,15 @@ def ess_acor_or_sokal(ess_acor, var, series):
     ok = np.isfinite(ess_acor)
     comb = np.where(ok, ess_acor, ess_s)
     return float(np.min(comb)), int((~ok).sum()), ess_s
+
+
+def mcse_of(r):
+    """Monte-Carlo standard error of each coordinate's posterior mean: acor's
+    sigma where it gives one, else sqrt(tau_sokal var(series) / n)."""
+    m = np.asarray(r["means"], np.float64)
+    sig = np.asarray(r["sigma"], np.float64)
+    sok = np.sqrt(sokal_tau(m) * m.var(0) / m.shape[0])
+    return np.where(np.isfinite(sig), sig, sok)
 
 
 def sokal_tau(series, c=5.0):
@@ -881,8 +915,9 @@ def c3_problem(N=C3_N, d=C3_D, seed=1003):
     beta_true = rng.standard_normal(d)
     eta = Xd @ beta_true
     y = (rng.random(N) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float32)
-    # a Laplace box for init-position!: Newton steps to the posterior mode,
-    # walkers uniform within 4 sd of it (setup, not timed)
+    # the init-position! box: Newton steps to the posterior mode (setup, not
+    # timed), then per coordinate a uniform box of half-width sqrt(3) sd of
+    # the Laplace approximation, whose variance is the Laplace variance
     b = np.zeros(d)
     for _ in range(8):
         p = 1.0 / (1.0 + np.exp(-(Xd @ b)))
@@ -890,7 +925,8 @@ def c3_problem(N=C3_N, d=C3_D, seed=1003):
         H = (Xd * (p * (1 - p))[:, None]).T @ Xd + np.eye(d) / C3_PRIOR_SD ** 2
         b = b + np.linalg.solve(H, g)
     sd = np.sqrt(np.diag(np.linalg.inv(H)))
-    limits = np.stack([b - 4 * sd, b + 4 * sd], 1).astype(np.float32).reshape(-1)
+    half = C3_BOX_SD * sd
+    limits = np.stack([b - half, b + half], 1).astype(np.float32).reshape(-1)
     return Xb, Xd, y, b, sd, limits
 
 
@@ -901,6 +937,32 @@ def c3_sampler(sf, prec, guard=0.0, seed=123):
     if guard:
         s.set_glm_guard(guard)
     return s
+
+
+def c3_timed(sf, prec, limits, warmup, steps, guard=0.0, seed=123):
+    """The timed region: a fresh sampler, init-position!, `warmup` untimed
+    move-bare! steps, then exactly `steps` accounted move! steps bracketed by
+    device synchronisations; the likelihood kernel's launches in them timed
+    with HIP events on the sampler's stream."""
+    import torch
+    s = c3_sampler(sf, prec, guard, seed)
+    s.init(seed)
+    s.init_position(seed + 1, limits)
+    if warmup:
+        s.burn_in(warmup, A)
+    s.init_move(A, steps)
+    s.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.move()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    kt = s.kernel_timing()
+    gs = s.glm_guard_stats()
+    s.release()
+    return {"wall_s": t, "ms_per_step": t / steps * 1e3,
+            "walker_steps_per_s": C3_W * steps / t, "kernel": kt, "guard": gs}
 
 
 def c3_run(sf, prec, limits, burn, run, guard=0.0, seed=123):
@@ -929,9 +991,27 @@ def c3_run(sf, prec, limits, burn, run, guard=0.0, seed=123):
     ess = ess_from(var, ac.sigma)
     return {"wall_s": (t1 - t0) + (t2 - t1), "run_gpu_ms": run_gpu_ms, "ess": ess,
             "means": res["means"], "grand_mean": res["means"].astype(np.float64).mean(0),
-            "sigma": ac.sigma, "sd": np.sqrt(var.astype(np.float64)), "tau": ac.tau,
-            "acc": res["acceptance-rate"], "kernel": kt, "xs": end[0], "guard": gs,
-            "after_burn": after_burn, "end": end}
+            "sigma": ac.sigma, "sd": np.sqrt(var.astype(np.float64)), "var": var,
+            "tau": ac.tau, "acc": res["acceptance-rate"], "kernel": kt, "xs": end[0],
+            "guard": gs, "after_burn": after_burn, "end": end, "burn": burn, "run": run}
+
+
+def c3_ess_summary(r):
+    """ESS of a job: min over dims of acor's ESS (Sokal's where acor gives
+    none), per walker-step of the run phase and per second of the job."""
+    m = np.asarray(r["means"], np.float64)
+    ess, nan_dims, ess_sd = ess_acor_or_sokal(r["ess"], np.asarray(r["var"], np.float64), m)
+    tau_s = sokal_tau(m)
+    return {"ess_min": round(ess, 1), "acor_nan_dims": nan_dims,
+            "ess_per_walker_step": ess / (C3_W * r["run"]),
+            "ess_per_s_job": round(ess / r["wall_s"], 2),
+            "ess_sokal_min": round(float(np.min(ess_sd)), 1),
+            "tau_max": round(float(np.nanmax(r["tau"])), 2),
+            "tau_sokal_max": round(float(tau_s.max()), 1),
+            "run_over_tau_max": round(r["run"] / float(np.nanmax(r["tau"])), 1),
+            "acceptance_rate": round(r["acc"], 4), "job_wall_s": round(r["wall_s"], 3),
+            "job_steps": r["burn"] + r["run"],
+            "job_ms_per_step_run": round(r["run_gpu_ms"] / r["run"], 3)}
 
 
 def c3_window(sf, state, prec, guard, steps, accounted):
@@ -950,31 +1030,134 @@ def c3_window(sf, state, prec, guard, steps, accounted):
     return xs, means, gs
 
 
-def c3_cpu_baseline(Xd, y, limits, budget_s=20.0):
-    """Walker-steps/s of the oracle's fp64 GLM sampler on a bounded sample:
-    walkers x one half-step over the full 1M x 64 data, OpenMP over walkers."""
+def c3_cpu_baseline(Xd, y, limits, budget_s=30.0):
+    """Walker-steps/s of the reference's arithmetic on the host: the logistic
+    posterior as the reference's fp32 user model (posterior.cl:1-6 with the
+    row loop of gaussian.cl:29-42 inside LOGFN), one stretch half-step of
+    the whole 8192-walker half over the full 1M x 64 data, OpenMP over
+    blocks of walkers (the oracle's -O3 AVX build)."""
     import oracle as O
     threads = _threads()
-    Xf = np.ascontiguousarray(Xd, np.float32)
-    W = 1024  # the sampler's smallest ensemble; half of it moves per half-step
-    g = O.GlmStretch(Xf, y, C3_PRIOR_SD, W, 5, partner=O.PARTNER_WALKER, threads=threads)
-    # positions only (skip the initial log posteriors of all walkers)
-    g.xs[:] = O.init_walkers(6, limits.reshape(-1, 2), C3_D, W)
+    L = O.fast_lib()
+    N, d = Xd.shape
+    params = np.concatenate([np.concatenate([Xd.astype(np.float32), y[:, None]], 1).reshape(-1),
+                             [C3_PRIOR_SD]]).astype(np.float32)
+    K = C3_W // 2
+    xs = O.init_walkers(6, limits.reshape(-1, 2), d, 2 * K).astype(np.float32).reshape(-1)
+    # the current half's log posteriors enter only the accept test; the
+    # half-step's cost is the K proposals' log posteriors
+    lf = np.zeros(2 * K, np.float32)
+    acc = np.zeros(K, np.uint8)
+    # a probe on 128 walkers sizes the sample to the budget (at least one
+    # 128-walker block, at most the whole half)
+    kk = 128
     t0 = time.perf_counter()
-    n_half = 0
-    while n_half == 0 or (time.perf_counter() - t0 < budget_s and n_half < 4):
-        g._half(5, 3333, n_half, True, A, 1.0)
-        n_half += 1
+    L.bho_logistic_half_blocked(5, 3333, 0, kk, d, xs.ctypes.data + 4 * K * d, xs.ctypes.data,
+                                lf.ctypes.data, A, 1.0, O.PARTNER_WALKER, N * (d + 1), params,
+                                acc.ctypes.data, threads)
+    t_probe = time.perf_counter() - t0
+    blocks = int(min(K // 128, max(1, budget_s / max(t_probe, 1e-9))))
+    kk = 128 * blocks
+    t0 = time.perf_counter()
+    L.bho_logistic_half_blocked(5, 3333, 1, kk, d, xs.ctypes.data + 4 * K * d, xs.ctypes.data,
+                                lf.ctypes.data, A, 1.0, O.PARTNER_WALKER, N * (d + 1), params,
+                                acc.ctypes.data, threads)
     t = time.perf_counter() - t0
-    # each half-step moves W / 2 walkers: W / 2 walker-steps
-    return {"walker_steps_per_s": (W // 2) * n_half / t, "threads": threads,
-            "sample": f"{n_half} half-step(s) of {W // 2} walkers over the full "
-                      f"{C3_N}x{C3_D} data in {t:.1f} s"}
+    return {"walker_steps_per_s": kk / t, "threads": threads, "build": O.fast_lib_name(),
+            "sample": (f"{kk} of the {K} walkers of one half-step over the full {N}x{d} data "
+                       f"(fp32 user-model arithmetic, {O.fast_lib_name()}) in {t:.1f} s"
+                       + ("" if kk == K else "; the rate scales linearly in walkers"))}
+
+
+def c3_long_run(sf, prec, limits, burn, run, guard, seed, acor, chunk=10000):
+    """A job of `run` run-sampler! steps as consecutive run-sampler! calls of
+    `chunk` steps (the sampler's state and counters carry over), one
+    progress line per call; acor of the whole concatenated means series."""
+    s = c3_sampler(sf, prec, guard, seed)
+    s.init(seed)
+    s.init_position(seed + 1, limits)
+    t0 = time.perf_counter()
+    for b0 in range(0, burn, chunk):
+        s.burn_in(min(chunk, burn - b0), A)
+    parts, accs = [], []
+    done = 0
+    while done < run:
+        n = min(chunk, run - done)
+        res = s.run_sampler(n, A)
+        parts.append(res["means"].copy())
+        accs.append(res["acceptance-rate"] * n)
+        done += n
+        print(f"  {done}/{run} steps, {time.perf_counter() - t0:.0f} s", file=sys.stderr,
+              flush=True)
+    var = s.variance()
+    wall = time.perf_counter() - t0
+    gs = s.glm_guard_stats()
+    xs = s.state()[0]
+    s.release()
+    means = np.ascontiguousarray(np.concatenate(parts))
+    ac = acor.acor(means)
+    return {"wall_s": wall, "run_gpu_ms": wall * 1e3, "ess": ess_from(var, ac.sigma),
+            "means": means, "sigma": ac.sigma, "var": var, "tau": ac.tau,
+            "grand_mean": means.astype(np.float64).mean(0), "sd": np.sqrt(var.astype(np.float64)),
+            "xs": xs, "acc": float(np.sum(accs) / run), "guard": gs, "burn": burn, "run": run}
+
+
+def c3_calibration():
+    """ESS per walker-step of the logistic workload, measured on long runs:
+    for each of three seeds, a guarded fp16x2 job of 500 burn-in +
+    C3_CAL_RUN run-sampler steps (about 40 tau_max), and, from the first
+    C3_RUN steps of the same series, the estimate a job of the bench's length
+    gives.  (The guarded chain is the fp64 chain step for step; the plain
+    fp32 chain is the same Markov kernel up to rounding, so its ESS per
+    walker-step is the same quantity.)"""
+    from bayadera_amd import _lib, engine, models
+    Xb, Xd, y, _, _, limits = c3_problem()
+    fac = engine.HipBayaderaFactory(0, 256)
+    sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=C3_PRIOR_SD))
+    acor = fac.acor_engine()
+    out = {"workload": "logistic regression 1000000x64, 16384 walkers, a=2, prior N(0,5^2), "
+                       "guarded fp16x2 likelihood (the fp64 chain)",
+           "command": "python bench.py --c3-ess-calibration", "runs": []}
+    for seed in C3_CAL_SEEDS:
+        print(f"calibration: seed {seed}, {C3_BURN} + {C3_CAL_RUN} steps", file=sys.stderr,
+              flush=True)
+        r = c3_long_run(sf, _lib.BH_GLM_FP16X2, limits, C3_BURN, C3_CAL_RUN, C3_GUARD, seed,
+                        acor)
+        full = c3_ess_summary(r)
+        full.pop("ess_per_s_job")
+        full.pop("job_ms_per_step_run")
+        pre = np.ascontiguousarray(r["means"][:C3_RUN])
+        ac = acor.acor(pre)
+        r_pre = dict(r, means=pre, ess=ess_from(r["var"], ac.sigma), tau=ac.tau, run=C3_RUN)
+        prefix = c3_ess_summary(r_pre)
+        for k in ("ess_per_s_job", "job_ms_per_step_run", "job_wall_s", "job_steps"):
+            prefix.pop(k)
+        out["runs"].append({"seed": seed, "full": full, f"first_{C3_RUN}_steps": prefix,
+                            "guard_overflow": r["guard"]["overflow"]})
+        print(json.dumps(out["runs"][-1]), file=sys.stderr, flush=True)
+    e = np.array([x["full"]["ess_per_walker_step"] for x in out["runs"]])
+    ep = np.array([x[f"first_{C3_RUN}_steps"]["ess_per_walker_step"] for x in out["runs"]])
+    out["ess_per_walker_step"] = {"mean": float(e.mean()), "sd": float(e.std(ddof=1)),
+                                  "run_steps": C3_CAL_RUN, "seeds": len(e)}
+    out[f"ess_per_walker_step_first_{C3_RUN}"] = {"mean": float(ep.mean()),
+                                                  "sd": float(ep.std(ddof=1))}
+    print(json.dumps(out), flush=True)
+
+
+def c3_calibrated():
+    path = os.path.join(ROOT, "profiles", C3_CAL_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def main_c3(args):
     import torch
     from bayadera_amd import _lib, engine, models
+    K, W = args.steps, args.warmup
+    if K < 1 or W < 0:
+        raise SystemExit("bench.py: --steps must be >= 1 and --warmup >= 0")
     t_setup = time.perf_counter()
     Xb, Xd, y, b_map, sd_map, limits = c3_problem()
     t_setup = time.perf_counter() - t_setup
@@ -983,107 +1166,140 @@ def main_c3(args):
     burn, run = args.c3_burn, args.c3_run
     mname = args.c3_mfma
     mprec, msplit, mkern = C3_MFMA[mname]
-    # warm-up: every kernel of the guarded and plain paths once
-    c3_run(sf, mprec, limits, 2, 64, guard=args.c3_guard)
-    c3_run(sf, _lib.BH_GLM_FP32, limits, 1, 50)
-    torch.cuda.synchronize()
-    legs = [(mname, mprec, args.c3_guard), ("fp32", _lib.BH_GLM_FP32, 0.0)]
-    if args.c3_other_mfma:
-        other = "bf16x3" if mname == "fp16x2" else "fp16x2"
-        legs.append((other, C3_MFMA[other][0], args.c3_guard))
+    other = "bf16x3" if mname == "fp16x2" else "fp16x2"
+    # first use of every kernel of the guarded and plain paths (fp16 copy of
+    # X, hipRTC modules), outside every timed region
+    c3_timed(sf, mprec, limits, 1, 2, guard=args.c3_guard)
+    c3_timed(sf, C3_MFMA[other][0], limits, 1, 2, guard=args.c3_guard)
+    c3_timed(sf, _lib.BH_GLM_FP32, limits, 1, 2)
+    legs = [(mname, mprec, args.c3_guard), (other, C3_MFMA[other][0], args.c3_guard),
+            ("fp32", _lib.BH_GLM_FP32, 0.0)]
     if args.c3_bf16:
         legs.append(("bf16", _lib.BH_GLM_BF16, 0.0))
-    res = {}
+    timed = {}
     for name, prec, g in legs:
-        print(f"bench: {name} leg", file=sys.stderr, flush=True)
-        res[name] = c3_run(sf, prec, limits, burn, run, guard=g)
-    ref = None
+        print(f"bench: {name} timed region ({W} warm-up + {K} steps)", file=sys.stderr, flush=True)
+        timed[name] = c3_timed(sf, prec, limits, W, K, guard=g)
+    # the efficiency: ESS per walker-step of the MFMA leg's whole job
+    print(f"bench: {mname} job ({burn} burn-in + {run} run-sampler steps)", file=sys.stderr,
+          flush=True)
+    job = c3_run(sf, mprec, limits, burn, run, guard=args.c3_guard)
+    js = c3_ess_summary(job)
+    ess_ws = js["ess_per_walker_step"]
+    ref = r32 = None
     if args.c3_fp64_full:
-        print("bench: fp64 leg", file=sys.stderr, flush=True)
-        ref = c3_run(sf, _lib.BH_GLM_FP64, limits, burn, run)
+        # run-sampler! in pieces with a progress line each (the trajectory
+        # is that of one call: the step counter carries over)
+        # (init-move! draws fresh seeds per call, so every leg here runs the
+        # same pieces)
+        acor = fac.acor_engine()
+        print(f"bench: {mname} job in pieces", file=sys.stderr, flush=True)
+        jc = c3_long_run(sf, mprec, limits, burn, run, args.c3_guard, 123, acor, chunk=500)
+        print("bench: fp64 job", file=sys.stderr, flush=True)
+        ref = c3_long_run(sf, _lib.BH_GLM_FP64, limits, burn, run, 0.0, 123, acor, chunk=500)
+        print("bench: plain fp32 job", file=sys.stderr, flush=True)
+        r32 = c3_long_run(sf, _lib.BH_GLM_FP32, limits, burn, run, 0.0, 123, acor, chunk=500)
+    # log-posterior error of each precision against the fp64 kernel on the
+    # job's final ensemble (16384 points), and of the Metropolis log ratio
+    # lp(x_k) - lp(x_{k+K}) over pairs of them
+    print("bench: log-density errors", file=sys.stderr, flush=True)
+    pts = np.ascontiguousarray(job["end"][0])
+    lp64 = sf.log_density(pts, _lib.BH_GLM_FP64)
+    d64 = lp64[: C3_W // 2] - lp64[C3_W // 2:]
+    lp_err = {}
+    for name, prec, _g in legs:
+        lp = sf.log_density(pts, prec)
+        e = lp - lp64
+        de = (lp[: C3_W // 2] - lp[C3_W // 2:]) - d64
+        lp_err[name] = {"lp_abs_err_max": float(np.max(np.abs(e))),
+                        "lp_rel_err_max": float(np.max(np.abs(e) / np.abs(lp64))),
+                        "log_ratio_err_sd": float(np.std(de)),
+                        "log_ratio_err_max": float(np.max(np.abs(de)))}
     windows = []
     if not args.no_fp64:
         # coupled windows against the fp64 sampler from the MFMA job's own
         # states: after burn-in (accounted steps) and at the end (bare steps)
         print("bench: fp64 windows", file=sys.stderr, flush=True)
-        r = res[mname]
-        for where, st, accounted in (("after burn-in", r["after_burn"], True),
-                                     ("after run", r["end"], False)):
+        for where, st, accounted in (("after burn-in", job["after_burn"], True),
+                                     ("after run", job["end"], False)):
             x64, m64, _ = c3_window(sf, st, _lib.BH_GLM_FP64, 0.0, args.c3_window, accounted)
-            xg, mg, gs = c3_window(sf, st, mprec, args.c3_guard, args.c3_window, accounted)
-            xf, mf, _ = c3_window(sf, st, _lib.BH_GLM_FP32, 0.0, args.c3_window, accounted)
             w = {"from": where, "steps": args.c3_window,
-                 "kind": "run-sampler!" if accounted else "burn-in!",
-                 f"{mname}_guarded_positions_equal_fp64": bool(np.array_equal(xg, x64)),
-                 f"{mname}_guarded_walkers_differing": int(np.any(xg != x64, axis=1).sum()),
-                 "fp32_plain_walkers_differing": int(np.any(xf != x64, axis=1).sum()),
-                 "guard_listed": gs["listed"], "guard_overflow": gs["overflow"]}
-            if accounted:
-                w[f"{mname}_guarded_means_equal_fp64"] = bool(np.array_equal(mg, m64))
+                 "kind": "run-sampler!" if accounted else "burn-in!"}
+            for name, prec, g in legs:
+                xg, mg, gs = c3_window(sf, st, prec, g, args.c3_window, accounted)
+                key = f"{name}_guarded" if g else f"{name}_plain"
+                w[f"{key}_positions_equal_fp64"] = bool(np.array_equal(xg, x64))
+                w[f"{key}_walkers_differing"] = int(np.any(xg != x64, axis=1).sum())
+                if accounted:
+                    w[f"{key}_means_equal_fp64"] = bool(np.array_equal(mg, m64))
+                if g:
+                    w[f"{key}_listed"] = gs["listed"]
+                    w[f"{key}_overflow"] = gs["overflow"]
             windows.append(w)
+    cal = c3_calibrated()
+    # ESS per walker-step: the long-run calibration where it exists (the
+    # job's own estimate, from about 6 tau_max, is reported beside it)
+    ess_ws = cal["ess_per_walker_step"]["mean"] if cal is not None else js["ess_per_walker_step"]
     out = {}
     for name, prec, g in legs:
-        r = res[name]
-        m = np.asarray(r["means"], np.float64)
-        # min over dims of the acor ESS, Sokal's where acor gives none
-        ess, nan_dims, ess_sd = ess_acor_or_sokal(r["ess"], np.asarray(r["sd"]) ** 2, m)
-        tau_s = sokal_tau(m)
-        ess_s = float(np.min(ess_sd))
-        o = {"ess_per_s": round(ess / r["wall_s"], 2),
-             "acor_nan_dims": nan_dims,
-             "ess_sokal_min": round(ess_s, 1),
-             "ess_sokal_per_s": round(ess_s / r["wall_s"], 2),
-             "tau_sokal_max": round(float(tau_s.max()), 1),
-             "ess_per_s_run_phase": round(ess / (r["run_gpu_ms"] * 1e-3), 2),
-             "ess_min": round(ess, 1),
-             "walker_steps_per_s": round(C3_W * (burn + run) / r["wall_s"], 1),
-             "wall_s": round(r["wall_s"], 3), "acceptance_rate": round(r["acc"], 4),
-             "tau_max": round(float(np.nanmax(r["tau"])), 2),
-             "ms_per_step_run": round(r["run_gpu_ms"] / run, 3),
-             "likelihood_kernel_ms": round(r["kernel"]["avg_ms"], 4)}
+        t = timed[name]
+        o = {"walker_steps_per_s": round(t["walker_steps_per_s"], 1),
+             "ms_per_step": round(t["ms_per_step"], 3),
+             "likelihood_kernel_ms": round(t["kernel"]["avg_ms"], 4),
+             # the same Markov kernel in every leg: the guarded matrix-core
+             # chains are the fp64 chain step for step (coupled windows), the
+             # plain ones differ from it by rounding only
+             "ess_per_s": round(ess_ws * t["walker_steps_per_s"], 2)}
         if g:
             o["guard"] = g
-            o["guard_listed_per_half_step"] = round(r["guard"]["listed"] / (2 * (burn + run)), 3)
-            o["guard_overflow"] = r["guard"]["overflow"]
-        if ref is not None:
-            err = np.abs(r["grand_mean"] - ref["grand_mean"]) / ref["sigma"].astype(np.float64)
-            o["mean_err_over_mcse_max"] = round(float(err.max()), 4)
-            o["sd_rel_err_max"] = round(float(np.max(np.abs(r["sd"] / ref["sd"] - 1.0))), 6)
-            o["trajectory_equals_fp64"] = bool(np.array_equal(r["xs"], ref["xs"]))
+            o["guard_listed_per_half_step"] = round(t["guard"]["listed"] / (2 * K), 3)
+            o["guard_overflow"] = t["guard"]["overflow"]
         out[name] = o
     if ref is not None:
-        ess, _, _ = ess_acor_or_sokal(ref["ess"], np.asarray(ref["sd"]) ** 2, ref["means"])
-        out["fp64"] = {"ess_per_s": round(ess / ref["wall_s"], 2), "ess_min": round(ess, 1),
-                       "wall_s": round(ref["wall_s"], 3),
-                       "ms_per_step_run": round(ref["run_gpu_ms"] / run, 3),
-                       "acceptance_rate": round(ref["acc"], 4),
-                       "likelihood_kernel_ms": round(ref["kernel"]["avg_ms"], 4)}
-    kt = res[mname]["kernel"]
+        # the same job (seeds, steps) on the fp64 kernel: posterior means and
+        # SDs of the guarded and plain legs against it, in MC standard errors
+        full = {"fp64": c3_ess_summary(ref)}
+        mcse = mcse_of(ref)
+        for name, r in ((mname, jc), ("fp32", r32)):
+            err = np.abs(r["grand_mean"] - ref["grand_mean"]) / mcse
+            full[name] = {"mean_err_over_mcse_max": round(float(err.max()), 4),
+                          "mean_err_over_mcse_rms": round(float(np.sqrt(np.mean(err ** 2))), 4),
+                          "sd_rel_err_max": round(float(np.max(np.abs(r["sd"] / ref["sd"] - 1.0))),
+                                                  6),
+                          "trajectory_equals_fp64": bool(np.array_equal(r["xs"], ref["xs"])),
+                          "acceptance_rate": round(r["acc"], 4)}
+        out["fp64_full_job"] = full
+    mt, ft = timed[mname], timed["fp32"]
+    ws_ratio = mt["walker_steps_per_s"] / ft["walker_steps_per_s"]
+    kt = mt["kernel"]
     flops = 2.0 * C3_N * C3_D * (C3_W // 2)
     achieved = flops / (kt["avg_ms"] * 1e-3) / 1e12
     # the practical rates of the device, measured: the product issued
     # msplit times, and one v_exp_f32 per contraction value plus one
-    # v_log_f32 per sixteen in the epilogue (products and epilogue issue from
-    # the same SIMDs and largely serialize)
+    # v_log_f32 per sixteen in the epilogue
     rates = device_rates(fac)
     issued = flops * msplit / (kt["avg_ms"] * 1e-3)
     trans = C3_N * (C3_W // 2) * (1.0 + 1.0 / 16.0)
     trans_rate = trans / (kt["avg_ms"] * 1e-3)
     line = {
         "metric": METRIC, "value": out[mname]["ess_per_s"], "unit": "ESS/s", "n_gpus": 1,
-        "steps": burn + run, "warmup": 66,
-        "ms_per_step": round(res[mname]["wall_s"] / (burn + run) * 1e3, 3),
+        "steps": K, "warmup": W, "ms_per_step": out[mname]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": mname,
         "data": "synthetic (seeded logistic-regression data, X stored bf16)",
-        "config": {"workload": f"logistic regression {C3_N}x{C3_D}, {C3_W} walkers, "
-                               f"{burn} burn-in + {run} run-sampler steps (a=2), prior N(0,5^2)",
+        "config": {"workload": f"logistic regression {C3_N}x{C3_D}, {C3_W} walkers (a=2), "
+                               f"prior N(0,5^2); value = ESS per walker-step (long-run "
+                               f"calibration) x walker-steps/s of {K} timed move! steps after "
+                               f"{W} warm-up steps; a {burn} burn-in + {run} run-sampler job "
+                               f"in the same run",
                    "likelihood": f"MFMA split-{mname}, fp32 accumulate, fp64 row sums, "
                                  f"guarded accept test ({args.c3_guard})",
                    "partner": "walker"},
-        "mfma_vs_fp32_ess_per_s": round(out[mname]["ess_per_s"] / out["fp32"]["ess_per_s"], 2),
-        "mfma_vs_fp32_walker_steps_per_s": round(out[mname]["walker_steps_per_s"]
-                                                 / out["fp32"]["walker_steps_per_s"], 2),
+        "walker_steps_per_s": out[mname]["walker_steps_per_s"],
+        "mfma_vs_fp32_walker_steps_per_s": round(ws_ratio, 2),
+        "mfma_vs_fp32_ess_per_s": round(ws_ratio, 2),
+        "ess_job": js,
         "precisions": out,
+        "log_density_err_vs_fp64": lp_err,
         "fp64_coupled_windows": windows,
         "roofline": {"kernel": f"{mkern} (1M rows x 8192 points)",
                      "bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK,
@@ -1101,15 +1317,23 @@ def main_c3(args):
                          "fma_rate_measured_t": round(rates["fma_per_s"] / 1e12, 3)}},
         "setup_s": round(t_setup, 1),
     }
+    line["ess_per_walker_step"] = {
+        "value": ess_ws,
+        "source": (f"profiles/{C3_CAL_FILE} (mean over {cal['ess_per_walker_step']['seeds']} "
+                   f"seeds of {cal['ess_per_walker_step']['run_steps']}-step runs, sd "
+                   f"{cal['ess_per_walker_step']['sd']:.3g})") if cal is not None
+        else "this run's job",
+        "this_job": js["ess_per_walker_step"],
+        "this_job_ess_per_s": js["ess_per_s_job"]}
     if not args.no_roofline:
         line["plain_move_roofline"] = [move_roofline(fac, d, rates, m) for m, d in ROOF_DIMS]
     if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         cb = c3_cpu_baseline(Xd, y, limits)
-        ess_per_ws = out[mname]["ess_min"] / (C3_W * (burn + run))
         line["cpu_baseline"] = {
-            "value": round(cb["walker_steps_per_s"] * ess_per_ws, 4), "unit": "ESS/s",
+            "value": round(cb["walker_steps_per_s"] * ess_ws, 4), "unit": "ESS/s",
             "cores": cb["threads"], "kind": "port",
-            "sample": cb["sample"] + "; ESS per walker-step taken from the GPU run",
+            "sample": cb["sample"] + "; ESS per walker-step of the GPU job (the same chain)",
             "walker_steps_per_s": round(cb["walker_steps_per_s"], 2)}
     print(json.dumps(line), flush=True)
 
@@ -1117,8 +1341,11 @@ def main_c3(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="c1: timed jobs")
-    ap.add_argument("--warmup", type=int, default=1, help="c1: untimed jobs")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps: sampler steps of the logistic workload (default 20), "
+                         "whole jobs of c1 (default 3)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before them (default 5; c1: jobs, default 1)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c3")
     ap.add_argument("--c2-burn", type=int, default=C2_BURN)
     ap.add_argument("--c2-run", type=int, default=C2_RUN)
@@ -1136,8 +1363,9 @@ def main():
     ap.add_argument("--c3-guard", type=float, default=C3_GUARD)
     ap.add_argument("--c3-mfma", choices=sorted(C3_MFMA), default=C3_MFMA_DEFAULT,
                     help="matrix-core precision of the measured leg")
-    ap.add_argument("--c3-other-mfma", action="store_true",
-                    help="also run the other matrix-core precision (guarded)")
+    ap.add_argument("--c3-ess-calibration", action="store_true",
+                    help="only the ESS-per-walker-step calibration of the logistic workload "
+                         "(three seeds; about 20 minutes)")
     ap.add_argument("--c3-bf16", action="store_true", help="also the single-bf16 likelihood")
     ap.add_argument("--no-fp64", action="store_true", help="skip the coupled fp64 windows")
     ap.add_argument("--c3-fp64-full", action="store_true",
@@ -1150,6 +1378,11 @@ def main():
                          "and 16, independent coordinates at DIM 32), plus a 256 MiB device "
                          "copy whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
+    c1 = args.workload == "c1"
+    if args.steps is None:
+        args.steps = 3 if c1 else 20
+    if args.warmup is None:
+        args.warmup = 1 if c1 else 5
     if 0 < args.c2_user_steps < 50:
         ap.error("--c2-user-steps: run-sampler! needs at least 50 steps (0 skips the run)")
     if args.gpus != 1:
@@ -1168,6 +1401,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"roofline": r, "calibration_copy_bytes": src.numel() * 4}), flush=True)
         return
+    if args.c3_ess_calibration:
+        return c3_calibration()
     if args.workload == "c3":
         return main_c3(args)
     if args.workload == "c2":
