@@ -1317,6 +1317,16 @@ def main_c3(args):
                          "fma_rate_measured_t": round(rates["fma_per_s"] / 1e12, 3)}},
         "setup_s": round(t_setup, 1),
     }
+    fj = os.path.join(ROOT, "profiles", "glm_logistic_1Mx64_fp64_full_job.json")
+    if os.path.exists(fj):
+        with open(fj) as f:
+            fjd = json.load(f)
+        g = fjd["fp64_full_job"].get(mname, {})
+        line["fp64_full_job"] = {"profile": "profiles/glm_logistic_1Mx64_fp64_full_job.json",
+                                 "steps": "500 burn-in + 2000 run-sampler",
+                                 f"{mname}_trajectory_equals_fp64": g.get("trajectory_equals_fp64"),
+                                 f"{mname}_mean_err_over_mcse_max": g.get("mean_err_over_mcse_max"),
+                                 f"{mname}_sd_rel_err_max": g.get("sd_rel_err_max")}
     line["ess_per_walker_step"] = {
         "value": ess_ws,
         "source": (f"profiles/{C3_CAL_FILE} (mean over {cal['ess_per_walker_step']['seeds']} "
