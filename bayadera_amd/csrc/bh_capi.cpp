@@ -274,7 +274,7 @@ struct GlmData {
     DevBuf lin;               // logistic: double[d]: X'y; linear regression: the
                               // four constants of its log posterior
     DevBuf lin_half;          // logistic: double[d]: X'y - (1/2) X'1 (log2-unit kernels)
-    DevBuf X16;               // logistic: fp16 copy of X, made on first fp16 use
+    DevBuf X16;               // logistic: fp16 copy of X in fragment order, made on first fp16 use
     mutable unsigned bpc[BH_GLM_FP16X2 + 1] = {};  // resident work-groups per CU, by precision
     DevBuf Xf, y;             // linear regression: fp32 N x D and N
 };
@@ -499,12 +499,11 @@ int glm_prepare(bh_stretch_factory* f, int prec) {
     if (prec != BH_GLM_FP16X2 || g.X16.p) return BH_OK;
     bh_ctx* c = f->ctx;
     BH_HIP(hipSetDevice(c->device));
-    const unsigned long long count = g.N * g.d;
     DevBuf x16, over;
-    BH_TRY(x16.alloc(count * sizeof(uint16_t)));
+    BH_TRY(x16.alloc(bh::glm_x_f16_bytes(g.N, g.d)));
     BH_TRY(over.alloc(sizeof(unsigned)));
     BH_HIP(hipMemsetAsync(over.p, 0, sizeof(unsigned), c->stream));
-    BH_HIP(bh::launch_glm_x_to_f16(c->stream, g.X.p, count, x16.p, over.as<unsigned>()));
+    BH_HIP(bh::launch_glm_x_to_f16(c->stream, g.X.p, g.N, g.d, x16.p, over.as<unsigned>()));
     unsigned bad = 0;
     BH_HIP(hipMemcpyAsync(&bad, over.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     BH_TRY(sync(c->stream));

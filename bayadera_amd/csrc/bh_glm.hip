@@ -58,6 +58,36 @@ struct Rows16 {
     }
 };
 
+// A fragments from the fp16 copy of X, stored in fragment order: for each
+// 16-row block b and 32-feature chunk kc, the 64 lanes' eight values (lane
+// l: row 16 b + l % 16, features 32 kc + 8 (l / 16) .. +7) one after the
+// other, rows past N zero.  One fragment load of a wave reads 1 KB of
+// consecutive bytes (the row-major layout reads half of each of 16 lines).
+template <int D, class T, class V>
+struct Frag16 {
+    using vec = V;
+    const T* X;
+    unsigned lr, lg;
+    __device__ const T* at(unsigned rb, int kc) const {
+        return X + ((size_t)(rb / 16) * (D / 32) + kc) * 512 + (lg * 16 + lr) * 8;
+    }
+    __device__ void full(unsigned rb, V (&a)[D / 32]) const {
+#pragma unroll
+        for (int kc = 0; kc < D / 32; ++kc) a[kc] = *reinterpret_cast<const V*>(at(rb, kc));
+    }
+    __device__ void masked(unsigned rb, unsigned r1, V (&a)[D / 32]) const {
+        const bool live = rb + lr < r1;
+#pragma unroll
+        for (int kc = 0; kc < D / 32; ++kc) {
+            a[kc] = *reinterpret_cast<const V*>(at(rb, kc));
+            if (!live) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[kc][j] = (T)0.0f;
+            }
+        }
+    }
+};
+
 // MFMA kernel: v_mfma_f32_16x16x32_bf16 (T = __bf16) or _f16 (T =
 // _Float16).  A block of 4 waves owns 256 points, 64 per wave as four
 // 16-point column blocks whose B fragments (all D features, NS split terms)
@@ -68,7 +98,8 @@ struct Rows16 {
 // (glm_mfma_rows) is software-pipelined: the matrix-core products of row
 // block i+1 are issued with the softplus epilogue of block i placed between
 // them.  Split terms are accumulated smallest first.  Rows are masked only in
-// the last block.
+// the last block.  The fp16 kernels read X in fragment order (Frag16), the
+// bf16 ones the caller's row-major bf16 X (Rows16).
 template <int D, int NS, class T, class V>
 __global__ void __launch_bounds__(256)
 glm_logistic_mfma_kernel(const T* __restrict__ X, unsigned N, const T* __restrict__ ys, unsigned n,
@@ -96,7 +127,12 @@ glm_logistic_mfma_kernel(const T* __restrict__ X, unsigned N, const T* __restric
     const unsigned r0 = range * rows_per_range;
     const unsigned r1 = (r0 >= N) ? r0 : ((N - r0 < rows_per_range) ? N : r0 + rows_per_range);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (r0 < r1) glm_mfma_rows<D, NS>(Rows16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
+    if (r0 < r1) {
+        if constexpr (__is_same(T, _Float16))
+            glm_mfma_rows<D, NS>(Frag16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
+        else
+            glm_mfma_rows<D, NS>(Rows16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
+    }
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {
         double v = acc[wb];
@@ -106,13 +142,19 @@ glm_logistic_mfma_kernel(const T* __restrict__ X, unsigned N, const T* __restric
     }
 }
 
-// fp16 copy of bf16 X for the fp16 matrix-core kernels, eight values per
-// thread; counts the values outside the fp16 range (|x| > 65504).
+// fp16 copy of bf16 X (N x D row-major) for the fp16 matrix-core kernels,
+// in Frag16's fragment order, eight values (one 16-byte piece of a row) per
+// thread; counts the values outside the fp16 range (|x| > 65504).  The
+// padding rows of the last block are left to the caller (zeroed).
+template <int D>
 __global__ void __launch_bounds__(256)
 glm_x_to_f16_kernel(const __bf16* __restrict__ X, unsigned long long n8,
                     _Float16* __restrict__ out, unsigned* __restrict__ overflow) {
     const unsigned long long i = (unsigned long long)blockIdx.x * 256u + threadIdx.x;
     if (i >= n8) return;
+    const unsigned long long r = i / (D / 8);
+    const unsigned c = (unsigned)(i % (D / 8));
+    const unsigned long long o = ((r / 16) * (D / 32) + c / 4) * 64 + (c % 4) * 16 + r % 16;
     const bf16x8 v = reinterpret_cast<const bf16x8*>(X)[i];
     f16x8 h;
     unsigned bad = 0;
@@ -122,7 +164,7 @@ glm_x_to_f16_kernel(const __bf16* __restrict__ X, unsigned long long n8,
         bad += __builtin_fabsf(f) > 65504.0f ? 1u : 0u;
         h[j] = (_Float16)f;
     }
-    reinterpret_cast<f16x8*>(out)[i] = h;
+    reinterpret_cast<f16x8*>(out)[o] = h;
     if (bad) atomicAdd(overflow, bad);
 }
 
@@ -655,13 +697,24 @@ hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks
     return hipGetLastError();
 }
 
-hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long count, void* out,
-                               unsigned* overflow) {
-    if (count % 8) return hipErrorInvalidValue;
-    const unsigned long long n8 = count / 8;
+size_t glm_x_f16_bytes(unsigned long long N, unsigned d) {
+    return (size_t)((N + 15) / 16) * 16 * d * sizeof(_Float16);
+}
+
+hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                               void* out, unsigned* overflow) {
+    if (d != 32 && d != 64) return hipErrorInvalidValue;
+    const unsigned long long n8 = N * d / 8;
     if (n8 == 0) return hipSuccess;
-    glm_x_to_f16_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, s>>>(
-        static_cast<const __bf16*>(X), n8, static_cast<_Float16*>(out), overflow);
+    const hipError_t e = hipMemsetAsync(out, 0, glm_x_f16_bytes(N, d), s);
+    if (e != hipSuccess) return e;
+    const unsigned blocks = (unsigned)((n8 + 255) / 256);
+    if (d == 64)
+        glm_x_to_f16_kernel<64><<<blocks, 256, 0, s>>>(static_cast<const __bf16*>(X), n8,
+                                                       static_cast<_Float16*>(out), overflow);
+    else
+        glm_x_to_f16_kernel<32><<<blocks, 256, 0, s>>>(static_cast<const __bf16*>(X), n8,
+                                                       static_cast<_Float16*>(out), overflow);
     return hipGetLastError();
 }
 

@@ -31,10 +31,13 @@ hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks
 // n16 random 16-byte gathers from src (n16 x 16 bytes; n16 a power of two and
 // a multiple of 256), one per thread, sums to out (n16 floats).
 hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16);
-// fp16 copy of `count` bf16 values (a multiple of 8); *overflow is
-// incremented by the number of values outside the fp16 range.
-hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long count, void* out,
-                               unsigned* overflow);
+// fp16 copy of the N x d bf16 X (d = 32 or 64) in the fragment order the fp16
+// matrix-core kernels read (glm_x_f16_bytes bytes: whole 16-row blocks, the
+// padding rows zero); *overflow is incremented by the number of values
+// outside the fp16 range.
+size_t glm_x_f16_bytes(unsigned long long N, unsigned d);
+hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                               void* out, unsigned* overflow);
 // fp64 softplus sums of the points listed by the guarded accept test: pts
 // (double, 2 lmax x d) holds in rows i < c the proposals and in rows
 // lmax + i their walkers' positions (c = min(*count, lmax), read on the
