@@ -114,6 +114,8 @@ SIGNATURES = {
     "bh_glm_batch_create": [_P, _I, _U32, _P, _P, _P, _P, _P, _F, _U32, _PP],
     "bh_glm_batch_destroy": [_P],
     "bh_glm_batch_set_precision": [_P, _I],
+    "bh_glm_batch_set_tiles": [_P, _U32, _U32, _U32],
+    "bh_glm_batch_autotune": [_P, _U32, _P, _P, _P],
     "bh_glm_batch_init_position": [_P, _U32, _P],
     "bh_glm_batch_burn_in": [_P, _U32, _U64, _F],
     "bh_glm_batch_run": [_P, _U32, _U64, _F, _P, _P, _P, _P],

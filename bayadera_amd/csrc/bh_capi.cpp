@@ -2048,6 +2048,11 @@ struct bh_glm_batch {
     int prec = BH_GLM_BF16X3;
     double prior_prec = 1.0;
     std::vector<uint32_t> dims_h;
+    std::vector<uint64_t> rows_h;
+    // tile shape of the matrix-core contraction per row stride (index 0:
+    // stride 32, 1: stride 64): rows per unit, 16-point column blocks per wave
+    uint32_t unit_rows[2] = {bh::kGbUnitRows, bh::kGbUnitRows};
+    uint32_t col_blocks[2] = {4, 4};
     DevBuf dims, scales, tscale, X8, xoff, units32, units64, ubeg, lin, lin_half;
     DevBuf xs, lp, Y, z, u2, ysc, ysplit, partial, acc_cnt, msum, msum2, acc1, acc2;
     uint32_t bare_counter = 0;
@@ -2067,18 +2072,59 @@ float fp8_e4m3_value(uint8_t c) {
     return s ? -v : v;
 }
 
+bool gb_mfma(int prec) {
+    return prec == BH_GLM_BF16 || prec == BH_GLM_BF16X3 || prec == BH_GLM_FP16X2;
+}
+
+// The contraction units of every problem: its rows in pieces of the tile's
+// unit rows of its stride class, the units of one stride in problem order,
+// global unit ids in problem order (ubeg[p] the first of problem p).
+int gb_build_units(bh_glm_batch* b) {
+    std::vector<uint32_t> u32, u64, ub(b->P + 1);
+    uint32_t uid = 0;
+    for (uint32_t p = 0; p < b->P; ++p) {
+        const int cls = b->dims_h[p] <= 32 ? 0 : 1;
+        const unsigned long long N = b->rows_h[p], ur = b->unit_rows[cls];
+        auto& dst = cls == 0 ? u32 : u64;
+        ub[p] = uid;
+        for (unsigned long long r0 = 0; r0 < N; r0 += ur) {
+            const unsigned long long r1 = std::min<unsigned long long>(N, r0 + ur);
+            dst.insert(dst.end(), {p, (uint32_t)r0, (uint32_t)r1, uid++});
+        }
+    }
+    ub[b->P] = uid;
+    // the kernels running on the stream may still read the old units
+    BH_TRY(sync(b->stream));
+    b->U = uid;
+    b->n32 = (uint32_t)(u32.size() / 4);
+    b->n64 = (uint32_t)(u64.size() / 4);
+    auto up = [&](DevBuf& d, const void* h, size_t bytes) -> int {
+        BH_TRY(d.alloc(std::max<size_t>(bytes, 16)));
+        if (bytes) BH_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, b->stream));
+        return BH_OK;
+    };
+    BH_TRY(up(b->units32, u32.data(), u32.size() * 4));
+    BH_TRY(up(b->units64, u64.data(), u64.size() * 4));
+    BH_TRY(up(b->ubeg, ub.data(), ub.size() * 4));
+    BH_TRY(b->partial.alloc((size_t)std::max(uid, 1u) * b->W * 8));
+    return sync(b->stream);
+}
+
+// one stride class's contraction of n points (cls 0: stride 32, 1: 64)
+int gb_contract_class(bh_glm_batch* b, int cls, const float* Ypts, unsigned n) {
+    const int cb = gb_mfma(b->prec) ? (int)b->col_blocks[cls] : 4;
+    BH_HIP(bh::launch_gb_contract(b->prec, cls == 0 ? 32 : 64, cb, b->stream,
+                                  b->X8.as<unsigned char>(), b->xoff.as<unsigned long long>(),
+                                  cls == 0 ? b->units32.p : b->units64.p,
+                                  cls == 0 ? b->n32 : b->n64, b->P, b->scales.as<float>(), Ypts,
+                                  b->ysc.as<float>(), b->ysplit.p, n, b->partial.as<double>()));
+    return BH_OK;
+}
+
 // log posteriors of the walkers of every problem (n = W points each)
 int gb_contract(bh_glm_batch* b, const float* Ypts, unsigned n) {
-    const unsigned char* x8 = b->X8.as<unsigned char>();
-    const unsigned long long* xoff = b->xoff.as<unsigned long long>();
-    double* partial = b->partial.as<double>();
-    BH_HIP(bh::launch_gb_contract(b->prec, 32, b->stream, x8, xoff, b->units32.p, b->n32, b->P,
-                                  b->scales.as<float>(), Ypts, b->ysc.as<float>(), b->ysplit.p, n,
-                                  partial));
-    BH_HIP(bh::launch_gb_contract(b->prec, 64, b->stream, x8, xoff, b->units64.p, b->n64, b->P,
-                                  b->scales.as<float>(), Ypts, b->ysc.as<float>(), b->ysplit.p, n,
-                                  partial));
-    return BH_OK;
+    BH_TRY(gb_contract_class(b, 0, Ypts, n));
+    return gb_contract_class(b, 1, Ypts, n);
 }
 
 const double* gb_lin(const bh_glm_batch* b) {
@@ -2196,8 +2242,6 @@ int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t
     std::vector<uint8_t> xpad(total, 0);
     std::vector<float> sc((size_t)problems * 64, 0.0f), ts((size_t)problems * 64, 0.0f);
     std::vector<double> lin((size_t)problems * 64, 0.0), linh((size_t)problems * 64, 0.0);
-    std::vector<uint32_t> u32, u64, ub(problems + 1);
-    uint32_t uid = 0;
     for (uint32_t p = 0; p < problems; ++p) {
         const unsigned d = dims[p], D = d <= 32 ? 32 : 64;
         const unsigned long long N = rows[p];
@@ -2224,20 +2268,11 @@ int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t
             lin[(size_t)p * 64 + j] = l[j];
             linh[(size_t)p * 64 + j] = l[j] - 0.5 * cs[j];
         }
-        ub[p] = uid;
-        for (unsigned long long r0 = 0; r0 < N; r0 += bh::kGbUnitRows) {
-            const unsigned long long r1 = std::min<unsigned long long>(N, r0 + bh::kGbUnitRows);
-            auto& dst = D == 32 ? u32 : u64;
-            dst.insert(dst.end(), {p, (uint32_t)r0, (uint32_t)r1, uid++});
-        }
         xin_off += N * d;
         yoff += N;
         soff += d;
     }
-    ub[problems] = uid;
-    b->U = uid;
-    b->n32 = (uint32_t)(u32.size() / 4);
-    b->n64 = (uint32_t)(u64.size() / 4);
+    b->rows_h.assign(rows, rows + problems);
     const uint32_t P = problems, W = walkers, K = walkers / 2;
     BH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     BH_HIP(hipEventCreate(&b->ev0));
@@ -2252,9 +2287,6 @@ int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t
     BH_TRY(up(b->tscale, ts.data(), ts.size() * 4));
     BH_TRY(up(b->X8, xpad.data(), xpad.size()));
     BH_TRY(up(b->xoff, xoff.data(), xoff.size() * 8));
-    BH_TRY(up(b->units32, u32.data(), u32.size() * 4));
-    BH_TRY(up(b->units64, u64.data(), u64.size() * 4));
-    BH_TRY(up(b->ubeg, ub.data(), ub.size() * 4));
     BH_TRY(up(b->lin, lin.data(), lin.size() * 8));
     BH_TRY(up(b->lin_half, linh.data(), linh.size() * 8));
     BH_TRY(b->xs.alloc((size_t)P * W * 64 * 4));
@@ -2264,7 +2296,7 @@ int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t
     BH_TRY(b->u2.alloc((size_t)P * K * 4));
     BH_TRY(b->ysc.alloc((size_t)P * W * 64 * 4));
     BH_TRY(b->ysplit.alloc((size_t)3 * P * W * 64 * 2));
-    BH_TRY(b->partial.alloc((size_t)std::max(uid, 1u) * W * 8));
+    BH_TRY(gb_build_units(b.get()));
     BH_TRY(b->acc_cnt.alloc((size_t)P * 8));
     BH_TRY(b->msum.alloc((size_t)P * (W / 256) * 64 * 4));
     BH_TRY(b->msum2.alloc((size_t)P * (W / 256) * 64 * 4));
@@ -2292,6 +2324,87 @@ int bh_glm_batch_set_precision(bh_glm_batch* b, int precision) {
     BH_ARG(b, "NULL batch");
     BH_ARG(precision >= BH_GLM_FP64 && precision <= BH_GLM_FP16X2, "bad precision %d", precision);
     b->prec = precision;
+    return BH_OK;
+}
+
+static int gb_check_tiles(const bh_glm_batch* b, uint32_t row_stride, uint32_t unit_rows,
+                          uint32_t col_blocks) {
+    BH_ARG(row_stride == 32 || row_stride == 64, "row stride %u is not 32 or 64", row_stride);
+    BH_ARG(unit_rows >= 256 && unit_rows <= 65536 && unit_rows % 16 == 0,
+           "unit rows %u outside [256, 65536] or not a multiple of 16", unit_rows);
+    BH_ARG(col_blocks == 4 || (col_blocks == 8 && row_stride == 32),
+           "column blocks %u: 4, or 8 for row stride 32", col_blocks);
+    BH_ARG(b->K % (64 * col_blocks) == 0, "%u column blocks need a multiple of %u walkers",
+           col_blocks, 128 * col_blocks);
+    return BH_OK;
+}
+
+int bh_glm_batch_set_tiles(bh_glm_batch* b, uint32_t row_stride, uint32_t unit_rows,
+                           uint32_t col_blocks) {
+    BH_ARG(b, "NULL batch");
+    BH_TRY(gb_check_tiles(b, row_stride, unit_rows, col_blocks));
+    BH_HIP(hipSetDevice(b->ctx->device));
+    const int cls = row_stride == 32 ? 0 : 1;
+    b->unit_rows[cls] = unit_rows;
+    b->col_blocks[cls] = col_blocks;
+    BH_TRY(gb_build_units(b));
+    // the stored log posteriors are sums over units: recomputed over the
+    // new ones
+    BH_TRY(gb_logpost_all(b));
+    return sync(b->stream);
+}
+
+int bh_glm_batch_autotune(bh_glm_batch* b, uint32_t reps, uint32_t* unit_rows,
+                          uint32_t* col_blocks, float* best_ms) {
+    BH_ARG(b, "NULL batch");
+    BH_ARG(reps >= 1 && reps <= 1000, "reps %u outside [1, 1000]", reps);
+    BH_HIP(hipSetDevice(b->ctx->device));
+    static const uint32_t kRows[] = {1024, 2048, 4096, 8192, 16384};
+    const uint32_t keep_rows[2] = {b->unit_rows[0], b->unit_rows[1]};
+    const uint32_t keep_cb[2] = {b->col_blocks[0], b->col_blocks[1]};
+    uint32_t best_rows[2] = {keep_rows[0], keep_rows[1]}, best_cb[2] = {keep_cb[0], keep_cb[1]};
+    float best[2] = {0.0f, 0.0f};
+    // the shape of a half-step's contraction: K points (the current
+    // positions' first K of every problem, split as the precision needs)
+    const unsigned n = b->K;
+    BH_HIP(bh::launch_gb_points(b->stream, b->xs.as<float>(), b->dims.as<unsigned>(),
+                                b->tscale.as<float>(), b->P, n, b->ysc.as<float>(), b->ysplit.p,
+                                glm_nsplit(b->prec), glm_f16(b->prec)));
+    for (int cls = 0; cls < 2; ++cls) {
+        for (uint32_t ur : kRows) {
+            for (uint32_t cb : {4u, 8u}) {
+                if (cb == 8 && (cls == 1 || !gb_mfma(b->prec) || b->K % 512 != 0)) continue;
+                b->unit_rows[cls] = ur;
+                b->col_blocks[cls] = cb;
+                BH_TRY(gb_build_units(b));
+                if ((cls == 0 ? b->n32 : b->n64) == 0) continue;
+                BH_TRY(gb_contract_class(b, cls, b->xs.as<float>(), n));  // warm-up
+                BH_HIP(hipEventRecord(b->ev0, b->stream));
+                for (uint32_t r = 0; r < reps; ++r)
+                    BH_TRY(gb_contract_class(b, cls, b->xs.as<float>(), n));
+                BH_HIP(hipEventRecord(b->ev1, b->stream));
+                BH_HIP(hipEventSynchronize(b->ev1));
+                float ms = 0.0f;
+                BH_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                ms /= (float)reps;
+                if (best[cls] == 0.0f || ms < best[cls]) {
+                    best[cls] = ms;
+                    best_rows[cls] = ur;
+                    best_cb[cls] = cb;
+                }
+            }
+        }
+        b->unit_rows[cls] = best_rows[cls];
+        b->col_blocks[cls] = best_cb[cls];
+    }
+    BH_TRY(gb_build_units(b));
+    BH_TRY(gb_logpost_all(b));
+    BH_TRY(sync(b->stream));
+    for (int cls = 0; cls < 2; ++cls) {
+        if (unit_rows) unit_rows[cls] = b->unit_rows[cls];
+        if (col_blocks) col_blocks[cls] = b->col_blocks[cls];
+        if (best_ms) best_ms[cls] = best[cls];
+    }
     return BH_OK;
 }
 

@@ -129,9 +129,9 @@ glm_logistic_mfma_kernel(const T* __restrict__ X, unsigned N, const T* __restric
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (r0 < r1) {
         if constexpr (__is_same(T, _Float16))
-            glm_mfma_rows<D, NS>(Frag16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
+            glm_mfma_rows<D, NS, 4>(Frag16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
         else
-            glm_mfma_rows<D, NS>(Rows16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
+            glm_mfma_rows<D, NS, 4>(Rows16<D, T, V>{X, lr, lg}, bfrag, r0, r1, lg, acc);
     }
 #pragma unroll
     for (int wb = 0; wb < 4; ++wb) {

@@ -340,8 +340,11 @@ __device__ __forceinline__ bool gb_tile(unsigned nunits, unsigned groups, unsign
 
 // MFMA contraction of the units of one row stride D (units: {problem, r0,
 // r1, global unit}); partial[unit][point] as the single-posterior kernel
-// (log2 units, |u|/2 + log2(1 + 2^-|u|) summed over the unit's rows).
-template <int D, int NS, class T, class V>
+// (log2 units, |u|/2 + log2(1 + 2^-|u|) summed over the unit's rows).  Each
+// wave holds the B fragments of CB 16-point column blocks (a block of four
+// waves: 64 CB points); CB and the unit rows are the tile shape the batch's
+// autotuner chooses per row stride.
+template <int D, int NS, int CB, class T, class V>
 __global__ void __launch_bounds__(256)
 gb_mfma_kernel(const unsigned char* __restrict__ X8, const unsigned long long* __restrict__ xoff,
                const uint4* __restrict__ units, unsigned nunits, unsigned P,
@@ -354,10 +357,10 @@ gb_mfma_kernel(const unsigned char* __restrict__ X8, const unsigned long long* _
     const unsigned p = unit.x;
     const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const unsigned lr = lane & 15u, lg = lane >> 4;
-    const unsigned w0 = g * 256u + wave * 64u;
-    V bfrag[4][KC][NS];
+    const unsigned w0 = g * (64u * CB) + wave * (16u * CB);
+    V bfrag[CB][KC][NS];
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb)
+    for (int wb = 0; wb < CB; ++wb)
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
@@ -365,11 +368,14 @@ gb_mfma_kernel(const unsigned char* __restrict__ X8, const unsigned long long* _
                 bfrag[wb][kc][s] = *reinterpret_cast<const V*>(
                     ysplit + (((size_t)s * P + p) * n + w0 + wb * 16 + lr) * kGbD + kc * 32 +
                     lg * 8);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (unit.y < unit.z)
-        glm_mfma_rows<D, NS>(Fp8Rows<D, V>{X8 + xoff[p], lr, lg}, bfrag, unit.y, unit.z, lg, acc);
+    double acc[CB];
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb) {
+    for (int wb = 0; wb < CB; ++wb) acc[wb] = 0.0;
+    if (unit.y < unit.z)
+        glm_mfma_rows<D, NS, CB>(Fp8Rows<D, V>{X8 + xoff[p], lr, lg}, bfrag, unit.y, unit.z, lg,
+                                 acc);
+#pragma unroll
+    for (int wb = 0; wb < CB; ++wb) {
         double v = acc[wb];
         v += __shfl_xor(v, 16, 64);
         v += __shfl_xor(v, 32, 64);
@@ -706,32 +712,49 @@ hipError_t launch_gb_propose(hipStream_t s, unsigned seed0, int half, unsigned o
     return hipGetLastError();
 }
 
-hipError_t launch_gb_contract(int precision, int D, hipStream_t s, const void* X8,
+hipError_t launch_gb_contract(int precision, int D, int CB, hipStream_t s, const void* X8,
                               const unsigned long long* xoff, const void* units, unsigned nunits,
                               unsigned P, const float* scales, const float* Ypts,
                               const float* ysc, const void* ysplit, unsigned n,
                               double* partial) {
     if (nunits == 0) return hipSuccess;
-    const unsigned groups = n / 256;
+    // CB = 8 (512 points per block) only for the matrix-core kernels of
+    // stride 32; the fp32 and fp64 kernels take 256 points per block
+    const bool mfma = precision == kGlmBf16x3 || precision == kGlmFp16x2 || precision == kGlmBf16;
+    if (CB != 4 && !(CB == 8 && D == 32 && mfma)) return hipErrorInvalidValue;
+    const unsigned pts = mfma ? 64u * CB : 256u;
+    if (n % pts) return hipErrorInvalidValue;
+    const unsigned groups = n / pts;
     const unsigned blocks = (nunits + 7) / 8 * 8 * groups;
     const unsigned char* x = static_cast<const unsigned char*>(X8);
     const uint4* un = static_cast<const uint4*>(units);
     const __bf16* ys = static_cast<const __bf16*>(ysplit);
     const _Float16* ys16 = static_cast<const _Float16*>(ysplit);
+#define BH_GB_MFMA(DD, CBV)                                                                     \
+    switch (precision) {                                                                        \
+    case kGlmBf16x3:                                                                            \
+        gb_mfma_kernel<DD, 3, CBV, __bf16, bf16x8><<<blocks, 256, 0, s>>>(                      \
+            x, xoff, un, nunits, P, ys, n, groups, partial);                                    \
+        break;                                                                                  \
+    case kGlmFp16x2:                                                                            \
+        gb_mfma_kernel<DD, 2, CBV, _Float16, f16x8><<<blocks, 256, 0, s>>>(                     \
+            x, xoff, un, nunits, P, ys16, n, groups, partial);                                  \
+        break;                                                                                  \
+    default:                                                                                    \
+        gb_mfma_kernel<DD, 1, CBV, __bf16, bf16x8><<<blocks, 256, 0, s>>>(                      \
+            x, xoff, un, nunits, P, ys, n, groups, partial);                                    \
+        break;                                                                                  \
+    }
+    if (mfma && CB == 8) {
+        BH_GB_MFMA(32, 8)
+        return hipGetLastError();
+    }
 #define BH_GB_CASES(DD)                                                                         \
     switch (precision) {                                                                        \
     case kGlmBf16x3:                                                                            \
-        gb_mfma_kernel<DD, 3, __bf16, bf16x8><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P,    \
-                                                                     ys, n, groups, partial);   \
-        break;                                                                                  \
     case kGlmFp16x2:                                                                            \
-        gb_mfma_kernel<DD, 2, _Float16, f16x8><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P,   \
-                                                                      ys16, n, groups,          \
-                                                                      partial);                 \
-        break;                                                                                  \
     case kGlmBf16:                                                                              \
-        gb_mfma_kernel<DD, 1, __bf16, bf16x8><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P,    \
-                                                                     ys, n, groups, partial);   \
+        BH_GB_MFMA(DD, 4)                                                                       \
         break;                                                                                  \
     case kGlmFp32:                                                                              \
         gb_fp32_kernel<DD><<<blocks, 256, 0, s>>>(x, xoff, un, nunits, P, ysc, n, groups,       \
@@ -752,6 +775,7 @@ hipError_t launch_gb_contract(int precision, int D, hipStream_t s, const void* X
         return hipErrorInvalidValue;
     }
 #undef BH_GB_CASES
+#undef BH_GB_MFMA
     return hipGetLastError();
 }
 

@@ -8,7 +8,8 @@
 
 namespace bh {
 
-// rows of one unit of the batched contraction (a multiple of 16)
+// rows of one unit of the batched contraction (a multiple of 16) until the
+// batch's tile shapes are set or autotuned
 constexpr unsigned kGbUnitRows = 2048;
 
 // f16 = 0: the fp32 path's decoding; 1: the fp16 matrix-core path's.
@@ -24,9 +25,11 @@ hipError_t launch_gb_propose(hipStream_t s, unsigned seed0, int half, unsigned o
                              unsigned P, unsigned K, float* Y, float* z, float* u2, float* ysc,
                              void* ysplit, int nsplit, int f16);
 // precision: kGlmFp64 / kGlmFp32 / kGlmBf16 / kGlmBf16x3 / kGlmFp16x2; D: 32
-// or 64.
+// or 64; CB: 16-point column blocks per wave of the matrix-core kernels (4,
+// or 8 for D = 32; n a multiple of 64 CB; the fp32 and fp64 kernels take
+// CB = 4 and n a multiple of 256).
 // Ypts: the unscaled points (fp64 kernel), ysc / ysplit the scaled ones.
-hipError_t launch_gb_contract(int precision, int D, hipStream_t s, const void* X8,
+hipError_t launch_gb_contract(int precision, int D, int CB, hipStream_t s, const void* X8,
                               const unsigned long long* xoff, const void* units, unsigned nunits,
                               unsigned P, const float* scales, const float* Ypts,
                               const float* ysc, const void* ysplit, unsigned n, double* partial);

@@ -35,11 +35,11 @@ __device__ __forceinline__ float softplus2_rest4(float u0, float u1, float u2, f
     return __builtin_fmaf(0.5f, a, __builtin_amdgcn_logf(p01 * p23));
 }
 
-template <int D, int NS, class V>
-__device__ __forceinline__ void glm_mfma_block(const V (&a)[D / 32], const V (&bfrag)[4][D / 32][NS],
-                                               f32x4 (&c)[4]) {
+template <int D, int NS, int CB, class V>
+__device__ __forceinline__ void glm_mfma_block(const V (&a)[D / 32], const V (&bfrag)[CB][D / 32][NS],
+                                               f32x4 (&c)[CB]) {
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb) {
+    for (int wb = 0; wb < CB; ++wb) {
         c[wb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = NS - 1; s >= 0; --s)
@@ -49,18 +49,20 @@ __device__ __forceinline__ void glm_mfma_block(const V (&a)[D / 32], const V (&b
     }
 }
 
-__device__ __forceinline__ void glm_epilogue(const f32x4 (&c)[4], double (&acc)[4]) {
+template <int CB>
+__device__ __forceinline__ void glm_epilogue(const f32x4 (&c)[CB], double (&acc)[CB]) {
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb)
+    for (int wb = 0; wb < CB; ++wb)
         acc[wb] += (double)softplus2_rest4(c[wb][0], c[wb][1], c[wb][2], c[wb][3]);
 }
 
 // rows at or past r1 contribute nothing: u = +inf gives the factor 1 + 0,
 // and its |u|/2 is taken out by zeroing it in the sum
-__device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double (&acc)[4],
+template <int CB>
+__device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[CB], double (&acc)[CB],
                                                     unsigned orow, unsigned r1) {
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb) {
+    for (int wb = 0; wb < CB; ++wb) {
         float p = 1.0f, a = 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -82,14 +84,15 @@ __device__ __forceinline__ void glm_epilogue_masked(const f32x4 (&c)[4], double 
 // that starts the run (H = 0) or continues it (H = 2) (exponentials, product
 // and |u| sum carried in st), seven for the block that ends it (H = 1: the
 // same, then the logarithm and the double accumulation).
+template <int CB>
 struct GlmEpiState {
-    float p[4];  // product of the factors so far, per column block
-    float a[4];  // sum of the |u| so far
+    float p[CB];  // product of the factors so far, per column block
+    float a[CB];  // sum of the |u| so far
 };
 
-template <int H>
+template <int H, class St>
 __device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& c, float (&e)[4],
-                                                   GlmEpiState& st, double& acc) {
+                                                   St& st, double& acc) {
     switch (ph) {
     case 0:
         e[0] = __builtin_amdgcn_exp2f(-__builtin_fabsf(c[0]));
@@ -128,28 +131,28 @@ __device__ __forceinline__ void glm_epilogue_piece(int ph, int wb, const f32x4& 
 
 // The products of one 16-row block into cn (zeroed by the first product of
 // each chain) with part H of the run epilogue of the previous block's c
-// spread between them: products of the four column blocks alternate, so
+// spread between them: products of the CB column blocks alternate, so
 // consecutive products are independent, and scheduling barriers keep each
 // epilogue piece between two products, where it runs on the VALU while the
 // matrix core works.  The per-column-block product order is that of
 // glm_mfma_block (split terms smallest first, chunks in order).
-template <int D, int NS, int H, class V>
+template <int D, int NS, int H, int CB, class V>
 __device__ __forceinline__ void glm_mfma_overlap(const V (&a)[D / 32],
-                                                 const V (&bfrag)[4][D / 32][NS],
-                                                 f32x4 (&cn)[4], const f32x4 (&c)[4],
-                                                 double (&acc)[4], GlmEpiState& st) {
+                                                 const V (&bfrag)[CB][D / 32][NS],
+                                                 f32x4 (&cn)[CB], const f32x4 (&c)[CB],
+                                                 double (&acc)[CB], GlmEpiState<CB>& st) {
     constexpr int KC = D / 32;
-    constexpr int NM = 4 * KC * NS;
+    constexpr int NM = CB * KC * NS;
     constexpr int PW = H == 1 ? 7 : 5;  // pieces per column block
-    constexpr int NP = 4 * PW;
-    float e[4][4];
+    constexpr int NP = CB * PW;
+    float e[CB][4];
     int m = 0;
 #pragma unroll
     for (int s = NS - 1; s >= 0; --s)
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
-            for (int wb = 0; wb < 4; ++wb) {
+            for (int wb = 0; wb < CB; ++wb) {
                 const f32x4 acc0 = (s == NS - 1 && kc == 0) ? f32x4{0.0f, 0.0f, 0.0f, 0.0f}
                                                             : cn[wb];
                 cn[wb] = mfma_16x16x32(a[kc], bfrag[wb][kc][s], acc0);
@@ -164,29 +167,30 @@ __device__ __forceinline__ void glm_mfma_overlap(const V (&a)[D / 32],
 }
 
 // Half 1 of the pair epilogue without products (the last pair of a range).
-__device__ __forceinline__ void glm_epilogue_second(const f32x4 (&c)[4], double (&acc)[4],
-                                                    GlmEpiState& st) {
-    float e[4][4];
+template <int CB>
+__device__ __forceinline__ void glm_epilogue_second(const f32x4 (&c)[CB], double (&acc)[CB],
+                                                    GlmEpiState<CB>& st) {
+    float e[CB][4];
 #pragma unroll
-    for (int wb = 0; wb < 4; ++wb)
+    for (int wb = 0; wb < CB; ++wb)
 #pragma unroll
         for (int ph = 0; ph < 7; ++ph) glm_epilogue_piece<1>(ph, wb, c[wb], e[wb], st, acc[wb]);
 }
 
-// The row blocks [r0, r1) of one range for the 64 points whose B fragments
-// the wave holds: full 16-row blocks through the software pipeline, then one
+// The row blocks [r0, r1) of one range for the 16 CB points whose B
+// fragments the wave holds: full 16-row blocks through the software pipeline, then one
 // masked block.  `ld` loads the A fragments of the block starting at a row
 // (ld.full(rb, a): all 16 rows valid; ld.masked(rb, r1, a): rows >= r1 read
 // as zeros; Loader::vec is the operand type).  acc[wb] accumulates this
-// lane's four rows of column block wb.
-template <int D, int NS, class Loader>
+// lane's four rows of column block wb.  CB = 4 (64 points per wave) or 8.
+template <int D, int NS, int CB, class Loader>
 __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
-                                              const typename Loader::vec (&bfrag)[4][D / 32][NS],
+                                              const typename Loader::vec (&bfrag)[CB][D / 32][NS],
                                               unsigned r0,
-                                              unsigned r1, unsigned lg, double (&acc)[4]) {
+                                              unsigned r1, unsigned lg, double (&acc)[CB]) {
     constexpr int KC = D / 32;
     const unsigned rfull = r0 + (r1 - r0) / 16 * 16;
-    f32x4 c0[4], c1[4];
+    f32x4 c0[CB], c1[CB];
     if (rfull > r0) {
         // software pipeline over 16-row blocks, the accumulator sets c0/c1
         // alternating without copies: the products of block i+1 run with the
@@ -200,30 +204,30 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
         typename Loader::vec a0[KC], a1[KC];
         ld.full(r0, a0);
         ld.full(clamp(r0 + 16), a1);
-        glm_mfma_block<D, NS>(a0, bfrag, c0);
-        GlmEpiState st;
+        glm_mfma_block<D, NS, CB>(a0, bfrag, c0);
+        GlmEpiState<CB> st;
         unsigned rb = r0 + 16;
         // runs of four blocks rb - 16 .. rb + 32 (products of rb .. rb + 48)
         for (; rb + 48 < rfull; rb += 64) {
             ld.full(clamp(rb + 16), a0);
-            glm_mfma_overlap<D, NS, 0>(a1, bfrag, c1, c0, acc, st);
+            glm_mfma_overlap<D, NS, 0, CB>(a1, bfrag, c1, c0, acc, st);
             ld.full(clamp(rb + 32), a1);
-            glm_mfma_overlap<D, NS, 2>(a0, bfrag, c0, c1, acc, st);
+            glm_mfma_overlap<D, NS, 2, CB>(a0, bfrag, c0, c1, acc, st);
             ld.full(clamp(rb + 48), a0);
-            glm_mfma_overlap<D, NS, 2>(a1, bfrag, c1, c0, acc, st);
+            glm_mfma_overlap<D, NS, 2, CB>(a1, bfrag, c1, c0, acc, st);
             ld.full(clamp(rb + 64), a1);
-            glm_mfma_overlap<D, NS, 1>(a0, bfrag, c0, c1, acc, st);
+            glm_mfma_overlap<D, NS, 1, CB>(a0, bfrag, c0, c1, acc, st);
         }
         // pairs of blocks (rb - 16, rb): the first's epilogue runs with the
         // products of rb, the second's with those of rb + 16
         for (; rb + 16 < rfull; rb += 32) {
             ld.full(clamp(rb + 16), a0);
-            glm_mfma_overlap<D, NS, 0>(a1, bfrag, c1, c0, acc, st);
+            glm_mfma_overlap<D, NS, 0, CB>(a1, bfrag, c1, c0, acc, st);
             ld.full(clamp(rb + 32), a1);
-            glm_mfma_overlap<D, NS, 1>(a0, bfrag, c0, c1, acc, st);
+            glm_mfma_overlap<D, NS, 1, CB>(a0, bfrag, c0, c1, acc, st);
         }
         if (rb < rfull) {
-            glm_mfma_overlap<D, NS, 0>(a1, bfrag, c1, c0, acc, st);
+            glm_mfma_overlap<D, NS, 0, CB>(a1, bfrag, c1, c0, acc, st);
             glm_epilogue_second(c1, acc, st);
         } else {
             glm_epilogue(c0, acc);
@@ -232,8 +236,8 @@ __device__ __forceinline__ void glm_mfma_rows(const Loader& ld,
     if (rfull < r1) {
         typename Loader::vec a[KC];
         ld.masked(rfull, r1, a);
-        f32x4 c[4];
-        glm_mfma_block<D, NS>(a, bfrag, c);
+        f32x4 c[CB];
+        glm_mfma_block<D, NS, CB>(a, bfrag, c);
         glm_epilogue_masked(c, acc, rfull + lg * 4, r1);
     }
 }

@@ -534,6 +534,25 @@ class HipGlmBatch(_Handle):
         call("bh_glm_batch_set_precision", self._h, int(precision))
         return self
 
+    def set_tiles(self, row_stride: int, unit_rows: int, col_blocks: int):
+        """Tile shape of the contraction for the problems of one row stride
+        (32 or 64): rows per unit and 16-point column blocks per wave."""
+        call("bh_glm_batch_set_tiles", self._h, int(row_stride), int(unit_rows),
+             int(col_blocks))
+        return self
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times every candidate tile shape per row stride on the current
+        positions and keeps the fastest; returns the choices and their
+        milliseconds per contraction launch."""
+        ur = np.zeros(2, np.uint32)
+        cb = np.zeros(2, np.uint32)
+        ms = np.zeros(2, np.float32)
+        call("bh_glm_batch_autotune", self._h, int(reps), ur.ctypes.data, cb.ctypes.data,
+             ms.ctypes.data)
+        return {str(s): {"unit_rows": int(ur[i]), "col_blocks": int(cb[i]),
+                         "ms": float(ms[i])} for i, s in enumerate((32, 64))}
+
     def init_position(self, seed: int, limits):
         """limits: (problems, 64, 2) or (problems, d, 2) with d <= 64."""
         lim = np.zeros((self.problems, 64, 2), np.float32)

@@ -71,6 +71,7 @@ METRIC = ("ESS/sec & walker-steps/sec, MFMA vs plain FP32 at posterior-mean err 
 A = 2.0
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK = 2500.0  # TFLOP/s, dense
+MFMA_FP8_PEAK = 5000.0  # TFLOP/s, dense
 
 # correlated Gaussian workload
 DIM = 4
@@ -784,9 +785,12 @@ def c5_cpu_baseline(m, lim, budget_s=12.0):
                       f"({t:.1f} s), scaled by the batch's mean N_p d_p"}
 
 
-def c5_run(fac, m, lim, prec, burn, run, series=None):
+def c5_run(fac, m, lim, prec, burn, run, series=None, tune=False):
     bt = fac.glm_batch(m, C5_W).set_precision(prec)
     bt.init_position(9, lim)
+    # the contraction's tile shapes, chosen by the batch's autotuner on these
+    # positions (setup, before the timed region)
+    bt.tuned = bt.autotune(3) if tune else None
     t0 = time.perf_counter()
     bt.burn_in(100, burn, A)
     burn_ms = bt.last_gpu_ms()
@@ -811,7 +815,8 @@ def main_c5(args):
     bt.release()
     torch.cuda.synchronize()
     print(f"bench: {mname} batched run", file=sys.stderr, flush=True)
-    bt, res, wall, burn_ms, run_ms = c5_run(fac, m, lim, mprec, burn, run, series)
+    bt, res, wall, burn_ms, run_ms = c5_run(fac, m, lim, mprec, burn, run, series, tune=True)
+    tuned = bt.tuned
     # after the timed region: ESS per problem from the engine's acor of its
     # recorded means; the precision bias of the matrix-core chain's
     # stationary distribution, estimated per problem by reweighting its final
@@ -884,8 +889,10 @@ def main_c5(args):
                                f"{m.problems} problems x 1024 points", "bound": "mfma",
                      "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_BF16_PEAK, 4), "traffic": None,
+                     "frac_of_fp8_peak": round(achieved / MFMA_FP8_PEAK, 4),
                      "avg_half_step_ms": round(half_ms, 4),
-                     "algorithmic_flops_per_half_step": flops},
+                     "algorithmic_flops_per_half_step": flops,
+                     "tiles_autotuned": tuned},
         "setup_s": round(t_setup, 1),
     }
     if not args.no_cpu_baseline:

@@ -373,6 +373,26 @@ int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t
                         const float* y, float prior_sd, uint32_t walkers, bh_glm_batch** out);
 int bh_glm_batch_destroy(bh_glm_batch* b);
 int bh_glm_batch_set_precision(bh_glm_batch* b, int precision);
+/* Tile shape of the matrix-core contraction for the problems of one row
+ * stride (32: d_p <= 32; 64: d_p > 32), no reference counterpart: the rows
+ * per unit (a multiple of 16 in [256, 65536]; each unit is one work-group's
+ * range of a problem's rows) and the 16-point column blocks per wave (4, or
+ * 8 for stride 32; a work-group takes 64 x col_blocks points; needs
+ * walkers / 2 a multiple of 64 x col_blocks).  The stored log posteriors are
+ * recomputed over the new units; contraction values are the same for every
+ * shape, their sums over units differ only in rounding order. */
+int bh_glm_batch_set_tiles(bh_glm_batch* b, uint32_t row_stride, uint32_t unit_rows,
+                           uint32_t col_blocks);
+/* The batch's own tile autotuner (no reference counterpart): for each row
+ * stride, times `reps` launches of the half-step contraction (walkers / 2
+ * points per problem, the batch's precision, its current positions) for
+ * every candidate shape (unit rows 1024 .. 16384, column blocks 4 and, for
+ * stride 32 at matrix-core precisions, 8) with HIP events, and keeps the
+ * fastest.  unit_rows, col_blocks and best_ms (index 0: stride 32, 1: stride
+ * 64; best_ms 0 for a stride with no problems) may be NULL.  Rebuilds the
+ * units and recomputes the stored log posteriors. */
+int bh_glm_batch_autotune(bh_glm_batch* b, uint32_t reps, uint32_t* unit_rows,
+                          uint32_t* col_blocks, float* best_ms);
 /* limits: problems x 64 x 2 (lo, hi per coordinate; rows past d_p unused);
  * problem p's walkers from init_walkers' stream with seed + p
  * (amd-gcn-mcmc-stretch.cl:138-159), then their log posteriors. */

@@ -145,3 +145,9 @@ def test_acor_batch_rejects_null():
     lib = _lib.load()
     dims = (ctypes.c_uint32 * 1)(4)
     assert lib.bh_acor_batch(None, None, 1, dims, 100, None, None, None, None) == _lib.BH_ERR_ARG
+
+
+def test_glm_batch_tile_calls_reject_null():
+    lib = _lib.load()
+    assert lib.bh_glm_batch_set_tiles(None, 32, 2048, 4) == _lib.BH_ERR_ARG
+    assert lib.bh_glm_batch_autotune(None, 3, None, None, None) == _lib.BH_ERR_ARG

@@ -154,3 +154,57 @@ def test_batch_matrix_core_precisions_agree_with_fp64_on_moves(fac):
         for p in range(m.problems):
             same = np.all(out[prec][p] == out[_lib.BH_GLM_FP64][p], axis=1).mean()
             assert same > 0.99, (prec, p, same)
+
+
+TILE_SHAPES = [(1000, 8), (2049, 16), (4100, 32), (20011, 64), (300, 3), (7000, 40), (9000, 24)]
+
+
+def test_batch_tile_shapes_give_the_same_log_posteriors(fac):
+    """The contraction's values do not depend on the tile shape: 8 column
+    blocks per wave give the 4-block kernel's log posteriors bit for bit at
+    the same unit rows.  Other unit rows move the boundaries of the
+    epilogue's runs (up to four 16-row blocks share one fp32 logarithm of
+    their factors' product) and the order of the units' double sums: the
+    log posteriors agree to fp32 rounding of the run sums (~1e-9 relative
+    at these sizes)."""
+    m = _problems(5, TILE_SHAPES)
+    bt = fac.glm_batch(m, 2048).set_precision(_lib.BH_GLM_FP16X2)
+    lim = np.tile(np.array([-0.8, 0.8], np.float32), (m.problems, 64, 1))
+    bt.init_position(12, lim)
+    out = {}
+    for ur, cb in ((2048, 4), (2048, 8), (1024, 4), (4096, 8)):
+        print(f"tiles: stride-32 unit rows {ur}, column blocks {cb}", flush=True)
+        bt.set_tiles(32, ur, cb).set_tiles(64, ur, 4)
+        out[(ur, cb)] = bt.log_density(_lib.BH_GLM_FP16X2)
+    assert np.array_equal(out[(2048, 4)], out[(2048, 8)])
+    for k in ((1024, 4), (4096, 8)):
+        assert np.allclose(out[k], out[(2048, 4)], rtol=1e-8, atol=0)
+    with pytest.raises(_lib.ArgumentError):
+        bt.set_tiles(64, 2048, 8)
+    with pytest.raises(_lib.ArgumentError):
+        bt.set_tiles(32, 100, 4)
+    bt.release()
+
+
+def test_batch_autotune_then_log_posteriors_vs_oracle(fac):
+    """The autotuner keeps a valid shape per row stride, and the log
+    posteriors it recomputes over the chosen units match the oracle's."""
+    m = _problems(6, TILE_SHAPES)
+    bt = fac.glm_batch(m, 2048).set_precision(_lib.BH_GLM_FP16X2)
+    lim = np.tile(np.array([-0.8, 0.8], np.float32), (m.problems, 64, 1))
+    bt.init_position(13, lim)
+    tuned = bt.autotune(2)
+    print("autotune", tuned, flush=True)
+    for s in ("32", "64"):
+        assert tuned[s]["unit_rows"] in (1024, 2048, 4096, 8192, 16384)
+        assert tuned[s]["col_blocks"] in ((4, 8) if s == "32" else (4,))
+        assert tuned[s]["ms"] > 0.0
+    xs, lp = bt.state()
+    for p in range(m.problems):
+        ref = O.glm_logpost(m.dequantized(p), m.y[p], 5.0, xs[p][:64])
+        err = np.abs(lp[p, :64] - ref) / (np.abs(ref) + 1.0)
+        assert err.max() < 2e-6, (p, err.max())
+    # and it keeps sampling: a few accounted steps after the rebuild
+    res = bt.run(31, 4, 2.0)
+    assert np.all(res["acceptance-rate"] > 0.0)
+    bt.release()
