@@ -124,6 +124,8 @@ SIGNATURES = {
     "bh_glm_batch_log_density": [_P, _I, _P],
     "bh_fp8_decode": [_P, _P, _P, _U64],
     "bh_device_rates": [_P, _P, _P, _P, _P],
+    "bh_device_copy_rate": [_P, _U64, _P, _P],
+    "bh_device_gather_rate": [_P, _U64, _I, _P],
     "bh_fp8_decode_f16": [_P, _P, _P, _U64],
     "bh_batch_state_get": [_P, _P, _P],
     "bh_batch_last_gpu_ms": [_P, _P],

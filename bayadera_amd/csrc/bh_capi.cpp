@@ -918,6 +918,72 @@ int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs) {
     return BH_OK;
 }
 
+int bh_device_copy_rate(bh_ctx* ctx, uint64_t bytes, double* gbps, double* gbps_nt) {
+    BH_ARG(ctx, "ctx is NULL");
+    BH_ARG(bytes >= (1u << 20) && bytes % 16 == 0 && bytes <= (16ull << 30),
+           "copy size %llu: a multiple of 16 bytes in [1 MiB, 16 GiB]",
+           (unsigned long long)bytes);
+    BH_HIP(hipSetDevice(ctx->device));
+    DevBuf src, dst;
+    BH_TRY(src.alloc(bytes));
+    BH_TRY(dst.alloc(bytes));
+    BH_HIP(hipMemsetAsync(src.p, 0, bytes, ctx->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BH_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    double rate[2] = {0.0, 0.0};
+    const int reps = 10;
+    for (int nt = 0; nt < 2 && err == hipSuccess; ++nt) {
+        err = bh::launch_copy16(ctx->stream, src.p, dst.p, bytes / 16, nt != 0);  // warm-up
+        if (err == hipSuccess) err = hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && err == hipSuccess; ++r)
+            err = bh::launch_copy16(ctx->stream, src.p, dst.p, bytes / 16, nt != 0);
+        if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        // bytes read plus bytes written, per second
+        if (err == hipSuccess && ms > 0.0f) rate[nt] = 2.0 * reps * (double)bytes / (ms * 1e-3) / 1e9;
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    BH_HIP(err);
+    if (gbps) *gbps = rate[0];
+    if (gbps_nt) *gbps_nt = rate[1];
+    return BH_OK;
+}
+
+int bh_device_gather_rate(bh_ctx* ctx, uint64_t bytes, int nontemporal, double* gathers_per_s) {
+    BH_ARG(ctx && gathers_per_s, "NULL argument");
+    const uint64_t n16 = bytes / 16;
+    BH_ARG(n16 >= 256 && (n16 & (n16 - 1)) == 0 && n16 <= (1ull << 28) && bytes % 16 == 0,
+           "gather buffer %llu bytes: 16 x a power of two in [4 KiB, 4 GiB]",
+           (unsigned long long)bytes);
+    BH_HIP(hipSetDevice(ctx->device));
+    DevBuf src, out;
+    BH_TRY(src.alloc(bytes));
+    BH_TRY(out.alloc(n16 * sizeof(float)));
+    BH_HIP(hipMemsetAsync(src.p, 0, bytes, ctx->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BH_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    const bool nt = nontemporal != 0;
+    for (int rep = 0; rep < 2 && err == hipSuccess; ++rep)
+        err = bh::launch_glm_gather(ctx->stream, src.p, out.as<float>(), (unsigned)n16, nt);
+    if (err == hipSuccess) err = hipEventRecord(e0, ctx->stream);
+    for (int rep = 0; rep < 5 && err == hipSuccess; ++rep)
+        err = bh::launch_glm_gather(ctx->stream, src.p, out.as<float>(), (unsigned)n16, nt);
+    if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    BH_HIP(err);
+    *gathers_per_s = ms > 0.0f ? 5.0 * (double)n16 / ((double)ms * 1e-3) : 0.0;
+    return BH_OK;
+}
+
 int bh_device_rates(bh_ctx* ctx, double* exp2_per_s, double* fma_per_s,
                     double* mfma_f16_flops_per_s, double* gathers_per_s) {
     BH_ARG(ctx, "ctx is NULL");

@@ -672,18 +672,25 @@ __global__ void __launch_bounds__(256) glm_rate_mfma_kernel(float* __restrict__ 
 }
 
 // Random 16-byte gathers: thread k reads record (k * 2654435761) mod n16
-// of src (n16 a power of two; 16-byte records) and writes their sum.
-__global__ void __launch_bounds__(256) glm_rate_gather_kernel(const float4* __restrict__ src,
+// of src (n16 a power of two; 16-byte records) and writes their sum; NT:
+// the loads are non-temporal.
+template <bool NT>
+__global__ void __launch_bounds__(256) glm_rate_gather_kernel(const f32x4* __restrict__ src,
                                                               float* __restrict__ out,
                                                               unsigned n16) {
     const unsigned k = blockIdx.x * 256 + threadIdx.x;
-    const float4 v = src[(k * 2654435761u) & (n16 - 1u)];
+    const f32x4* p = src + ((k * 2654435761u) & (n16 - 1u));
+    const f32x4 v = NT ? __builtin_nontemporal_load(p) : *p;
     out[k] = (v.x + v.y) + (v.z + v.w);
 }
 
-hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16) {
+hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16, bool nt) {
     if (n16 == 0 || (n16 & (n16 - 1)) || n16 % 256) return hipErrorInvalidValue;
-    glm_rate_gather_kernel<<<n16 / 256, 256, 0, s>>>(static_cast<const float4*>(src), out, n16);
+    const f32x4* q = static_cast<const f32x4*>(src);
+    if (nt)
+        glm_rate_gather_kernel<true><<<n16 / 256, 256, 0, s>>>(q, out, n16);
+    else
+        glm_rate_gather_kernel<false><<<n16 / 256, 256, 0, s>>>(q, out, n16);
     return hipGetLastError();
 }
 

@@ -29,8 +29,10 @@ hipError_t launch_glm_logistic(int precision, hipStream_t s, const void* X, unsi
 // out: blocks x 256 floats.
 hipError_t launch_glm_rate(hipStream_t s, int which, float* out, unsigned blocks, int iters);
 // n16 random 16-byte gathers from src (n16 x 16 bytes; n16 a power of two and
-// a multiple of 256), one per thread, sums to out (n16 floats).
-hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16);
+// a multiple of 256), one per thread, sums to out (n16 floats); nt:
+// non-temporal loads.
+hipError_t launch_glm_gather(hipStream_t s, const void* src, float* out, unsigned n16,
+                             bool nt = false);
 // fp16 copy of the N x d bf16 X (d = 32 or 64) in the fragment order the fp16
 // matrix-core kernels read (glm_x_f16_bytes bytes: whole 16-row blocks, the
 // padding rows zero); *overflow is incremented by the number of values

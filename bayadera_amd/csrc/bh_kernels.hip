@@ -585,6 +585,34 @@ acor_batch_kernel(float* __restrict__ series, const unsigned long long* __restri
 }
 
 // --------------------------------------------------------------------------
+// Streaming copy (the measured HBM denominator): each thread copies four
+// 16-byte pieces, 4 KB apart within its block's 16 KB chunk, the four loads
+// in flight before the stores; NT: non-temporal loads and stores.
+// --------------------------------------------------------------------------
+typedef float bh_copy_f4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__global__ void __launch_bounds__(256)
+copy16_kernel(const bh_copy_f4* __restrict__ src, bh_copy_f4* __restrict__ dst,
+              unsigned long long n16) {
+    const unsigned long long base = (unsigned long long)blockIdx.x * 1024u + threadIdx.x;
+    bh_copy_f4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const unsigned long long i = base + 256u * u;
+        if (i < n16) v[u] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const unsigned long long i = base + 256u * u;
+        if (i < n16) {
+            if (NT) __builtin_nontemporal_store(v[u], dst + i);
+            else dst[i] = v[u];
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
 // Host-side launchers (all on the caller's stream)
 // --------------------------------------------------------------------------
 static unsigned tile_rows(unsigned m) {
@@ -597,6 +625,19 @@ static unsigned blocks_for(unsigned long long n, unsigned tn, unsigned cap) {
     unsigned long long b = (n + tn - 1) / tn;
     if (b > cap) b = cap;
     return b ? (unsigned)b : 1u;
+}
+
+hipError_t launch_copy16(hipStream_t s, const void* src, void* dst, unsigned long long n16,
+                         bool nt) {
+    const unsigned long long blocks = (n16 + 1023) / 1024;
+    if (blocks == 0) return hipSuccess;
+    if (nt)
+        copy16_kernel<true><<<(unsigned)blocks, 256, 0, s>>>(
+            static_cast<const bh_copy_f4*>(src), static_cast<bh_copy_f4*>(dst), n16);
+    else
+        copy16_kernel<false><<<(unsigned)blocks, 256, 0, s>>>(
+            static_cast<const bh_copy_f4*>(src), static_cast<bh_copy_f4*>(dst), n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits, unsigned dim,

@@ -22,6 +22,9 @@ hipError_t noalloc_scratch_init(int device);
 // The address noalloc_scratch_init found for `device`.
 hipError_t noalloc_scratch(int device, void** ptr);
 
+// dst[i] = src[i] for n16 16-byte pieces (nt: non-temporal loads and stores)
+hipError_t launch_copy16(hipStream_t s, const void* src, void* dst, unsigned long long n16,
+                         bool nt);
 hipError_t launch_init_walkers(hipStream_t s, unsigned seed, const float* limits, unsigned dim,
                                unsigned walkers, float* xs);
 hipError_t launch_uniform_sample(hipStream_t s, unsigned seed, float lo, float hi, unsigned quads,

@@ -193,6 +193,20 @@ class HipBayaderaFactory(_Handle):
         return {"exp2_per_s": e.value, "fma_per_s": f.value, "mfma_f16_flops_per_s": m.value,
                 "gathers_per_s": g.value}
 
+    def device_copy_rate(self, nbytes: int = 1 << 30) -> dict:
+        """Read + written GB/s of the engine's own streaming copy kernel of
+        nbytes each way, with plain and with non-temporal accesses."""
+        g, gn = ctypes.c_double(), ctypes.c_double()
+        call("bh_device_copy_rate", self.handle, int(nbytes), ctypes.byref(g), ctypes.byref(gn))
+        return {"copy_gbps": g.value, "copy_nt_gbps": gn.value}
+
+    def device_gather_rate(self, nbytes: int = 1 << 30, nontemporal: bool = False) -> float:
+        """Random 16-byte gathers per second from a buffer of nbytes."""
+        r = ctypes.c_double()
+        call("bh_device_gather_rate", self.handle, int(nbytes), int(bool(nontemporal)),
+             ctypes.byref(r))
+        return r.value
+
 
 class HipDirectSamplerEngine:
     """GCNDirectSamplerEngine (amd_gcn.clj:49-62) for the library models with

@@ -216,8 +216,15 @@ def hbm_copy_gbps(mib=1024, reps=10):
 
 
 def device_rates(fac):
+    """The device's measured rates; the HBM denominator is the engine's own
+    streaming copy of 1 GiB each way (the better of plain and non-temporal
+    accesses), torch's copy kernel beside it."""
     rates = fac.device_rates()
-    rates["hbm_copy_gbps"] = hbm_copy_gbps()
+    cr = fac.device_copy_rate(1 << 30)
+    rates["hbm_copy_gbps"] = max(cr["copy_gbps"], cr["copy_nt_gbps"])
+    rates["hbm_copy_plain_gbps"] = cr["copy_gbps"]
+    rates["hbm_copy_nt_gbps"] = cr["copy_nt_gbps"]
+    rates["torch_copy_gbps"] = hbm_copy_gbps()
     return rates
 
 
@@ -279,6 +286,7 @@ def move_roofline(fac, dim, rates=None, model="mvn"):
             "alpha": round(alpha, 4),
             "algorithmic_bytes_per_launch": int(walkers_per_launch * bytes_per_walker),
             "hbm_copy_measured_gbps": round(rates["hbm_copy_gbps"], 1) if rates else None,
+            "torch_copy_gbps": round(rates["torch_copy_gbps"], 1) if rates else None,
             "frac_of_measured_copy": (round(achieved / rates["hbm_copy_gbps"], 4)
                                       if rates else None),
             "gather_bound": gather}
@@ -1412,11 +1420,17 @@ def main():
         fac = engine.HipBayaderaFactory(0, 256)
         rates = device_rates(fac)
         r = [move_roofline(fac, d, rates, m) for m, d in ROOF_DIMS]
+        # random 16-byte gathers from 1 GiB, plain and non-temporal: under
+        # rocprofv3 --pmc TCC_EA0_RDREQ_*B_sum their request sizes show
+        # whether a load policy fetches less than a line
+        gathers = {"plain": fac.device_gather_rate(1 << 30, False),
+                   "nontemporal": fac.device_gather_rate(1 << 30, True)}
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"roofline": r, "calibration_copy_bytes": src.numel() * 4}), flush=True)
+        print(json.dumps({"roofline": r, "gathers_per_s_1GiB": gathers,
+                          "calibration_copy_bytes": src.numel() * 4}), flush=True)
         return
     if args.c3_ess_calibration:
         return c3_calibration()

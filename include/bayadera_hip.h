@@ -88,6 +88,18 @@ int bh_ctx_create(int device, uint32_t wgs, bh_ctx** out);
 int bh_ctx_destroy(bh_ctx* ctx);
 /* compute units and WGS; processing-elements = cu * wgs (amd_gcn.clj:663-664) */
 int bh_ctx_info(const bh_ctx* ctx, int* compute_units, uint32_t* wgs);
+/* The device's streaming copy rate (no reference counterpart): read plus
+ * written bytes per second (GB/s) of a plain copy kernel of `bytes` each way
+ * (four 16-byte pieces per thread in flight; 10 launches after a warm-up),
+ * and the same with non-temporal loads and stores; either output may be
+ * NULL.  The measured HBM denominator beside the published 8 TB/s. */
+int bh_device_copy_rate(bh_ctx* ctx, uint64_t bytes, double* gbps, double* gbps_nt);
+/* Random 16-byte gathers per second (no reference counterpart) from a
+ * buffer of `bytes` (16 x a power of two, at most 4 GiB), one per thread, 5
+ * launches after a warm-up; nontemporal != 0 issues them as non-temporal
+ * loads.  The stretch move's partner reads at large ensembles are such
+ * gathers. */
+int bh_device_gather_rate(bh_ctx* ctx, uint64_t bytes, int nontemporal, double* gathers_per_s);
 /* Measured rates of the device (no reference counterpart): lane operations
  * per second of v_exp_f32 (the transcendental of the GLM likelihood's
  * softplus epilogue) and of v_fma_f32, flops per second of

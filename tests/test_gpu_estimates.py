@@ -140,3 +140,13 @@ def test_rtc_error_surfaces(fac):
     bad = models.distribution_model("inline REAL nope(const uint a) { return q; }", name="nope")
     with pytest.raises(_lib.BayaderaError, match="failed to compile"):
         fac.distribution_engine(bad)
+
+
+def test_device_copy_rate(fac):
+    """The engine's streaming copy measures a plausible HBM rate both ways
+    (read + written bytes per second of a 256 MiB copy)."""
+    r = fac.device_copy_rate(256 << 20)
+    assert 500.0 < r["copy_gbps"] < 20000.0
+    assert 500.0 < r["copy_nt_gbps"] < 20000.0
+    with pytest.raises(_lib.ArgumentError):
+        fac.device_copy_rate(1000)
