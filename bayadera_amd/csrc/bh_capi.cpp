@@ -373,16 +373,19 @@ constexpr unsigned kGlmRounds = 4;
 
 // Guarded accept test: the list holds every walker of a half (K), so no
 // walker inside the guard is ever decided on reduced-precision values; the
-// fp64 pass over the listed points uses up to kGuardRangesMax row ranges,
-// fewer for very large halves so that its partial sums (ranges x 2K doubles)
-// stay within kGuardPartialMax doubles.
-constexpr unsigned kGuardRangesMax = 1024;
+// fp64 pass over the listed points uses up to kGuardRangesMax row ranges (a
+// multiple of 8; with its kGuardSlots work-groups per range, of which ~5
+// have points at the usual ~18 listed walkers, the busy ones fit the device
+// at once, five per CU, with ~15 rows per thread at 1M rows), fewer for
+// very large halves so that its partial sums (ranges x 2K doubles) stay
+// within kGuardPartialMax doubles.
+constexpr unsigned kGuardRangesMax = 256;
 constexpr size_t kGuardPartialMax = (size_t)1 << 25;  // 256 MB of doubles
 constexpr unsigned kGuardResolveBlocks = 16;
 
 static unsigned guard_ranges(unsigned k_half) {
     size_t r = kGuardPartialMax / (2 * (size_t)k_half);
-    r = std::min<size_t>(r, kGuardRangesMax);
+    r = std::min<size_t>(r, kGuardRangesMax) / 8 * 8;
     return (unsigned)std::max<size_t>(r, 8);
 }
 

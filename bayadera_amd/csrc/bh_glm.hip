@@ -363,17 +363,21 @@ glm_gauss_fp64_kernel(const float* __restrict__ X, const float* __restrict__ yv,
     partial[(size_t)range * n + k] = acc;
 }
 
-// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)) in double, to an
-// absolute error of a few 1e-16 (the guard sums it over up to 2^32 rows and
-// needs the total to ~1e-9, far above this; the library's exp and log1p aim
-// at last-bit accuracy and cost about three times as many instructions).
+// The softplus sums of the guard in double: softplus(eta) = max(eta, 0) +
+// log1p(exp(-|eta|)), summed over a run of up to 16 rows as the sum of the
+// max(eta, 0) plus the logarithm of the product of the factors
+// 1 + exp(-|eta|) (each in (1, 2], the product in [1, 2^16], one rounding
+// per factor), to an absolute error of a few 1e-16 per row (the guard sums
+// over up to 2^32 rows and needs the total to ~1e-9, far above this; the
+// library's exp and log1p aim at last-bit accuracy and cost about three
+// times as many instructions).
 //   t = exp(-a): a = n ln2 + r with n = rint(a / ln2), ln2 split in a
 //     32-bit head (n ln2_hi exact) and a tail, |r| <= ln2 / 2; exp(-r) by
 //     its Taylor series to degree 13 (remainder < 5e-18), scaled by 2^-n.
-//   log1p(t), t in [0, 1]: 1 + t = 2^k (1 + f) with k = 1 when
-//     t > sqrt(2) - 1 (f = (t - 1) / 2), else k = 0 (f = t), so
-//     f in [-0.293, 0.415]; log(1 + f) = 2 atanh(s), s = f / (2 + f),
-//     |s| <= 0.172, by its series to s^23 (remainder < 1e-18), plus k ln2.
+//   log P, P in [1, 2^16]: P = 2^k m with m in [sqrt(1/2), sqrt(2)) (frexp,
+//     then one doubling), f = m - 1 exact, log m = 2 atanh(s), s = f / (2 +
+//     f), |s| <= 0.172, by its series to s^23 (remainder < 1e-18), plus
+//     k ln2.
 // Its constants, read from memory by the kernel that uses it (as immediates
 // the compiler keeps them all in scalar registers, which the list kernel's
 // coefficient loads need): ln2 head (32 significant bits) and tail, 1/ln2,
@@ -386,8 +390,9 @@ __constant__ double kGlmSoftplus64[3 + 14 + 12] = {
     2.0 / 23.0, 2.0 / 21.0, 2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0,
     2.0 / 11.0, 2.0 / 9.0, 2.0 / 7.0, 2.0 / 5.0, 2.0 / 3.0, 2.0};
 
-__device__ __forceinline__ double glm_softplus64(double eta, const double* __restrict__ k) {
-    const double a = __builtin_fabs(eta);
+// exp(-a) for a >= 0 (0 past the double range; a NaN gives 0, the caller
+// carries it through the max(eta, 0) sum)
+__device__ __forceinline__ double glm_exp_neg64(double a, const double* __restrict__ k) {
     const double n = __builtin_rint(a * k[2]);
     double r = __builtin_fma(-n, k[0], a);
     r = __builtin_fma(-n, k[1], r);
@@ -397,104 +402,136 @@ __device__ __forceinline__ double glm_softplus64(double eta, const double* __res
 #pragma unroll
     for (int j = 4; j < 17; ++j) q = __builtin_fma(q, x, k[j]);
     const bool small = a < 745.0;  // else exp(-a) underflows to 0
-    const double t = small ? __builtin_ldexp(q, -(int)(small ? n : 0.0)) : 0.0;
-    const bool up = t > 0.41421356237309503;  // sqrt(2) - 1
-    const double f = up ? (t - 1.0) * 0.5 : t;
+    return small ? __builtin_ldexp(q, -(int)(small ? n : 0.0)) : 0.0;
+}
+
+// log P for P in [1, 2^16] (log 1 = 0 exactly)
+__device__ __forceinline__ double glm_log64(double P, const double* __restrict__ k) {
+    int e;
+    double m = __builtin_frexp(P, &e);  // P = m 2^e, m in [1/2, 1)
+    if (m < 0.70710678118654752) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double f = m - 1.0;
     const double sv = f / (2.0 + f);
     const double s2 = sv * sv;
     // 2 atanh(s) = s (2 + 2 s^2/3 + ... + 2 s^22/23)
     double g = k[17];
 #pragma unroll
     for (int j = 18; j < 29; ++j) g = __builtin_fma(g, s2, k[j]);
-    double l = sv * g;
-    if (up) l = k[0] + (k[1] + l);
-    return eta != eta ? eta : __builtin_fmax(eta, 0.0) + l;
+    const double de = (double)e;
+    return __builtin_fma(de, k[0], __builtin_fma(de, k[1], sv * g));
 }
 
 // fp64 log-likelihood sums of the points listed by the guarded accept test.
-// pts holds them in double, row i < c a proposal and row lmax + i its
-// walker's position (c = min(*count, lmax), read on the device).  A block
-// takes one row range; each thread a row at a time, the row converted to
-// double eight features at a time, against eight points whose coefficients
-// are uniform across the wave (scalar loads).  fma chains over the features
-// in order as the fp64 kernel, then glm_softplus64 (within a few 1e-16 per
-// row of the fp64 kernel's library softplus).  partial64[r * 2 lmax + i]
-// gets the block's sum for point row i.
+// pts holds the 2c listed points in double (c = min(*count, lmax), read on
+// the device): row 2i the proposal of listed walker i, row 2i + 1 its
+// position.  Blocks come in groups of kGuardSlots per row range, dealt so
+// that the group's blocks run on one XCD one after the other
+// (block b: XCD b % 8), and the range's rows, read from HBM by the first,
+// come from that XCD's L2 for the others; slot s takes the points 8s .. 8s +
+// 7, then 8 (s + kGuardSlots) .., and so on.  Each thread takes a row at a
+// time, the row converted to double eight features at a time, against eight
+// points whose coefficients are uniform across the wave (scalar loads): fma
+// chains over the features in order as the fp64 kernel, then the run
+// softplus above.  partial64[r * 2 lmax + j] gets the block's sum for point
+// row j < 2c.
 template <int D>
 __global__ void __launch_bounds__(256)
 glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
                      const double* __restrict__ pts, const unsigned* __restrict__ count,
-                     unsigned lmax, unsigned long long rows_per_range,
+                     unsigned lmax, unsigned nranges, unsigned long long rows_per_range,
                      double* __restrict__ partial64) {
     constexpr int PC = 8;
     __shared__ double red[PC][4];
     const unsigned n = *count;
     const unsigned c = n < lmax ? n : lmax;
-    if (c == 0) return;
+    const unsigned m = 2 * c;
+    const unsigned b = blockIdx.x, idx = b >> 3;
+    const unsigned slot = idx % kGuardSlots;
+    const unsigned range = (idx / kGuardSlots) * 8 + (b & 7);
+    if (slot * PC >= m || range >= nranges) return;
     const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_range;
+    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
     unsigned long long r1 = r0 + rows_per_range;
     if (r1 > N) r1 = N;
-    const unsigned chunks = (c + PC - 1) / PC;
-    for (unsigned h = 0; h < 2; ++h) {
-        for (unsigned ch = 0; ch < chunks; ++ch) {
-            const unsigned p0 = h * lmax + ch * PC;
-            const double* b = pts + (size_t)p0 * D;
-            double acc[PC];
+    const double* k = kGlmSoftplus64;
+    for (unsigned p0 = slot * PC; p0 < m; p0 += kGuardSlots * PC) {
+        const double* bp = pts + (size_t)p0 * D;
+        double lsum[PC], prod[PC];
 #pragma unroll
-            for (int q = 0; q < PC; ++q) acc[q] = 0.0;
-            for (unsigned long long row = r0 + t; row < r1; row += 256) {
-                // the coefficients are read again for every row (a compiler
-                // barrier keeps all 8 x D of them from being hoisted into
-                // scalar registers, which would spill)
-                asm volatile("" ::: "memory");
-                const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
-                double eta[PC];
+        for (int q = 0; q < PC; ++q) {
+            lsum[q] = 0.0;
+            prod[q] = 1.0;
+        }
+        unsigned run = 0;
+        for (unsigned long long row = r0 + t; row < r1; row += 256) {
+            // the coefficients are read again for every row (a compiler
+            // barrier keeps all 8 x D of them from being hoisted into
+            // scalar registers, which would spill)
+            asm volatile("" ::: "memory");
+            const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
+            double eta[PC];
 #pragma unroll
-                for (int q = 0; q < PC; ++q) eta[q] = 0.0;
+            for (int q = 0; q < PC; ++q) eta[q] = 0.0;
 #pragma unroll
-                for (int v = 0; v < D / 8; ++v) {
-                    const bf16x8 xv = xr[v];
-                    double x[8];
+            for (int v = 0; v < D / 8; ++v) {
+                const bf16x8 xv = xr[v];
+                double x[8];
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) x[e] = (double)(float)xv[e];
+                for (int e = 0; e < 8; ++e) x[e] = (double)(float)xv[e];
 #pragma unroll
-                    for (int q = 0; q < PC; ++q)
+                for (int q = 0; q < PC; ++q)
 #pragma unroll
-                        for (int e = 0; e < 8; ++e)
-                            eta[q] = __builtin_fma(x[e], b[q * D + v * 8 + e], eta[q]);
-                }
-                // the softplus constants are read after the coefficients
-                asm volatile("" ::: "memory");
-#pragma unroll
-                for (int q = 0; q < PC; ++q) acc[q] += glm_softplus64(eta[q], kGlmSoftplus64);
+                    for (int e = 0; e < 8; ++e)
+                        eta[q] = __builtin_fma(x[e], bp[q * D + v * 8 + e], eta[q]);
             }
+            // the softplus constants are read after the coefficients
+            asm volatile("" ::: "memory");
 #pragma unroll
             for (int q = 0; q < PC; ++q) {
-                double v = acc[q];
-#pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-                if (lane == 0) red[q][wave] = v;
+                const double et = eta[q];
+                prod[q] = __builtin_fma(glm_exp_neg64(__builtin_fabs(et), k), prod[q], prod[q]);
+                lsum[q] += et != et ? et : __builtin_fmax(et, 0.0);
             }
-            __syncthreads();
-            if (t < PC && ch * PC + t < c)
-                partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
-                    (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
-            __syncthreads();
+            if (++run == 16) {
+#pragma unroll
+                for (int q = 0; q < PC; ++q) {
+                    lsum[q] += glm_log64(prod[q], k);
+                    prod[q] = 1.0;
+                }
+                run = 0;
+            }
         }
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+            double v = lsum[q] + glm_log64(prod[q], k);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[q][wave] = v;
+        }
+        __syncthreads();
+        if (t < PC && p0 + t < m)
+            partial64[(size_t)range * 2 * lmax + p0 + t] =
+                (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+        __syncthreads();
     }
 }
 
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
                                 const double* pts, const unsigned* count, unsigned lmax,
                                 unsigned nranges, double* partial64) {
-    if (lmax % 8) return hipErrorInvalidValue;
+    if (lmax % 8 || nranges % 8 || nranges == 0) return hipErrorInvalidValue;
     const unsigned long long rpr = (N + nranges - 1) / nranges;
     const __bf16* x = static_cast<const __bf16*>(X);
+    const unsigned blocks = nranges * kGuardSlots;
     if (d == 64)
-        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, pts, count, lmax, rpr, partial64);
+        glm_fp64_list_kernel<64><<<blocks, 256, 0, s>>>(x, N, pts, count, lmax, nranges, rpr,
+                                                        partial64);
     else if (d == 32)
-        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, pts, count, lmax, rpr, partial64);
+        glm_fp64_list_kernel<32><<<blocks, 256, 0, s>>>(x, N, pts, count, lmax, nranges, rpr,
+                                                        partial64);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

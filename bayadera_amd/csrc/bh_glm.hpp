@@ -41,10 +41,13 @@ size_t glm_x_f16_bytes(unsigned long long N, unsigned d);
 hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long N, unsigned d,
                                void* out, unsigned* overflow);
 // fp64 softplus sums of the points listed by the guarded accept test: pts
-// (double, 2 lmax x d) holds in rows i < c the proposals and in rows
-// lmax + i their walkers' positions (c = min(*count, lmax), read on the
-// device); partial64[r * 2 lmax + row] for r < nranges.  Does nothing when
-// the list is empty.  lmax a multiple of 8.
+// (double, 2 lmax x d) holds in row 2i the proposal of listed walker i and
+// in row 2i + 1 its position (c = min(*count, lmax) listed, read on the
+// device); partial64[r * 2 lmax + row] for r < nranges, rows < 2c.  Does
+// nothing when the list is empty.  lmax and nranges multiples of 8; the
+// launch has nranges x kGuardSlots work-groups, each taking the points of
+// one slot (eight at a time, kGuardSlots x 8 apart) on one row range.
+constexpr unsigned kGuardSlots = 8;
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
                                 const double* pts, const unsigned* count, unsigned lmax,
                                 unsigned nranges, double* partial64);

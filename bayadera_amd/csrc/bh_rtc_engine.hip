@@ -1159,11 +1159,11 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
                 list[i] = k;
                 f = 3;
                 // the two points in double for the fp64 likelihood kernel:
-                // row i the proposal, row lmax + i the current position
+                // row 2i the proposal, row 2i + 1 the current position
 #pragma unroll
                 for (int j = 0; j < DIM; ++j) {
-                    pts64[(size_t)i * DIM + j] = (double)y[j];
-                    pts64[((size_t)lmax + i) * DIM + j] = (double)Xpos[(size_t)k * DIM + j];
+                    pts64[(size_t)2 * i * DIM + j] = (double)y[j];
+                    pts64[((size_t)2 * i + 1) * DIM + j] = (double)Xpos[(size_t)k * DIM + j];
                 }
             }
         }
@@ -1176,8 +1176,8 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
 
 // Decides the listed walkers on their fp64 log posteriors, one wave per
 // walker: partial64 holds, per row range r (row stride `stride`), column
-// i < c the softplus sum of Y[list[i]] and column lmax + i that of
-// X[list[i]] (natural units).
+// 2i the softplus sum of Y[list[i]] and column 2i + 1 that of X[list[i]]
+// (natural units).
 // totals[0] += listed walkers, totals[1] += walkers past lmax.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
@@ -1195,8 +1195,8 @@ bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
     }
     const unsigned waves = WGS / 64;
     for (unsigned i = blockIdx.x * waves + threadIdx.x / 64; i < c; i += gridDim.x * waves) {
-        const double sy = bh_wave_range_sum(partial64, nranges64, stride, i);
-        const double sx = bh_wave_range_sum(partial64, nranges64, stride, lmax + i);
+        const double sy = bh_wave_range_sum(partial64, nranges64, stride, 2 * i);
+        const double sx = bh_wave_range_sum(partial64, nranges64, stride, 2 * i + 1);
         if ((threadIdx.x & 63u) == 0) {
             const unsigned k = list[i];
             float y[DIM], xk[DIM];
