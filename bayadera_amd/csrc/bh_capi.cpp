@@ -359,11 +359,17 @@ struct bh_stretch {
     // guarded accept test of the reduced precisions (guard 0: off)
     double glm_guard = 0.0;
     DevBuf gflag, glist, gcount, gtotals, gpart64, gpts64, glpy;
+    // the split precisions' first-order correction (logistic): the softplus
+    // sum's gradient at a reference point, its block partials, and each
+    // point's correction
+    DevBuf gg0, ggpart, gcorr;
+    int g0_age = -1;  // steps since gg0 was computed; -1: not valid
     // event pairs around the dominant kernel's launches (the stretch move
     // kernels, or the GLM likelihood contraction), up to the pool's size
     bool ktiming = false;
     std::vector<hipEvent_t> kev;
     size_t kused = 0;
+    uint32_t kstride = 1, kcount = 0;  // every kstride-th launch is timed
 };
 
 constexpr size_t kKernelTimerPairs = 1024;
@@ -373,13 +379,11 @@ constexpr unsigned kGlmRounds = 4;
 
 // Guarded accept test: the list holds every walker of a half (K), so no
 // walker inside the guard is ever decided on reduced-precision values; the
-// fp64 pass over the listed points uses up to kGuardRangesMax row ranges (a
-// multiple of 8; with its kGuardSlots work-groups per range, of which ~5
-// have points at the usual ~18 listed walkers, the busy ones fit the device
-// at once, five per CU, with ~15 rows per thread at 1M rows), fewer for
-// very large halves so that its partial sums (ranges x 2K doubles) stay
-// within kGuardPartialMax doubles.
-constexpr unsigned kGuardRangesMax = 256;
+// fp64 pass over the listed points uses up to kGuardRangesMax row ranges
+// (one work-group each, ~4 rows per thread at 1M rows), fewer for very
+// large halves so that its partial sums (ranges x 2K doubles) stay within
+// kGuardPartialMax doubles.
+constexpr unsigned kGuardRangesMax = 1024;
 constexpr size_t kGuardPartialMax = (size_t)1 << 25;  // 256 MB of doubles
 constexpr unsigned kGuardResolveBlocks = 16;
 
@@ -423,7 +427,8 @@ struct Timed {
 // when kernel timing is on and the pool has room.
 template <class F>
 int ktimed(bh_stretch* s, F&& launch_fn) {
-    const bool rec = s->ktiming && s->kused + 2 <= s->kev.size();
+    const bool rec = s->ktiming && (s->kcount++ % s->kstride) == 0 &&
+                     s->kused + 2 <= s->kev.size();
     if (rec) BH_HIP(hipEventRecord(s->kev[s->kused], s->stream));
     BH_TRY(launch_fn());
     if (rec) {
@@ -485,11 +490,31 @@ int run_resident(bh_stretch* s, int mode, uint32_t seed_odd, uint32_t seed_even,
 // ---------------------------------------------------------------- GLM
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr float kLog2e = 1.44269504088896340736f;
+// threads per walker of bh_glm_propose (its BH_GLM_TPW)
+constexpr unsigned kGlmProposeThreads = 16;
+// steps between refreshes of the split precisions' correction gradient
+// (the ensemble's reference point moves little in a few steps; the
+// correction needs g0 to ~1e-3 relative)
+constexpr int kGlmGradRefresh = 4;
 
 int glm_nsplit(int prec) {
     return prec == BH_GLM_BF16X3 ? 3 : (prec == BH_GLM_FP16X2 ? 2 : (prec == BH_GLM_BF16 ? 1 : 0));
 }
 int glm_f16(int prec) { return prec == BH_GLM_FP16X2 ? 1 : 0; }
+
+// The gradient g0 of the logistic softplus sum at the mean of the first
+// min(n, 256) points of P (device, row stride d), for the split precisions'
+// first-order correction; nullptr (no correction) for the other precisions
+// and models.
+int glm_grad_ref(bh_stretch* s, const float* P, unsigned n, const double** g0) {
+    const GlmData& g = *s->f->glm;
+    *g0 = nullptr;
+    if (g.kind != BH_GLM_LOGISTIC || glm_nsplit(s->glm_prec) == 0 || n == 0) return BH_OK;
+    BH_HIP(bh::launch_glm_grad(s->stream, g.X.p, g.N, g.d, P, std::min(n, 256u),
+                               s->ggpart.as<double>(), s->gg0.as<double>()));
+    *g0 = s->gg0.as<double>();
+    return BH_OK;
+}
 
 // Checks a precision against the model and makes what it needs: the fp16
 // copy of X for BH_GLM_FP16X2 (refused when a value of X is outside the
@@ -546,8 +571,13 @@ int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
     int f16 = glm_f16(prec);
     float yscale = kLog2e;
     void* ys = s->gsplit.p;
+    double* corr = nullptr;
     if (nsplit) {
-        void* a[] = {&P, &n, &ys, &yscale, &nsplit, &f16};
+        const double* g0 = nullptr;
+        BH_TRY(glm_grad_ref(s, P, n, &g0));
+        double lnscale = kLn2;
+        corr = s->gcorr.as<double>();
+        void* a[] = {&P, &n, &ys, &yscale, &nsplit, &f16, &g0, &lnscale, &corr};
         BH_TRY(launch(f->glm_split, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a));
     }
     double* partial = s->gpartial.as<double>();
@@ -557,7 +587,8 @@ int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
     double pp = g.prior_prec;
     unsigned nr = nranges;
     const double* pc = partial;
-    void* a[] = {&P, &n, &pc, &nr, &lin, &scale, &pp, &lp};
+    const double* cc = corr;
+    void* a[] = {&P, &n, &pc, &nr, &lin, &scale, &pp, &lp, &cc};
     return launch(f->glm_logpost, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a);
 }
 
@@ -582,9 +613,21 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     const float* Xc = X;
     const bool guarded = s->glm_guard > 0.0 && prec != BH_GLM_FP64;
     unsigned* gcount = guarded ? s->gcount.as<unsigned>() : nullptr;
+    // the correction's gradient at the mean of 256 walkers of the ensemble,
+    // every kGlmGradRefresh steps (both halves use it)
+    const double* g0 = nullptr;
+    if (nsplit && g.kind == BH_GLM_LOGISTIC) {
+        if (first && (s->g0_age < 0 || ++s->g0_age >= kGlmGradRefresh)) {
+            BH_TRY(glm_grad_ref(s, s->xs.as<float>(), s->W, &g0));
+            s->g0_age = 0;
+        }
+        g0 = s->gg0.as<double>();
+    }
+    double lnscale = kLn2;
+    double* corr = nsplit ? s->gcorr.as<double>() : nullptr;
     void* pa[] = {&seed, &oe, &S, &Xc, &a, &step, &K, &partner, &Y, &z, &u2, &ys, &yscale, &nsplit,
-                  &f16, &gcount};
-    BH_TRY(launch(f->glm_propose, K / s->wgs, s->wgs, 0, s->stream, pa));
+                  &f16, &gcount, &g0, &lnscale, &corr};
+    BH_TRY(launch(f->glm_propose, K * kGlmProposeThreads / s->wgs, s->wgs, 0, s->stream, pa));
     unsigned nranges = glm_ranges(s->ctx, g, prec, K);
     const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
     double* partial = s->gpartial.as<double>();
@@ -612,8 +655,9 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         const double* lpc = lp;
         double* pts64 = s->gpts64.as<double>();
         double* lpy = s->glpy.as<double>();
+        const double* cc = corr;
         void* sa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &lpc, &beta, &K, &guard,
-                      &flag, &list, &gcount, &lmax, &Xc, &pts64, &lpy};
+                      &flag, &list, &gcount, &lmax, &Xc, &pts64, &lpy, &cc};
         BH_TRY(launch(f->glm_screen, K / s->wgs, s->wgs, 0, s->stream, sa));
         double* part64 = s->gpart64.as<double>();
         BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, pts64, gcount, lmax,
@@ -639,8 +683,9 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         return BH_OK;
     }
     const unsigned* flagc = flag;
+    const double* cc = corr;
     void* aa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &X, &lp, &beta, &K, &accu,
-                  &acc, &means, &mstep, &flagc};
+                  &acc, &means, &mstep, &flagc, &cc};
     return launch(f->glm_accept, K / s->wgs, s->wgs, 0, s->stream, aa);
 }
 
@@ -657,6 +702,10 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
                 maxr = std::max(maxr, (size_t)glm_ranges(s->ctx, g, prec, n) * n);
     BH_TRY(s->gpartial.alloc(std::max<size_t>(maxr, 1) * sizeof(double)));
     BH_TRY(s->gsplit.alloc((size_t)3 * n_points * dim * sizeof(uint16_t)));
+    BH_TRY(s->gcorr.alloc((size_t)std::max(std::max(n_points, k_half), 1u) * sizeof(double)));
+    BH_TRY(s->gg0.alloc((size_t)std::max<size_t>(dim, 64) * sizeof(double)));
+    BH_TRY(s->ggpart.alloc(bh::glm_grad_work_doubles(std::max<unsigned>(s->dim, 64)) *
+                           sizeof(double)));
     if (k_half) {
         BH_TRY(s->gY.alloc((size_t)k_half * dim * sizeof(float)));
         BH_TRY(s->gz.alloc((size_t)k_half * sizeof(float)));
@@ -1619,7 +1668,9 @@ int bh_stretch_set_kernel_timing(bh_stretch* s, int on) {
         for (hipEvent_t& e : s->kev) BH_HIP(hipEventCreate(&e));
     }
     BH_HIP(hipStreamSynchronize(s->stream));
-    s->ktiming = on != 0;
+    s->ktiming = on > 0;
+    s->kstride = on > 0 ? (uint32_t)on : 1u;
+    s->kcount = 0;
     s->kused = 0;
     return BH_OK;
 }
@@ -2600,7 +2651,10 @@ int bh_stretch_init(bh_stretch* s, uint32_t seed) {
 }
 
 static int logfn_all(bh_stretch* s) {
-    if (s->f->glm) return glm_logpost_points(s, s->xs.as<float>(), s->W, s->lp.as<double>());
+    if (s->f->glm) {
+        s->g0_age = -1;  // new positions: the next step computes the gradient afresh
+        return glm_logpost_points(s, s->xs.as<float>(), s->W, s->lp.as<double>());
+    }
     unsigned data_len = s->data_len, params_len = s->params_len, n = s->W;
     const float* params = s->params.as<float>();
     const float* xs = s->xs.as<float>();

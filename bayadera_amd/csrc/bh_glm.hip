@@ -427,37 +427,31 @@ __device__ __forceinline__ double glm_log64(double P, const double* __restrict__
 // fp64 log-likelihood sums of the points listed by the guarded accept test.
 // pts holds the 2c listed points in double (c = min(*count, lmax), read on
 // the device): row 2i the proposal of listed walker i, row 2i + 1 its
-// position.  Blocks come in groups of kGuardSlots per row range, dealt so
-// that the group's blocks run on one XCD one after the other
-// (block b: XCD b % 8), and the range's rows, read from HBM by the first,
-// come from that XCD's L2 for the others; slot s takes the points 8s .. 8s +
-// 7, then 8 (s + kGuardSlots) .., and so on.  Each thread takes a row at a
-// time, the row converted to double eight features at a time, against eight
-// points whose coefficients are uniform across the wave (scalar loads): fma
-// chains over the features in order as the fp64 kernel, then the run
-// softplus above.  partial64[r * 2 lmax + j] gets the block's sum for point
-// row j < 2c.
+// position.  A work-group takes one row range and the points eight at a
+// time (its rows come from L2 after the first eight); each thread a row at
+// a time, the row's pieces loaded together, converted to double eight
+// features at a time, against eight points whose coefficients are uniform
+// across the wave (scalar loads): fma chains over the features in order as
+// the fp64 kernel, then the run softplus above.  partial64[r * 2 lmax + j]
+// gets the work-group's sum for point row j < 2c.
 template <int D>
 __global__ void __launch_bounds__(256)
 glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
                      const double* __restrict__ pts, const unsigned* __restrict__ count,
-                     unsigned lmax, unsigned nranges, unsigned long long rows_per_range,
+                     unsigned lmax, unsigned long long rows_per_range,
                      double* __restrict__ partial64) {
     constexpr int PC = 8;
     __shared__ double red[PC][4];
     const unsigned n = *count;
     const unsigned c = n < lmax ? n : lmax;
     const unsigned m = 2 * c;
-    const unsigned b = blockIdx.x, idx = b >> 3;
-    const unsigned slot = idx % kGuardSlots;
-    const unsigned range = (idx / kGuardSlots) * 8 + (b & 7);
-    if (slot * PC >= m || range >= nranges) return;
+    if (m == 0) return;
     const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const unsigned long long r0 = (unsigned long long)range * rows_per_range;
+    const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_range;
     unsigned long long r1 = r0 + rows_per_range;
     if (r1 > N) r1 = N;
     const double* k = kGlmSoftplus64;
-    for (unsigned p0 = slot * PC; p0 < m; p0 += kGuardSlots * PC) {
+    for (unsigned p0 = 0; p0 < m; p0 += PC) {
         const double* bp = pts + (size_t)p0 * D;
         double lsum[PC], prod[PC];
 #pragma unroll
@@ -467,20 +461,23 @@ glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
         }
         unsigned run = 0;
         for (unsigned long long row = r0 + t; row < r1; row += 256) {
-            // the coefficients are read again for every row (a compiler
-            // barrier keeps all 8 x D of them from being hoisted into
-            // scalar registers, which would spill)
-            asm volatile("" ::: "memory");
             const bf16x8* xr = reinterpret_cast<const bf16x8*>(X + row * D);
+            bf16x8 xv[D / 8];
+#pragma unroll
+            for (int v = 0; v < D / 8; ++v) xv[v] = xr[v];
+            // the row's loads are issued before any of its arithmetic, and
+            // the coefficients are read again for every row (the barrier
+            // also keeps all 8 x D of them from being hoisted into scalar
+            // registers, which would spill)
+            asm volatile("" ::: "memory");
             double eta[PC];
 #pragma unroll
             for (int q = 0; q < PC; ++q) eta[q] = 0.0;
 #pragma unroll
             for (int v = 0; v < D / 8; ++v) {
-                const bf16x8 xv = xr[v];
                 double x[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) x[e] = (double)(float)xv[e];
+                for (int e = 0; e < 8; ++e) x[e] = (double)(float)xv[v][e];
 #pragma unroll
                 for (int q = 0; q < PC; ++q)
 #pragma unroll
@@ -513,7 +510,7 @@ glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
         }
         __syncthreads();
         if (t < PC && p0 + t < m)
-            partial64[(size_t)range * 2 * lmax + p0 + t] =
+            partial64[(size_t)blockIdx.x * 2 * lmax + p0 + t] =
                 (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
         __syncthreads();
     }
@@ -522,18 +519,153 @@ glm_fp64_list_kernel(const __bf16* __restrict__ X, unsigned long long N,
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
                                 const double* pts, const unsigned* count, unsigned lmax,
                                 unsigned nranges, double* partial64) {
-    if (lmax % 8 || nranges % 8 || nranges == 0) return hipErrorInvalidValue;
+    if (lmax % 8 || nranges == 0) return hipErrorInvalidValue;
     const unsigned long long rpr = (N + nranges - 1) / nranges;
     const __bf16* x = static_cast<const __bf16*>(X);
-    const unsigned blocks = nranges * kGuardSlots;
     if (d == 64)
-        glm_fp64_list_kernel<64><<<blocks, 256, 0, s>>>(x, N, pts, count, lmax, nranges, rpr,
-                                                        partial64);
+        glm_fp64_list_kernel<64><<<nranges, 256, 0, s>>>(x, N, pts, count, lmax, rpr, partial64);
     else if (d == 32)
-        glm_fp64_list_kernel<32><<<blocks, 256, 0, s>>>(x, N, pts, count, lmax, nranges, rpr,
-                                                        partial64);
+        glm_fp64_list_kernel<32><<<nranges, 256, 0, s>>>(x, N, pts, count, lmax, rpr, partial64);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Gradient of the logistic softplus sum at a reference point, for the
+// first-order correction of the split precisions:
+// g0 = sum_r sigma(x_r . b) x_r (natural units), b the mean of the first
+// nref (1 .. 256) points of pts (row stride D).  The split precisions
+// evaluate the likelihood at coefficients y' = y + dy that differ from the
+// proposal y by the split's rounding (|dy| ~ 2^-22 |y| for fp16x2), and
+// g(y) . dy is nearly all of their log posterior's error against fp64;
+// with g0 from a point of the ensemble in place of g(y) the correction
+// leaves (g(y) - g0) . dy.  Only ~1e-3 relative accuracy is needed of g0:
+// rows in fp32 (u, sigma, the thread's sums), blocks and the total in
+// double.  Three launches: the reference point (one work-group), the rows
+// (one per thread, the next row's loads in flight; partial[block * D + j]),
+// the sum over blocks.
+template <int D>
+__global__ void __launch_bounds__(1024)
+glm_grad_ref_kernel(const float* __restrict__ pts, unsigned nref, float* __restrict__ bref) {
+    constexpr unsigned G = 1024 / D, PER = 256 / G;
+    __shared__ double red[1024];  // G x D
+    const unsigned t = threadIdx.x, j = t % D, g = t / D;
+    float v[PER];
+#pragma unroll
+    for (unsigned i = 0; i < PER; ++i) {
+        const unsigned p = g + i * G;
+        v[i] = p < nref ? pts[(size_t)p * D + j] : 0.0f;
+    }
+    double m = 0.0;
+#pragma unroll
+    for (unsigned i = 0; i < PER; ++i) m += (double)v[i];
+    red[g * D + j] = m;
+    __syncthreads();
+    if (t < D) {
+        double sm = 0.0;
+        for (unsigned q = 0; q < G; ++q) sm += red[q * D + t];
+        bref[t] = (float)(sm / (double)nref);
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_grad_partial_kernel(const __bf16* __restrict__ X, unsigned long long N,
+                        const float* __restrict__ bref_g, unsigned long long rows_per_block,
+                        double* __restrict__ partial) {
+    __shared__ double red[4 * D];
+    __shared__ float bref[D];
+    const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    if (t < D) bref[t] = bref_g[t];
+    __syncthreads();
+    const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_block;
+    unsigned long long r1 = r0 + rows_per_block;
+    if (r1 > N) r1 = N;
+    float acc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc[j] = 0.0f;
+    // one row per thread, the next row's pieces in flight meanwhile
+    bf16x8 xn[D / 8];
+    if (r0 + t < r1) {
+#pragma unroll
+        for (int v = 0; v < D / 8; ++v) xn[v] = reinterpret_cast<const bf16x8*>(X + (r0 + t) * D)[v];
+    }
+    for (unsigned long long row = r0 + t; row < r1; row += 256) {
+        float x[D];
+#pragma unroll
+        for (int v = 0; v < D / 8; ++v)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[v * 8 + e] = (float)xn[v][e];
+        if (row + 256 < r1) {
+#pragma unroll
+            for (int v = 0; v < D / 8; ++v)
+                xn[v] = reinterpret_cast<const bf16x8*>(X + (row + 256) * D)[v];
+        }
+        float u = 0.0f;
+#pragma unroll
+        for (int j = 0; j < D; ++j) u = __builtin_fmaf(x[j], bref[j], u);
+        const float sg = 1.0f / (1.0f + __expf(-u));
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] = __builtin_fmaf(sg, x[j], acc[j]);
+    }
+    double mine = 0.0;  // lane j < D of each wave keeps the wave's sum for feature j
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        float v = acc[j];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == (unsigned)j) mine = (double)v;
+    }
+    if (lane < D) red[wave * D + lane] = mine;
+    __syncthreads();
+    if (t < D)
+        partial[(size_t)blockIdx.x * D + t] =
+            (red[t] + red[D + t]) + (red[2 * D + t] + red[3 * D + t]);
+}
+
+// g0[j] = sum over the blocks' partials: thread (g, j) sums blocks g, g + G,
+// ... (all its loads in flight), then one thread per feature the G sums in
+// order.
+template <int D>
+__global__ void __launch_bounds__(1024)
+glm_grad_final_kernel(const double* __restrict__ partial, double* __restrict__ g0) {
+    constexpr unsigned G = 1024 / D, PER = kGlmGradBlocks / G;
+    __shared__ double red[1024];
+    const unsigned t = threadIdx.x, j = t % D, g = t / D;
+    double v[PER];
+#pragma unroll
+    for (unsigned i = 0; i < PER; ++i) v[i] = partial[(size_t)(g + i * G) * D + j];
+    double s = 0.0;
+#pragma unroll
+    for (unsigned i = 0; i < PER; ++i) s += v[i];
+    red[t] = s;
+    __syncthreads();
+    if (t < D) {
+        double sm = 0.0;
+        for (unsigned q = 0; q < G; ++q) sm += red[q * D + t];
+        g0[t] = sm;
+    }
+}
+
+hipError_t launch_glm_grad(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                           const float* pts, unsigned nref, double* work, double* g0) {
+    if (nref == 0 || nref > 256) return hipErrorInvalidValue;
+    static_assert(kGlmGradBlocks % 32 == 0, "the final sum's grouping");
+    const unsigned long long rpb = (N + kGlmGradBlocks - 1) / kGlmGradBlocks;
+    const __bf16* x = static_cast<const __bf16*>(X);
+    // work: kGlmGradBlocks x d partials, then the reference point (d floats)
+    float* bref = reinterpret_cast<float*>(work + (size_t)kGlmGradBlocks * d);
+    if (d == 64) {
+        glm_grad_ref_kernel<64><<<1, 1024, 0, s>>>(pts, nref, bref);
+        glm_grad_partial_kernel<64><<<kGlmGradBlocks, 256, 0, s>>>(x, N, bref, rpb, work);
+        glm_grad_final_kernel<64><<<1, 1024, 0, s>>>(work, g0);
+    } else if (d == 32) {
+        glm_grad_ref_kernel<32><<<1, 1024, 0, s>>>(pts, nref, bref);
+        glm_grad_partial_kernel<32><<<kGlmGradBlocks, 256, 0, s>>>(x, N, bref, rpb, work);
+        glm_grad_final_kernel<32><<<1, 1024, 0, s>>>(work, g0);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

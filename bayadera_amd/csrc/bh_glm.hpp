@@ -43,11 +43,18 @@ hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long 
 // fp64 softplus sums of the points listed by the guarded accept test: pts
 // (double, 2 lmax x d) holds in row 2i the proposal of listed walker i and
 // in row 2i + 1 its position (c = min(*count, lmax) listed, read on the
-// device); partial64[r * 2 lmax + row] for r < nranges, rows < 2c.  Does
-// nothing when the list is empty.  lmax and nranges multiples of 8; the
-// launch has nranges x kGuardSlots work-groups, each taking the points of
-// one slot (eight at a time, kGuardSlots x 8 apart) on one row range.
-constexpr unsigned kGuardSlots = 8;
+// device); partial64[r * 2 lmax + row] for r < nranges, rows < 2c, one
+// work-group per row range.  Does nothing when the list is empty.  lmax a
+// multiple of 8.
+// Gradient of the logistic softplus sum at the mean of the first nref
+// (1 .. 256) points of pts (device, row stride d = 32 or 64), natural units,
+// into g0[d] (double); work holds glm_grad_work_doubles(d) doubles.  The
+// split precisions' first-order correction: their log posterior at y plus
+// g0 . (y' - y), y' the coefficients the split represents.
+constexpr unsigned kGlmGradBlocks = 512;
+inline size_t glm_grad_work_doubles(unsigned d) { return (size_t)kGlmGradBlocks * d + d; }
+hipError_t launch_glm_grad(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                           const float* pts, unsigned nref, double* work, double* g0);
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
                                 const double* pts, const unsigned* count, unsigned lmax,
                                 unsigned nranges, double* partial64);

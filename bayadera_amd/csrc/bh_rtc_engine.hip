@@ -982,62 +982,113 @@ bh_batch_logfn(const unsigned* __restrict__ dims, const unsigned long long* __re
 
 // Writes yscale*Y as nsplit 16-bit terms (bf16, or fp16 when f16 is set),
 // hi + mid + lo, each the nearest value of what the previous terms leave
-// (exact residuals in fp32).
-__device__ __forceinline__ void bh_glm_split(const float (&y)[DIM], float yscale, int nsplit,
-                                             int f16, void* out, unsigned k, unsigned n) {
+// (exact residuals in fp32).  With g0 (the logistic softplus sum's gradient
+// at a point of the ensemble, natural units) it returns the first-order
+// correction of the log posterior for the split's rounding:
+// g0 . (lnscale * terms - y), lnscale = 1 / yscale in the epilogue's units
+// (0 when not finite).  The split's log posterior plus it is the log
+// posterior at y up to (g(y) - g0) . (lnscale * terms - y).
+__device__ __forceinline__ double bh_glm_split(const float (&y)[DIM], float yscale, int nsplit,
+                                               int f16, void* out, unsigned k, unsigned n,
+                                               const double* __restrict__ g0, double lnscale) {
+    double c = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         float r = y[i] * yscale;
+        double hs = 0.0;  // what the terms represent, in units of 1 / yscale
         const size_t o = (size_t)k * DIM + i;
         for (int sp = 0; sp < nsplit; ++sp) {
             if (f16) {
                 const _Float16 h = (_Float16)r;
                 static_cast<_Float16*>(out)[(size_t)sp * n * DIM + o] = h;
                 r = r - (float)h;
+                hs += (double)(float)h;
             } else {
                 const __bf16 h = (__bf16)r;
                 static_cast<__bf16*>(out)[(size_t)sp * n * DIM + o] = h;
                 r = r - (float)h;
+                hs += (double)(float)h;
             }
         }
+        if (g0) c = __builtin_fma(g0[i], __builtin_fma(hs, lnscale, -(double)y[i]), c);
     }
+    return __builtin_isfinite(c) ? c : 0.0;
 }
 
 // Proposals of one half-step: Y (fp32), z and u2 per walker, and the 16-bit
-// split of log2(e)*Y for the MFMA kernels; clears the guard list's count.
+// split of log2(e)*Y for the MFMA kernels with, given g0, each proposal's
+// correction for it in corr; clears the guard list's count.  BH_GLM_TPW
+// threads per walker, thread `sub` taking coordinates sub, sub + BH_GLM_TPW,
+// ... (so a walker's threads read and write consecutive coordinates); each
+// draws the walker's uniforms (the same Philox call), and the corrections'
+// parts are summed over the walker's threads.
+#define BH_GLM_TPW 16
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                const float* __restrict__ X, float a, unsigned step, unsigned K, int partner,
                float* __restrict__ Y, float* __restrict__ zbuf, float* __restrict__ u2buf,
                void* __restrict__ ysplit, float yscale, int nsplit, int f16,
-               unsigned* __restrict__ guard_count) {
-    const unsigned k = blockIdx.x * WGS + threadIdx.x;
-    if (guard_count && k == 0) *guard_count = 0u;
-    float xk[DIM], y[DIM];
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) xk[i] = X[(size_t)k * DIM + i];
+               unsigned* __restrict__ guard_count, const double* __restrict__ g0, double lnscale,
+               double* __restrict__ corr) {
+    const unsigned gt = blockIdx.x * WGS + threadIdx.x;
+    const unsigned k = gt / BH_GLM_TPW, sub = gt % BH_GLM_TPW;
+    if (guard_count && gt == 0) *guard_count = 0u;
     float z;
+    unsigned e;
     const float u2 = partner == BH_PARTNER_FLAT
-                         ? bh_propose<BH_PARTNER_FLAT>(k, K, seed, odd_or_even, step, a, S, xk, y, z)
-                         : bh_propose<BH_PARTNER_WALKER>(k, K, seed, odd_or_even, step, a, S, xk,
-                                                         y, z);
+                         ? bh_stretch_draw<BH_PARTNER_FLAT>(k, K, seed, odd_or_even, step, a, z, e)
+                         : bh_stretch_draw<BH_PARTNER_WALKER>(k, K, seed, odd_or_even, step, a, z,
+                                                              e);
+    const unsigned total = K * DIM;
+    double c = 0.0;
+    for (unsigned i = sub; i < DIM; i += BH_GLM_TPW) {
+        unsigned ei = e + i;
+        if (partner == BH_PARTNER_FLAT && ei >= total) ei -= total;
+        const float xj = S[ei];
+        const float yi = __builtin_fmaf(z, X[(size_t)k * DIM + i] - xj, xj);
+        Y[(size_t)k * DIM + i] = yi;
+        if (nsplit > 0) {
+            float r = yi * yscale;
+            double hs = 0.0;
+            const size_t o = (size_t)k * DIM + i;
+            for (int sp = 0; sp < nsplit; ++sp) {
+                if (f16) {
+                    const _Float16 h = (_Float16)r;
+                    static_cast<_Float16*>(ysplit)[(size_t)sp * K * DIM + o] = h;
+                    r = r - (float)h;
+                    hs += (double)(float)h;
+                } else {
+                    const __bf16 h = (__bf16)r;
+                    static_cast<__bf16*>(ysplit)[(size_t)sp * K * DIM + o] = h;
+                    r = r - (float)h;
+                    hs += (double)(float)h;
+                }
+            }
+            if (g0) c = __builtin_fma(g0[i], __builtin_fma(hs, lnscale, -(double)yi), c);
+        }
+    }
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) Y[(size_t)k * DIM + i] = y[i];
-    zbuf[k] = z;
-    u2buf[k] = u2;
-    if (nsplit > 0) bh_glm_split(y, yscale, nsplit, f16, ysplit, k, K);
+    for (int off = BH_GLM_TPW / 2; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (sub == 0) {
+        zbuf[k] = z;
+        u2buf[k] = u2;
+        if (nsplit > 0 && corr) corr[k] = __builtin_isfinite(c) ? c : 0.0;
+    }
 }
 
-// 16-bit split of n given points (for the initial log-densities).
+// 16-bit split of n given points (for the initial log-densities), with their
+// corrections given g0.
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_split_points(const float* __restrict__ P, unsigned n, void* __restrict__ ysplit,
-                    float yscale, int nsplit, int f16) {
+                    float yscale, int nsplit, int f16, const double* __restrict__ g0,
+                    double lnscale, double* __restrict__ corr) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     if (k >= n) return;
     float y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = P[(size_t)k * DIM + i];
-    bh_glm_split(y, yscale, nsplit, f16, ysplit, k, n);
+    const double c = bh_glm_split(y, yscale, nsplit, f16, ysplit, k, n, g0, lnscale);
+    if (corr) corr[k] = c;
 }
 
 // Log posterior of point y from the sum s over data rows of the likelihood
@@ -1075,7 +1126,8 @@ __device__ __forceinline__ double bh_glm_logpost_sum(const float (&y)[DIM], doub
 __device__ __forceinline__ double bh_glm_logpost(const float (&y)[DIM], unsigned k, unsigned n,
                                                  const double* __restrict__ partial,
                                                  unsigned nranges, const double* __restrict__ lin,
-                                                 double scale, double prior_prec) {
+                                                 double scale, double prior_prec,
+                                                 const double* __restrict__ corr) {
     // in range order, eight loads in flight at a time
     double s = 0.0;
     unsigned r = 0;
@@ -1087,17 +1139,26 @@ __device__ __forceinline__ double bh_glm_logpost(const float (&y)[DIM], unsigned
         for (int e = 0; e < 8; ++e) s += v[e];
     }
     for (; r < nranges; ++r) s += partial[(size_t)r * n + k];
-    return bh_glm_logpost_sum(y, s, lin, scale, prior_prec);
+    const double l = bh_glm_logpost_sum(y, s, lin, scale, prior_prec);
+    return corr ? l + corr[k] : l;
 }
 
 // Sum over r < nranges of partial[r * stride + col] by one wave: lane-strided
-// partial sums, then a butterfly (a fixed order).
+// partial sums (eight loads in flight), then a butterfly (a fixed order).
 __device__ __forceinline__ double bh_wave_range_sum(const double* __restrict__ partial,
                                                     unsigned nranges, unsigned stride,
                                                     unsigned col) {
     const unsigned lane = threadIdx.x & 63u;
     double s = 0.0;
-    for (unsigned r = lane; r < nranges; r += 64) s += partial[(size_t)r * stride + col];
+    unsigned r = lane;
+    for (; r + 7 * 64 < nranges; r += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = partial[(size_t)(r + e * 64) * stride + col];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s += v[e];
+    }
+    for (; r < nranges; r += 64) s += partial[(size_t)r * stride + col];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
@@ -1107,13 +1168,13 @@ __device__ __forceinline__ double bh_wave_range_sum(const double* __restrict__ p
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_logpost(const float* __restrict__ P, unsigned n, const double* __restrict__ partial,
                unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
-               double* __restrict__ lp) {
+               double* __restrict__ lp, const double* __restrict__ corr) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     if (k >= n) return;
     float y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = P[(size_t)k * DIM + i];
-    lp[k] = bh_glm_logpost(y, k, n, partial, nranges, lin, scale, prior_prec);
+    lp[k] = bh_glm_logpost(y, k, n, partial, nranges, lin, scale, prior_prec, corr);
 }
 
 // The accept decision of stretch_move (amd-gcn-mcmc-stretch.cl:66-71): q =
@@ -1142,12 +1203,13 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
               const double* __restrict__ lp, float beta, unsigned K, double guard,
               unsigned* __restrict__ flag, unsigned* __restrict__ list,
               unsigned* __restrict__ count, unsigned lmax, const float* __restrict__ Xpos,
-              double* __restrict__ pts64, double* __restrict__ lpy_out) {
+              double* __restrict__ pts64, double* __restrict__ lpy_out,
+              const double* __restrict__ corr) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
-    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
+    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec, corr);
     lpy_out[k] = lpy;
     unsigned f = 0;
     if (__builtin_isfinite(lpy)) {
@@ -1221,14 +1283,14 @@ bh_glm_accept(const float* __restrict__ Y, const float* __restrict__ zbuf,
               unsigned nranges, const double* __restrict__ lin, double scale, double prior_prec,
               float* __restrict__ X, double* __restrict__ lp, float beta, unsigned K, int accu,
               unsigned* __restrict__ accept, float* __restrict__ means, unsigned mstep,
-              const unsigned* __restrict__ flag) {
+              const unsigned* __restrict__ flag, const double* __restrict__ corr) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     float y[DIM], xk[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
     // the stored lp stays in this sampler's precision in every case, so that
     // its common offset from the fp64 values cancels in later differences
-    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec);
+    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec, corr);
     const unsigned f = flag ? flag[k] : 0u;
     const bool accepted = (f == 1u || f == 2u)
                               ? f == 2u

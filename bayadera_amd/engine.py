@@ -659,10 +659,10 @@ class HipStretch(_Handle):
         call("bh_stretch_set_glm_guard", self._h, float(guard))
         return self
 
-    def set_kernel_timing(self, on: bool = True):
-        """HIP events around the dominant kernel's launches (the first 1024
-        after this call)."""
-        call("bh_stretch_set_kernel_timing", self._h, int(bool(on)))
+    def set_kernel_timing(self, on: bool = True, every: int = 1):
+        """HIP events around every ``every``-th launch of the dominant kernel
+        (the first 1024 such launches after this call)."""
+        call("bh_stretch_set_kernel_timing", self._h, int(every) if on else 0)
         return self
 
     def kernel_timing(self) -> dict:

@@ -298,12 +298,14 @@ int bh_stretch_variance(bh_stretch* s, float* out);
  * counterpart: the reference has no timers (amd_gcn.clj:321-324). */
 int bh_stretch_set_timing(bh_stretch* s, int on);
 int bh_stretch_last_gpu_ms(const bh_stretch* s, double* ms);
-/* Kernel timing (no reference counterpart): with it on, the launches of the
- * sampler's dominant kernel (the stretch-move kernels, the resident kernel,
- * or a GLM sampler's likelihood contraction) are bracketed by HIP events on
- * the sampler's stream, the first 1024 after the call that turned it on;
- * bh_stretch_kernel_timing waits for the stream and returns their average
- * duration and count.  Turning it on again restarts the count. */
+/* Kernel timing (no reference counterpart): with on = n >= 1, every n-th
+ * launch of the sampler's dominant kernel (the stretch-move kernels, the
+ * resident kernel, or a GLM sampler's likelihood contraction) is bracketed
+ * by HIP events on the sampler's stream, the first 1024 such launches after
+ * the call that turned it on (each pair of events costs the stream ~10 us);
+ * on = 0 turns it off.  bh_stretch_kernel_timing waits for the stream and
+ * returns their average duration and count.  Turning it on again restarts
+ * the count. */
 int bh_stretch_set_kernel_timing(bh_stretch* s, int on);
 int bh_stretch_kernel_timing(bh_stretch* s, double* avg_ms, uint32_t* launches);
 
