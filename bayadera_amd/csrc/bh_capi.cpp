@@ -490,8 +490,10 @@ int run_resident(bh_stretch* s, int mode, uint32_t seed_odd, uint32_t seed_even,
 // ---------------------------------------------------------------- GLM
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr float kLog2e = 1.44269504088896340736f;
-// threads per walker of bh_glm_propose (its BH_GLM_TPW)
+// threads per walker of bh_glm_propose and bh_glm_screen (their
+// BH_GLM_TPW, BH_GLM_STPW)
 constexpr unsigned kGlmProposeThreads = 16;
+constexpr unsigned kGlmScreenThreads = 8;
 // steps between refreshes of the split precisions' correction gradient
 // (the ensemble's reference point moves little in a few steps; the
 // correction needs g0 to ~1e-3 relative)
@@ -658,7 +660,7 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         const double* cc = corr;
         void* sa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &lpc, &beta, &K, &guard,
                       &flag, &list, &gcount, &lmax, &Xc, &pts64, &lpy, &cc};
-        BH_TRY(launch(f->glm_screen, K / s->wgs, s->wgs, 0, s->stream, sa));
+        BH_TRY(launch(f->glm_screen, K * kGlmScreenThreads / s->wgs, s->wgs, 0, s->stream, sa));
         double* part64 = s->gpart64.as<double>();
         BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, pts64, gcount, lmax,
                                         guard_ranges(K), part64));

@@ -1194,8 +1194,10 @@ __device__ __forceinline__ bool bh_glm_decide(double lpy, double lpx, float z, f
 // size in d, so it is listed (up to lmax walkers, the rest counting as
 // overflow; the engine passes lmax = K, so none overflow) for an fp64
 // evaluation of lp(Y) and lp(X) and decided on those.
+// BH_GLM_STPW threads per walker.
 // flag[k]: 0 decided here, 3 listed (bh_glm_resolve turns it into 1 = reject
 // or 2 = accept).
+#define BH_GLM_STPW 8
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
               const float* __restrict__ u2buf, const double* __restrict__ partial,
@@ -1205,35 +1207,69 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
               unsigned* __restrict__ count, unsigned lmax, const float* __restrict__ Xpos,
               double* __restrict__ pts64, double* __restrict__ lpy_out,
               const double* __restrict__ corr) {
-    const unsigned k = blockIdx.x * WGS + threadIdx.x;
-    float y[DIM];
+    const unsigned gt = blockIdx.x * WGS + threadIdx.x;
+    const unsigned k = gt / BH_GLM_STPW, sub = gt % BH_GLM_STPW;
+    const unsigned base = (threadIdx.x & 63u) & ~(unsigned)(BH_GLM_STPW - 1);
+#if BH_GLM_KIND == 2
+    // (the guard is for logistic GLMs; this form is not launched)
+    double lpy = 0.0;
+    if (sub == 0) {
+        float y[DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
-    const double lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec, corr);
-    lpy_out[k] = lpy;
-    unsigned f = 0;
-    if (__builtin_isfinite(lpy)) {
-        const double m = log((double)u2buf[k]) - (double)(DIM - 1) * log((double)zbuf[k]) -
-                         (double)beta * (lpy - lp[k]);
-        if (__builtin_fabs(m) <= guard) {
-            const unsigned i = atomicAdd(count, 1u);
-            if (i < lmax) {
-                list[i] = k;
-                f = 3;
-                // the two points in double for the fp64 likelihood kernel:
-                // row 2i the proposal, row 2i + 1 the current position
+        for (int i = 0; i < DIM; ++i) y[i] = Y[(size_t)k * DIM + i];
+        lpy = bh_glm_logpost(y, k, K, partial, nranges, lin, scale, prior_prec, corr);
+    }
+#else
+    // the walker's BH_GLM_STPW threads (consecutive lanes) share its sums:
+    // row ranges sub, sub + STPW, ... and coordinates likewise
+    double sm = 0.0, l = 0.0, q = 0.0;
+    for (unsigned r = sub; r < nranges; r += BH_GLM_STPW) sm += partial[(size_t)r * K + k];
+    for (unsigned i = sub; i < DIM; i += BH_GLM_STPW) {
+        const double yi = (double)Y[(size_t)k * DIM + i];
+        l = __builtin_fma(lin[i], yi, l);
+        q = __builtin_fma(yi, yi, q);
+    }
 #pragma unroll
-                for (int j = 0; j < DIM; ++j) {
-                    pts64[(size_t)2 * i * DIM + j] = (double)y[j];
-                    pts64[((size_t)2 * i + 1) * DIM + j] = (double)Xpos[(size_t)k * DIM + j];
+    for (int off = BH_GLM_STPW / 2; off >= 1; off >>= 1) {
+        sm += __shfl_xor(sm, off, 64);
+        l += __shfl_xor(l, off, 64);
+        q += __shfl_xor(q, off, 64);
+    }
+    double lpy = l - scale * sm - 0.5 * prior_prec * q;
+    if (corr) lpy += corr[k];
+#endif
+    // the walker's first thread decides or lists it; all its threads stage
+    // the listed points
+    unsigned slot = 0xffffffffu;
+    if (sub == 0) {
+        lpy_out[k] = lpy;
+        unsigned f = 0;
+        if (__builtin_isfinite(lpy)) {
+            const double m = log((double)u2buf[k]) - (double)(DIM - 1) * log((double)zbuf[k]) -
+                             (double)beta * (lpy - lp[k]);
+            if (__builtin_fabs(m) <= guard) {
+                const unsigned i = atomicAdd(count, 1u);
+                if (i < lmax) {
+                    list[i] = k;
+                    f = 3;
+                    slot = i;
                 }
             }
         }
+        // walkers not listed are decided here, on this precision's values
+        // (1 reject, 2 accept; bh_glm_resolve decides the listed ones)
+        if (f == 0) f = bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta) ? 2u : 1u;
+        flag[k] = f;
     }
-    // walkers not listed are decided here, on this precision's values
-    // (1 reject, 2 accept; bh_glm_resolve decides the listed ones)
-    if (f == 0) f = bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta) ? 2u : 1u;
-    flag[k] = f;
+    slot = __shfl(slot, base, 64);
+    if (slot != 0xffffffffu) {
+        // the two points in double for the fp64 likelihood kernel: row 2i
+        // the proposal, row 2i + 1 the current position
+        for (unsigned j = sub; j < DIM; j += BH_GLM_STPW) {
+            pts64[(size_t)2 * slot * DIM + j] = (double)Y[(size_t)k * DIM + j];
+            pts64[((size_t)2 * slot + 1) * DIM + j] = (double)Xpos[(size_t)k * DIM + j];
+        }
+    }
 }
 
 // Decides the listed walkers on their fp64 log posteriors, one wave per
