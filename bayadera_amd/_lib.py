@@ -34,6 +34,7 @@ BH_GLM_FP32 = 1
 BH_GLM_BF16 = 2
 BH_GLM_BF16X3 = 3
 BH_GLM_FP16X2 = 4
+BH_GLM_FP16 = 5
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32

@@ -274,8 +274,9 @@ struct GlmData {
     DevBuf lin;               // logistic: double[d]: X'y; linear regression: the
                               // four constants of its log posterior
     DevBuf lin_half;          // logistic: double[d]: X'y - (1/2) X'1 (log2-unit kernels)
+    DevBuf half_colsum;       // logistic: double[d]: (1/2) X'1
     DevBuf X16;               // logistic: fp16 copy of X in fragment order, made on first fp16 use
-    mutable unsigned bpc[BH_GLM_FP16X2 + 1] = {};  // resident work-groups per CU, by precision
+    mutable unsigned bpc[BH_GLM_FP16 + 1] = {};  // resident work-groups per CU, by precision
     DevBuf Xf, y;             // linear regression: fp32 N x D and N
 };
 
@@ -295,7 +296,7 @@ int glm_contract(const GlmData& g, int prec, hipStream_t st, const float* P, uns
         BH_HIP(bh::launch_glm_gauss(prec, st, g.Xf.as<float>(), g.y.as<float>(), g.N, g.D, P,
                                     pstride, g.d, n, nranges, rpr, partial));
     } else {
-        const void* X = prec == BH_GLM_FP16X2 ? g.X16.p : g.X.p;
+        const void* X = (prec == BH_GLM_FP16X2 || prec == BH_GLM_FP16) ? g.X16.p : g.X.p;
         BH_HIP(bh::launch_glm_logistic(prec, st, X, g.N, g.d, P, ys, n, nranges, rpr, yscale,
                                        partial));
     }
@@ -364,6 +365,11 @@ struct bh_stretch {
     // point's correction
     DevBuf gg0, ggpart, gcorr;
     int g0_age = -1;  // steps since gg0 was computed; -1: not valid
+    // the one-term fp16 precision's second-order correction: the Hessian at
+    // the same reference point (d x d floats) and its work
+    DevBuf gh0, ghpart;
+    bool h0_valid = false;
+    DevBuf gscale;  // per-walker widening of the guard (second order)
     // event pairs around the dominant kernel's launches (the stretch move
     // kernels, or the GLM likelihood contraction), up to the pool's size
     bool ktiming = false;
@@ -500,33 +506,57 @@ constexpr unsigned kGlmScreenThreads = 8;
 constexpr int kGlmGradRefresh = 4;
 
 int glm_nsplit(int prec) {
-    return prec == BH_GLM_BF16X3 ? 3 : (prec == BH_GLM_FP16X2 ? 2 : (prec == BH_GLM_BF16 ? 1 : 0));
+    return prec == BH_GLM_BF16X3 ? 3
+           : prec == BH_GLM_FP16X2 ? 2
+           : (prec == BH_GLM_BF16 || prec == BH_GLM_FP16) ? 1
+                                                           : 0;
 }
-int glm_f16(int prec) { return prec == BH_GLM_FP16X2 ? 1 : 0; }
+int glm_f16(int prec) { return (prec == BH_GLM_FP16X2 || prec == BH_GLM_FP16) ? 1 : 0; }
+// the one-term fp16 precision's correction goes to second order
+bool glm_second_order(int prec) { return prec == BH_GLM_FP16; }
 
-// The gradient g0 of the logistic softplus sum at the mean of the first
-// min(n, 256) points of P (device, row stride d), for the split precisions'
-// first-order correction; nullptr (no correction) for the other precisions
-// and models.
+// For the split precisions' correction: g0, the gradient at the mean b of
+// the first min(n, 256) points of P (device, row stride d) of the part of
+// the logistic softplus sum the kernels evaluate at the split coefficients
+// (everything but the linear half (1/2) X'1 . y, which they add at the
+// exact y): sum_r sigma(x_r . b) x_r - (1/2) X'1; for the one-term fp16
+// precision the sum's Hessian H0 at b too.  nullptr (no correction) for the
+// other precisions and models.
 int glm_grad_ref(bh_stretch* s, const float* P, unsigned n, const double** g0) {
     const GlmData& g = *s->f->glm;
     *g0 = nullptr;
     if (g.kind != BH_GLM_LOGISTIC || glm_nsplit(s->glm_prec) == 0 || n == 0) return BH_OK;
     BH_HIP(bh::launch_glm_grad(s->stream, g.X.p, g.N, g.d, P, std::min(n, 256u),
-                               s->ggpart.as<double>(), s->gg0.as<double>()));
+                               g.half_colsum.as<double>(), s->ggpart.as<double>(),
+                               s->gg0.as<double>()));
+    if (glm_second_order(s->glm_prec)) {
+        BH_TRY(s->gh0.alloc((size_t)g.d * g.d * sizeof(float)));
+        BH_TRY(s->ghpart.alloc(bh::glm_hess_work_doubles(g.d) * sizeof(double)));
+        BH_HIP(bh::launch_glm_hess(s->stream, g.X.p, g.N, g.d,
+                                   bh::glm_grad_bref(s->ggpart.as<double>(), g.d),
+                                   s->ghpart.as<double>(), s->gh0.as<float>()));
+        s->h0_valid = true;
+    }
     *g0 = s->gg0.as<double>();
     return BH_OK;
 }
 
+// The second-order correction's H0 and reference point, or nullptrs.
+void glm_hess_args(const bh_stretch* s, const float** H0, const float** bref) {
+    const bool on = glm_second_order(s->glm_prec) && s->h0_valid;
+    *H0 = on ? s->gh0.as<float>() : nullptr;
+    *bref = on ? bh::glm_grad_bref(s->ggpart.as<double>(), s->f->glm->d) : nullptr;
+}
+
 // Checks a precision against the model and makes what it needs: the fp16
-// copy of X for BH_GLM_FP16X2 (refused when a value of X is outside the
-// fp16 range).
+// copy of X for BH_GLM_FP16X2 and BH_GLM_FP16 (refused when a value of X is
+// outside the fp16 range).
 int glm_prepare(bh_stretch_factory* f, int prec) {
-    BH_ARG(prec >= BH_GLM_FP64 && prec <= BH_GLM_FP16X2, "bad precision %d", prec);
+    BH_ARG(prec >= BH_GLM_FP64 && prec <= BH_GLM_FP16, "bad precision %d", prec);
     GlmData& g = *f->glm;
     BH_ARG(g.kind != BH_GLM_GAUSSIAN || prec <= BH_GLM_FP32,
            "linear-regression models run in fp64 or fp32");
-    if (prec != BH_GLM_FP16X2 || g.X16.p) return BH_OK;
+    if (!glm_f16(prec) || g.X16.p) return BH_OK;
     bh_ctx* c = f->ctx;
     BH_HIP(hipSetDevice(c->device));
     DevBuf x16, over;
@@ -579,7 +609,9 @@ int glm_logpost_points(bh_stretch* s, const float* P, unsigned n, double* lp) {
         BH_TRY(glm_grad_ref(s, P, n, &g0));
         double lnscale = kLn2;
         corr = s->gcorr.as<double>();
-        void* a[] = {&P, &n, &ys, &yscale, &nsplit, &f16, &g0, &lnscale, &corr};
+        const float *H0, *bref;
+        glm_hess_args(s, &H0, &bref);
+        void* a[] = {&P, &n, &ys, &yscale, &nsplit, &f16, &g0, &lnscale, &corr, &H0, &bref};
         BH_TRY(launch(f->glm_split, (n + s->wgs - 1) / s->wgs, s->wgs, 0, s->stream, a));
     }
     double* partial = s->gpartial.as<double>();
@@ -627,8 +659,13 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
     }
     double lnscale = kLn2;
     double* corr = nsplit ? s->gcorr.as<double>() : nullptr;
+    const float *H0, *bref;
+    glm_hess_args(s, &H0, &bref);
+    if (!g0) H0 = bref = nullptr;
+    // the second-order precision's per-walker widening of the guard
+    float* gscale = (H0 && guarded) ? s->gscale.as<float>() : nullptr;
     void* pa[] = {&seed, &oe, &S, &Xc, &a, &step, &K, &partner, &Y, &z, &u2, &ys, &yscale, &nsplit,
-                  &f16, &gcount, &g0, &lnscale, &corr};
+                  &f16, &gcount, &g0, &lnscale, &corr, &H0, &bref, &gscale};
     BH_TRY(launch(f->glm_propose, K * kGlmProposeThreads / s->wgs, s->wgs, 0, s->stream, pa));
     unsigned nranges = glm_ranges(s->ctx, g, prec, K);
     const unsigned long long rpr = ((g.N + nranges - 1) / nranges + 15) / 16 * 16;
@@ -658,8 +695,9 @@ int glm_half(bh_stretch* s, bool first, unsigned seed, unsigned oe, unsigned ste
         double* pts64 = s->gpts64.as<double>();
         double* lpy = s->glpy.as<double>();
         const double* cc = corr;
+        const float* gsc = gscale;
         void* sa[] = {&Yc, &zc, &u2c, &pc, &nranges, &lin, &scale, &pp, &lpc, &beta, &K, &guard,
-                      &flag, &list, &gcount, &lmax, &Xc, &pts64, &lpy, &cc};
+                      &flag, &list, &gcount, &lmax, &Xc, &pts64, &lpy, &cc, &gsc};
         BH_TRY(launch(f->glm_screen, K * kGlmScreenThreads / s->wgs, s->wgs, 0, s->stream, sa));
         double* part64 = s->gpart64.as<double>();
         BH_HIP(bh::launch_glm_fp64_list(s->stream, g.X.p, g.N, g.d, pts64, gcount, lmax,
@@ -698,13 +736,14 @@ int glm_alloc(bh_stretch* s, unsigned n_points, unsigned k_half) {
     const GlmData& g = *s->f->glm;
     const size_t dim = s->dim;
     size_t maxr = 0;
-    for (int prec = BH_GLM_FP64; prec <= BH_GLM_FP16X2; ++prec)
+    for (int prec = BH_GLM_FP64; prec <= BH_GLM_FP16; ++prec)
         for (unsigned n : {n_points, k_half})
             if (n && n % bh::glm_point_quantum(prec) == 0)
                 maxr = std::max(maxr, (size_t)glm_ranges(s->ctx, g, prec, n) * n);
     BH_TRY(s->gpartial.alloc(std::max<size_t>(maxr, 1) * sizeof(double)));
     BH_TRY(s->gsplit.alloc((size_t)3 * n_points * dim * sizeof(uint16_t)));
     BH_TRY(s->gcorr.alloc((size_t)std::max(std::max(n_points, k_half), 1u) * sizeof(double)));
+    BH_TRY(s->gscale.alloc((size_t)std::max(k_half, 1u) * sizeof(float)));
     BH_TRY(s->gg0.alloc((size_t)std::max<size_t>(dim, 64) * sizeof(double)));
     BH_TRY(s->ggpart.alloc(bh::glm_grad_work_doubles(std::max<unsigned>(s->dim, 64)) *
                            sizeof(double)));
@@ -1505,8 +1544,14 @@ int bh_glm_stretch_factory_create(bh_ctx* ctx, int kind, const void* X_bf16, con
     BH_HIP(hipMemcpyAsync(colsum.data(), g->lin_half.p, d * sizeof(double),
                           hipMemcpyDeviceToHost, ctx->stream));
     BH_TRY(sync(ctx->stream));
-    for (uint32_t j = 0; j < d; ++j) colsum[j] = lin[j] - 0.5 * colsum[j];
+    std::vector<double> half(d);
+    for (uint32_t j = 0; j < d; ++j) {
+        half[j] = 0.5 * colsum[j];
+        colsum[j] = lin[j] - half[j];
+    }
     BH_HIP(hipMemcpy(g->lin_half.p, colsum.data(), d * sizeof(double), hipMemcpyHostToDevice));
+    BH_TRY(g->half_colsum.alloc((size_t)d * sizeof(double)));
+    BH_HIP(hipMemcpy(g->half_colsum.p, half.data(), d * sizeof(double), hipMemcpyHostToDevice));
     f->glm = std::move(g);
     *out = f.release();
     return BH_OK;
@@ -1720,7 +1765,12 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision) {
     BH_ARG(s, "NULL sampler");
     BH_ARG(s->f->glm, "not a GLM sampler");
     BH_TRY(glm_prepare(s->f, precision));
-    s->glm_prec = precision;
+    if (precision != s->glm_prec) {
+        s->glm_prec = precision;
+        // the next step computes the correction's gradient (and Hessian) afresh
+        s->g0_age = -1;
+        s->h0_valid = false;
+    }
     return BH_OK;
 }
 

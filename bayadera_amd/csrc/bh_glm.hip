@@ -611,10 +611,10 @@ glm_grad_partial_kernel(const __bf16* __restrict__ X, unsigned long long N,
     double mine = 0.0;  // lane j < D of each wave keeps the wave's sum for feature j
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        float v = acc[j];
+        double v = (double)acc[j];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == (unsigned)j) mine = (double)v;
+        if (lane == (unsigned)j) mine = v;
     }
     if (lane < D) red[wave * D + lane] = mine;
     __syncthreads();
@@ -623,12 +623,13 @@ glm_grad_partial_kernel(const __bf16* __restrict__ X, unsigned long long N,
             (red[t] + red[D + t]) + (red[2 * D + t] + red[3 * D + t]);
 }
 
-// g0[j] = sum over the blocks' partials: thread (g, j) sums blocks g, g + G,
-// ... (all its loads in flight), then one thread per feature the G sums in
-// order.
+// g0[j] = sum over the blocks' partials (minus off[j], when given): thread
+// (g, j) sums blocks g, g + G, ... (all its loads in flight), then one
+// thread per feature the G sums in order.
 template <int D>
 __global__ void __launch_bounds__(1024)
-glm_grad_final_kernel(const double* __restrict__ partial, double* __restrict__ g0) {
+glm_grad_final_kernel(const double* __restrict__ partial, const double* __restrict__ off,
+                      double* __restrict__ g0) {
     constexpr unsigned G = 1024 / D, PER = kGlmGradBlocks / G;
     __shared__ double red[1024];
     const unsigned t = threadIdx.x, j = t % D, g = t / D;
@@ -643,12 +644,13 @@ glm_grad_final_kernel(const double* __restrict__ partial, double* __restrict__ g
     if (t < D) {
         double sm = 0.0;
         for (unsigned q = 0; q < G; ++q) sm += red[q * D + t];
-        g0[t] = sm;
+        g0[t] = off ? sm - off[t] : sm;
     }
 }
 
 hipError_t launch_glm_grad(hipStream_t s, const void* X, unsigned long long N, unsigned d,
-                           const float* pts, unsigned nref, double* work, double* g0) {
+                           const float* pts, unsigned nref, const double* off, double* work,
+                           double* g0) {
     if (nref == 0 || nref > 256) return hipErrorInvalidValue;
     static_assert(kGlmGradBlocks % 32 == 0, "the final sum's grouping");
     const unsigned long long rpb = (N + kGlmGradBlocks - 1) / kGlmGradBlocks;
@@ -658,14 +660,135 @@ hipError_t launch_glm_grad(hipStream_t s, const void* X, unsigned long long N, u
     if (d == 64) {
         glm_grad_ref_kernel<64><<<1, 1024, 0, s>>>(pts, nref, bref);
         glm_grad_partial_kernel<64><<<kGlmGradBlocks, 256, 0, s>>>(x, N, bref, rpb, work);
-        glm_grad_final_kernel<64><<<1, 1024, 0, s>>>(work, g0);
+        glm_grad_final_kernel<64><<<1, 1024, 0, s>>>(work, off, g0);
     } else if (d == 32) {
         glm_grad_ref_kernel<32><<<1, 1024, 0, s>>>(pts, nref, bref);
         glm_grad_partial_kernel<32><<<kGlmGradBlocks, 256, 0, s>>>(x, N, bref, rpb, work);
-        glm_grad_final_kernel<32><<<1, 1024, 0, s>>>(work, g0);
+        glm_grad_final_kernel<32><<<1, 1024, 0, s>>>(work, off, g0);
     } else {
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// Hessian of the logistic softplus sum at the reference point b:
+// H0 = sum_r sigma'(x_r . b) x_r x_r^T (natural units), for the second-order
+// correction of the one-term fp16 precision.  A work-group stages RB rows
+// in LDS as floats with their weights sigma'(u_r), and each of its threads
+// accumulates a TI x TI tile of H in fp32 over the work-group's rows (the
+// correction needs H0 to ~1e-3 relative); the tiles are written in double
+// and summed over work-groups in two passes (32 at a time, then the 32-sums
+// in order).
+template <int D>
+__global__ void __launch_bounds__(256)
+glm_hess_partial_kernel(const __bf16* __restrict__ X, unsigned long long N,
+                        const float* __restrict__ bref_g, unsigned long long rows_per_block,
+                        double* __restrict__ partial) {
+    constexpr int TI = D / 16;     // 4 (D = 64) or 2 (D = 32)
+    constexpr int PPR = D / 8;     // 16-byte pieces per row
+    constexpr int RB = 256 / PPR;  // rows per stage: 32 or 64
+    __shared__ float xs[RB][D + 4];
+    __shared__ float ws[RB];
+    __shared__ float bref[D];
+    const unsigned t = threadIdx.x, ti = t / 16, tj = t % 16;
+    if (t < D) bref[t] = bref_g[t];
+    const unsigned long long r0 = (unsigned long long)blockIdx.x * rows_per_block;
+    unsigned long long r1 = r0 + rows_per_block;
+    if (r1 > N) r1 = N;
+    float acc[TI][TI];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TI; ++j) acc[i][j] = 0.0f;
+    for (unsigned long long base = r0; base < r1; base += RB) {
+        __syncthreads();  // the previous stage is consumed (and bref visible)
+        {
+            const unsigned rr = t / PPR, pc = t % PPR;
+            const unsigned long long row = base + rr;
+            float v[8];
+            if (row < r1) {
+                const bf16x8 xv = reinterpret_cast<const bf16x8*>(X + row * D)[pc];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = (float)xv[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xs[rr][pc * 8 + e] = v[e];
+        }
+        __syncthreads();
+        if (t < RB) {
+            float u = 0.0f;
+#pragma unroll
+            for (int j = 0; j < D; ++j) u = __builtin_fmaf(xs[t][j], bref[j], u);
+            const float sg = 1.0f / (1.0f + __expf(-u));
+            ws[t] = base + t < r1 ? sg * (1.0f - sg) : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < RB; ++r) {
+            const float w = ws[r];
+            float a[TI], bb[TI];
+#pragma unroll
+            for (int i = 0; i < TI; ++i) {
+                a[i] = xs[r][ti * TI + i];
+                bb[i] = w * xs[r][tj * TI + i];
+            }
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TI; ++j) acc[i][j] = __builtin_fmaf(a[i], bb[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TI; ++j)
+            partial[(size_t)blockIdx.x * D * D + (ti * TI + i) * D + tj * TI + j] =
+                (double)acc[i][j];
+}
+
+// out[c * DD + e] = sum of in[(c * 32 + q) * DD + e], q < 32 (all loads in
+// flight); with final set, H0[e] = (float) of the sum of the nc 32-sums.
+__global__ void __launch_bounds__(256)
+glm_hess_sum_kernel(const double* __restrict__ in, unsigned dd, double* __restrict__ out) {
+    const unsigned e = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+    if (e >= dd) return;
+    double v[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) v[q] = in[((size_t)c * 32 + q) * dd + e];
+    double sm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) sm += v[q];
+    out[(size_t)c * dd + e] = sm;
+}
+
+__global__ void __launch_bounds__(256)
+glm_hess_final_kernel(const double* __restrict__ in, unsigned nc, unsigned dd,
+                      float* __restrict__ H0) {
+    const unsigned e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= dd) return;
+    double sm = 0.0;
+    for (unsigned c = 0; c < nc; ++c) sm += in[(size_t)c * dd + e];
+    H0[e] = (float)sm;
+}
+
+hipError_t launch_glm_hess(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                           const float* bref, double* work, float* H0) {
+    static_assert(kGlmHessBlocks % 32 == 0, "the sums over work-groups go 32 at a time");
+    const unsigned long long rpb = (N + kGlmHessBlocks - 1) / kGlmHessBlocks;
+    const __bf16* x = static_cast<const __bf16*>(X);
+    const unsigned dd = d * d, nc = kGlmHessBlocks / 32;
+    double* part2 = work + (size_t)kGlmHessBlocks * dd;
+    if (d == 64)
+        glm_hess_partial_kernel<64><<<kGlmHessBlocks, 256, 0, s>>>(x, N, bref, rpb, work);
+    else if (d == 32)
+        glm_hess_partial_kernel<32><<<kGlmHessBlocks, 256, 0, s>>>(x, N, bref, rpb, work);
+    else
+        return hipErrorInvalidValue;
+    glm_hess_sum_kernel<<<dim3((dd + 255) / 256, nc), 256, 0, s>>>(work, dd, part2);
+    glm_hess_final_kernel<<<(dd + 255) / 256, 256, 0, s>>>(part2, nc, dd, H0);
     return hipGetLastError();
 }
 
@@ -729,6 +852,12 @@ static hipError_t launch_d(int precision, hipStream_t s, const void* Xv, unsigne
             X16, (unsigned)N, ys16, n, groups, nranges, (unsigned)rpr, partial);
         break;
     }
+    case kGlmFp16: {
+        const unsigned groups = n / 256;
+        glm_logistic_mfma_kernel<D, 1, _Float16, f16x8><<<groups * nranges, 256, 0, s>>>(
+            X16, (unsigned)N, ys16, n, groups, nranges, (unsigned)rpr, partial);
+        break;
+    }
     case kGlmFp32: {
         const unsigned groups = n / 256;
         glm_logistic_fp32_kernel<D><<<groups * nranges, 256, 0, s>>>(X, N, P, n, groups, nranges,
@@ -768,6 +897,10 @@ unsigned glm_blocks_per_cu(int precision, unsigned d) {
     case kGlmFp16x2:
         k = d == 64 ? (const void*)glm_logistic_mfma_kernel<64, 2, _Float16, f16x8>
                     : (const void*)glm_logistic_mfma_kernel<32, 2, _Float16, f16x8>;
+        break;
+    case kGlmFp16:
+        k = d == 64 ? (const void*)glm_logistic_mfma_kernel<64, 1, _Float16, f16x8>
+                    : (const void*)glm_logistic_mfma_kernel<32, 1, _Float16, f16x8>;
         break;
     case kGlmFp32:
         k = d == 64 ? (const void*)glm_logistic_fp32_kernel<64>

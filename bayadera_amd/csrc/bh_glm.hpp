@@ -10,6 +10,7 @@ constexpr int kGlmFp32 = 1;
 constexpr int kGlmBf16 = 2;
 constexpr int kGlmBf16x3 = 3;
 constexpr int kGlmFp16x2 = 4;
+constexpr int kGlmFp16 = 5;
 
 bool glm_supported_dim(unsigned d);
 // the number of points n of a launch must be a multiple of this
@@ -48,13 +49,29 @@ hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long 
 // multiple of 8.
 // Gradient of the logistic softplus sum at the mean of the first nref
 // (1 .. 256) points of pts (device, row stride d = 32 or 64), natural units,
-// into g0[d] (double); work holds glm_grad_work_doubles(d) doubles.  The
-// split precisions' first-order correction: their log posterior at y plus
-// g0 . (y' - y), y' the coefficients the split represents.
+// minus off[d] when given, into g0[d] (double); work holds
+// glm_grad_work_doubles(d) doubles.  The split precisions' first-order
+// correction: their log posterior at y plus g0 . (y' - y), y' the
+// coefficients the split represents (off: the linear half (1/2) X'1, which
+// the kernels add at the exact y).
 constexpr unsigned kGlmGradBlocks = 512;
 inline size_t glm_grad_work_doubles(unsigned d) { return (size_t)kGlmGradBlocks * d + d; }
 hipError_t launch_glm_grad(hipStream_t s, const void* X, unsigned long long N, unsigned d,
-                           const float* pts, unsigned nref, double* work, double* g0);
+                           const float* pts, unsigned nref, const double* off, double* work,
+                           double* g0);
+// The reference point launch_glm_grad used (d floats inside its work).
+inline const float* glm_grad_bref(const double* work, unsigned d) {
+    return reinterpret_cast<const float*>(work + (size_t)kGlmGradBlocks * d);
+}
+// Hessian of the logistic softplus sum at bref (device, d floats), natural
+// units, into H0 (d x d floats); work holds glm_hess_work_doubles(d)
+// doubles.  The one-term fp16 precision's second-order correction.
+constexpr unsigned kGlmHessBlocks = 1024;
+inline size_t glm_hess_work_doubles(unsigned d) {
+    return (size_t)(kGlmHessBlocks + kGlmHessBlocks / 32) * d * d;
+}
+hipError_t launch_glm_hess(hipStream_t s, const void* X, unsigned long long N, unsigned d,
+                           const float* bref, double* work, float* H0);
 hipError_t launch_glm_fp64_list(hipStream_t s, const void* X, unsigned long long N, unsigned d,
                                 const double* pts, const unsigned* count, unsigned lmax,
                                 unsigned nranges, double* partial64);

@@ -983,15 +983,21 @@ bh_batch_logfn(const unsigned* __restrict__ dims, const unsigned long long* __re
 // Writes yscale*Y as nsplit 16-bit terms (bf16, or fp16 when f16 is set),
 // hi + mid + lo, each the nearest value of what the previous terms leave
 // (exact residuals in fp32).  With g0 (the logistic softplus sum's gradient
-// at a point of the ensemble, natural units) it returns the first-order
-// correction of the log posterior for the split's rounding:
-// g0 . (lnscale * terms - y), lnscale = 1 / yscale in the epilogue's units
-// (0 when not finite).  The split's log posterior plus it is the log
-// posterior at y up to (g(y) - g0) . (lnscale * terms - y).
+// at a reference point b of the ensemble, natural units) it returns the
+// correction of the log posterior for the split's rounding: with y' =
+// lnscale * terms (lnscale = 1 / yscale in the epilogue's units) and
+// dl = y - y', the first-order -g0 . dl, and given H0 (the sum's Hessian at
+// b) also -dl . H0 (y' - b + dl / 2), so that the split's log posterior plus
+// it is the log posterior at y up to terms of third order in (y' - b, dl)
+// (0 when not finite).
 __device__ __forceinline__ double bh_glm_split(const float (&y)[DIM], float yscale, int nsplit,
                                                int f16, void* out, unsigned k, unsigned n,
-                                               const double* __restrict__ g0, double lnscale) {
+                                               const double* __restrict__ g0, double lnscale,
+                                               const float* __restrict__ H0,
+                                               const float* __restrict__ bref) {
     double c = 0.0;
+    double dl[DIM];  // y - y', y' what the terms represent (natural units)
+    float w[DIM];    // y' - b + dl / 2
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         float r = y[i] * yscale;
@@ -1010,14 +1016,27 @@ __device__ __forceinline__ double bh_glm_split(const float (&y)[DIM], float ysca
                 hs += (double)(float)h;
             }
         }
-        if (g0) c = __builtin_fma(g0[i], __builtin_fma(hs, lnscale, -(double)y[i]), c);
+        const double yp = hs * lnscale;
+        dl[i] = (double)y[i] - yp;
+        if (g0) c = __builtin_fma(g0[i], -dl[i], c);
+        if (H0) w[i] = (float)(yp - (double)bref[i] + 0.5 * dl[i]);
+    }
+    if (g0 && H0) {
+        for (int i = 0; i < DIM; ++i) {
+            float v = 0.0f;
+            for (int j = 0; j < DIM; ++j) v = __builtin_fmaf(H0[i * DIM + j], w[j], v);
+            c = __builtin_fma(-dl[i], (double)v, c);
+        }
     }
     return __builtin_isfinite(c) ? c : 0.0;
 }
 
 // Proposals of one half-step: Y (fp32), z and u2 per walker, and the 16-bit
-// split of log2(e)*Y for the MFMA kernels with, given g0, each proposal's
-// correction for it in corr; clears the guard list's count.  BH_GLM_TPW
+// split of log2(e)*Y for the MFMA kernels with, given g0 (and H0, bref), each
+// proposal's correction for it in corr (as bh_glm_split); clears the guard
+// list's count.  Given H0 and gscale, gscale[k] = max(1, q / (2 DIM)), q the
+// larger H0-norm squared of the proposal's and the position's distances
+// from b, by which the screen widens the guard for walker k.  BH_GLM_TPW
 // threads per walker, thread `sub` taking coordinates sub, sub + BH_GLM_TPW,
 // ... (so a walker's threads read and write consecutive coordinates); each
 // draws the walker's uniforms (the same Philox call), and the corrections'
@@ -1029,7 +1048,11 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                float* __restrict__ Y, float* __restrict__ zbuf, float* __restrict__ u2buf,
                void* __restrict__ ysplit, float yscale, int nsplit, int f16,
                unsigned* __restrict__ guard_count, const double* __restrict__ g0, double lnscale,
-               double* __restrict__ corr) {
+               double* __restrict__ corr, const float* __restrict__ H0,
+               const float* __restrict__ bref, float* __restrict__ gscale) {
+    constexpr int CPT = (DIM + BH_GLM_TPW - 1) / BH_GLM_TPW;  // coordinates per thread
+    __shared__ float wsh[WGS / BH_GLM_TPW][DIM];
+    __shared__ float wxs[WGS / BH_GLM_TPW][DIM];
     const unsigned gt = blockIdx.x * WGS + threadIdx.x;
     const unsigned k = gt / BH_GLM_TPW, sub = gt % BH_GLM_TPW;
     if (guard_count && gt == 0) *guard_count = 0u;
@@ -1041,11 +1064,15 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                                                               e);
     const unsigned total = K * DIM;
     double c = 0.0;
-    for (unsigned i = sub; i < DIM; i += BH_GLM_TPW) {
+    double dl[CPT];
+    float wv[CPT], wx[CPT];
+    int m = 0;
+    for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
         unsigned ei = e + i;
         if (partner == BH_PARTNER_FLAT && ei >= total) ei -= total;
         const float xj = S[ei];
-        const float yi = __builtin_fmaf(z, X[(size_t)k * DIM + i] - xj, xj);
+        const float xi = X[(size_t)k * DIM + i];
+        const float yi = __builtin_fmaf(z, xi - xj, xj);
         Y[(size_t)k * DIM + i] = yi;
         if (nsplit > 0) {
             float r = yi * yscale;
@@ -1064,15 +1091,58 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                     hs += (double)(float)h;
                 }
             }
-            if (g0) c = __builtin_fma(g0[i], __builtin_fma(hs, lnscale, -(double)yi), c);
+            const double yp = hs * lnscale;
+            dl[m] = (double)yi - yp;
+            if (g0) c = __builtin_fma(g0[i], -dl[m], c);
+            if (H0) {
+                wv[m] = (float)(yp - (double)bref[i] + 0.5 * dl[m]);
+                wx[m] = xi - bref[i];
+            }
+        }
+    }
+    float qy = 0.0f, qx = 0.0f;
+    if (nsplit > 0 && g0 && H0) {
+        // the second-order term: the walker's y' - b + dl / 2 through LDS,
+        // each thread the rows of H0 of its coordinates; with it the
+        // H0-norms of the proposal's and the position's distances from b
+        const unsigned wl = threadIdx.x / BH_GLM_TPW;
+        m = 0;
+        for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
+            wsh[wl][i] = wv[m];
+            wxs[wl][i] = wx[m];
+        }
+        __syncthreads();
+        m = 0;
+        for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
+            float v = 0.0f, vx = 0.0f;
+#pragma unroll 8
+            for (int j = 0; j < DIM; ++j) {
+                const float h = H0[i * DIM + j];
+                v = __builtin_fmaf(h, wsh[wl][j], v);
+                vx = __builtin_fmaf(h, wxs[wl][j], vx);
+            }
+            c = __builtin_fma(-dl[m], (double)v, c);
+            qy = __builtin_fmaf(wv[m], v, qy);
+            qx = __builtin_fmaf(wx[m], vx, qx);
         }
     }
 #pragma unroll
-    for (int off = BH_GLM_TPW / 2; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    for (int off = BH_GLM_TPW / 2; off >= 1; off >>= 1) {
+        c += __shfl_xor(c, off, 64);
+        qy += __shfl_xor(qy, off, 64);
+        qx += __shfl_xor(qx, off, 64);
+    }
     if (sub == 0) {
         zbuf[k] = z;
         u2buf[k] = u2;
         if (nsplit > 0 && corr) corr[k] = __builtin_isfinite(c) ? c : 0.0;
+        // the guard's scale for this walker: the correction's remainder grows
+        // with the square of the distance from b, which near equilibrium is
+        // about DIM in the H0-norm
+        if (gscale) {
+            const float q = __builtin_fmaxf(qy, qx) / (2.0f * (float)DIM);
+            gscale[k] = q > 1.0f ? q : 1.0f;
+        }
     }
 }
 
@@ -1081,13 +1151,14 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
 extern "C" __global__ void __launch_bounds__(WGS)
 bh_glm_split_points(const float* __restrict__ P, unsigned n, void* __restrict__ ysplit,
                     float yscale, int nsplit, int f16, const double* __restrict__ g0,
-                    double lnscale, double* __restrict__ corr) {
+                    double lnscale, double* __restrict__ corr, const float* __restrict__ H0,
+                    const float* __restrict__ bref) {
     const unsigned k = blockIdx.x * WGS + threadIdx.x;
     if (k >= n) return;
     float y[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) y[i] = P[(size_t)k * DIM + i];
-    const double c = bh_glm_split(y, yscale, nsplit, f16, ysplit, k, n, g0, lnscale);
+    const double c = bh_glm_split(y, yscale, nsplit, f16, ysplit, k, n, g0, lnscale, H0, bref);
     if (corr) corr[k] = c;
 }
 
@@ -1190,10 +1261,11 @@ __device__ __forceinline__ bool bh_glm_decide(double lpy, double lpx, float z, f
 // Guarded accept test of the reduced-precision likelihoods.  The decision is
 // a monotone step in d = lp(Y) - lp(X); a walker whose log margin
 //   m = log u2 - (DIM-1) log z - beta d
-// is within `guard` of zero could be decided differently by an error of that
-// size in d, so it is listed (up to lmax walkers, the rest counting as
-// overflow; the engine passes lmax = K, so none overflow) for an fp64
-// evaluation of lp(Y) and lp(X) and decided on those.
+// is within `guard` of zero (times gscale[k] when given) could be decided
+// differently by an error of that size in d, so it is listed (up to lmax
+// walkers, the rest counting as overflow; the engine passes lmax = K, so
+// none overflow) for an fp64 evaluation of lp(Y) and lp(X) and decided on
+// those.
 // BH_GLM_STPW threads per walker.
 // flag[k]: 0 decided here, 3 listed (bh_glm_resolve turns it into 1 = reject
 // or 2 = accept).
@@ -1206,7 +1278,7 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
               unsigned* __restrict__ flag, unsigned* __restrict__ list,
               unsigned* __restrict__ count, unsigned lmax, const float* __restrict__ Xpos,
               double* __restrict__ pts64, double* __restrict__ lpy_out,
-              const double* __restrict__ corr) {
+              const double* __restrict__ corr, const float* __restrict__ gscale) {
     const unsigned gt = blockIdx.x * WGS + threadIdx.x;
     const unsigned k = gt / BH_GLM_STPW, sub = gt % BH_GLM_STPW;
     const unsigned base = (threadIdx.x & 63u) & ~(unsigned)(BH_GLM_STPW - 1);
@@ -1247,7 +1319,8 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
         if (__builtin_isfinite(lpy)) {
             const double m = log((double)u2buf[k]) - (double)(DIM - 1) * log((double)zbuf[k]) -
                              (double)beta * (lpy - lp[k]);
-            if (__builtin_fabs(m) <= guard) {
+            const double gk = gscale ? guard * (double)gscale[k] : guard;
+            if (__builtin_fabs(m) <= gk) {
                 const unsigned i = atomicAdd(count, 1u);
                 if (i < lmax) {
                     list[i] = k;

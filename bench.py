@@ -8,10 +8,15 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
   half-width sqrt(3) sd around the posterior mode per coordinate (Newton
   steps and the Laplace sd on the host, not timed).  Legs, from the same
   seeds:
-    fp16x2  likelihood contraction on the matrix cores, fp16 copy of X with
-            the proposals split in two fp16 terms (the headline, --c3-mfma),
-    bf16x3  the same on bf16 X with three bf16 terms,
-            both with fp32 accumulation, fp64 row sums and the guarded
+    fp16    likelihood contraction on the matrix cores, fp16 copy of X with
+            the proposals as one fp16 term and the log posterior corrected
+            to second order for that rounding (the softplus sum's gradient
+            and Hessian at a reference point of the ensemble; the headline,
+            --c3-mfma),
+    fp16x2  the same with the proposals split in two fp16 terms,
+    bf16x3  the same on bf16 X with three bf16 terms (both with the
+            first-order correction),
+            all with fp32 accumulation, fp64 row sums and the guarded
             accept test (decisions near the threshold taken on fp64 values),
     fp32    the plain fp32 VALU likelihood kernel,
     fp64    the fp64 likelihood kernel (coupled windows; a whole job with
@@ -100,20 +105,33 @@ C3_BOX_SD = 3.0 ** 0.5
 C3_CAL_SEEDS = (123, 124, 125)
 C3_CAL_RUN = 70000
 C3_CAL_FILE = "glm_logistic_1Mx64_ess_calibration.json"
-# guard of the reduced-precision accept test, in log units: walkers whose
-# accept margin is smaller are decided on fp64 log posteriors.  The measured
-# error of lp(Y) - lp(X) at this size, with the first-order correction of
-# the split's rounding, has sd 1.9e-4 and max 8.7e-4 over 8192 pairs (fp16x2
-# and bf16x3 alike; 1.9e-3 and 8.2e-3 for fp16x2 without the correction);
-# 2e-3 is ten standard deviations.
-C3_GUARD = 2e-3
+# guard of the reduced-precision accept test per precision, in log units:
+# walkers whose accept margin is smaller are decided on fp64 log posteriors.
+# The measured error of lp(Y) - lp(X) at this size over 8192 pairs of a
+# sampled ensemble: fp16 (one term, second-order correction of its
+# rounding) sd 4.8e-4, max 2.1e-3; fp16x2 and bf16x3 (first-order
+# correction) sd 1.9e-4, max 7.5e-4 (fp16x2 without the correction 1.9e-3,
+# 8.2e-3).  Each guard is ten or more standard deviations (fp16's the most,
+# since its remainder is of third order and grows fastest away from the
+# reference point; the engine also widens it per walker by that distance).
+C3_GUARDS = {"fp16": 6e-3, "fp16x2": 2e-3, "bf16x3": 2e-3}
+C3_GUARD = C3_GUARDS["fp16x2"]
 # the likelihood kernel's launches timed with HIP events: every fourth
 C3_KTIME_EVERY = 4
 # matrix-core precisions of the measured leg: (BH_GLM_*, products per
 # matrix-core block, kernel)
 C3_MFMA = {"bf16x3": (3, 3, "glm_logistic_mfma_kernel<64,3,bf16>"),
-           "fp16x2": (4, 2, "glm_logistic_mfma_kernel<64,2,fp16>")}
-C3_MFMA_DEFAULT = "fp16x2"
+           "fp16x2": (4, 2, "glm_logistic_mfma_kernel<64,2,fp16>"),
+           "fp16": (5, 1, "glm_logistic_mfma_kernel<64,1,fp16>")}
+C3_MFMA_DEFAULT = "fp16"
+C3_MFMA_DESC = {
+    "fp16": "MFMA on the fp16 copy of X, proposals as one fp16 term, log posterior corrected "
+            "to second order for that rounding (gradient and Hessian at a reference point)",
+    "fp16x2": "MFMA split-fp16x2, first-order correction of the split's rounding",
+    "bf16x3": "MFMA split-bf16x3, first-order correction of the split's rounding"}
+# the batched GLMs' matrix-core precisions
+C5_MFMA = {k: C3_MFMA[k] for k in ("bf16x3", "fp16x2")}
+C5_MFMA_DEFAULT = "fp16x2"
 # steps of each coupled window against the fp64 sampler
 C3_WINDOW = 60
 
@@ -822,7 +840,7 @@ def main_c5(args):
     dims = m.dims.astype(np.int64)
     series = torch.empty(run * int(dims.sum()), dtype=torch.float32, device="cuda")
     mname = args.c5_mfma
-    mprec, msplit, _ = C3_MFMA[mname]
+    mprec, msplit, _ = C5_MFMA[mname]
     bt, _, _, _, _ = c5_run(fac, m, lim, mprec, 2, 50)  # warm-up
     bt.release()
     torch.cuda.synchronize()
@@ -1187,14 +1205,14 @@ def main_c3(args):
     burn, run = args.c3_burn, args.c3_run
     mname = args.c3_mfma
     mprec, msplit, mkern = C3_MFMA[mname]
-    other = "bf16x3" if mname == "fp16x2" else "fp16x2"
+    mguard = args.c3_guard if args.c3_guard is not None else C3_GUARDS[mname]
+    others = [n for n in ("fp16", "fp16x2", "bf16x3") if n != mname]
+    legs = ([(mname, mprec, mguard)] + [(n, C3_MFMA[n][0], C3_GUARDS[n]) for n in others] +
+            [("fp32", _lib.BH_GLM_FP32, 0.0)])
     # first use of every kernel of the guarded and plain paths (fp16 copy of
     # X, hipRTC modules), outside every timed region
-    c3_timed(sf, mprec, limits, 1, 2, guard=args.c3_guard)
-    c3_timed(sf, C3_MFMA[other][0], limits, 1, 2, guard=args.c3_guard)
-    c3_timed(sf, _lib.BH_GLM_FP32, limits, 1, 2)
-    legs = [(mname, mprec, args.c3_guard), (other, C3_MFMA[other][0], args.c3_guard),
-            ("fp32", _lib.BH_GLM_FP32, 0.0)]
+    for _n, prec, g in legs:
+        c3_timed(sf, prec, limits, 1, 2, guard=g)
     if args.c3_bf16:
         legs.append(("bf16", _lib.BH_GLM_BF16, 0.0))
     timed = {}
@@ -1204,7 +1222,7 @@ def main_c3(args):
     # the efficiency: ESS per walker-step of the MFMA leg's whole job
     print(f"bench: {mname} job ({burn} burn-in + {run} run-sampler steps)", file=sys.stderr,
           flush=True)
-    job = c3_run(sf, mprec, limits, burn, run, guard=args.c3_guard)
+    job = c3_run(sf, mprec, limits, burn, run, guard=mguard)
     js = c3_ess_summary(job)
     ess_ws = js["ess_per_walker_step"]
     ref = r32 = None
@@ -1215,7 +1233,7 @@ def main_c3(args):
         # same pieces)
         acor = fac.acor_engine()
         print(f"bench: {mname} job in pieces", file=sys.stderr, flush=True)
-        jc = c3_long_run(sf, mprec, limits, burn, run, args.c3_guard, 123, acor, chunk=500)
+        jc = c3_long_run(sf, mprec, limits, burn, run, mguard, 123, acor, chunk=500)
         print("bench: fp64 job", file=sys.stderr, flush=True)
         ref = c3_long_run(sf, _lib.BH_GLM_FP64, limits, burn, run, 0.0, 123, acor, chunk=500)
         print("bench: plain fp32 job", file=sys.stderr, flush=True)
@@ -1312,8 +1330,8 @@ def main_c3(args):
                                f"calibration) x walker-steps/s of {K} timed move! steps after "
                                f"{W} warm-up steps; a {burn} burn-in + {run} run-sampler job "
                                f"in the same run",
-                   "likelihood": f"MFMA split-{mname}, fp32 accumulate, fp64 row sums, "
-                                 f"guarded accept test ({args.c3_guard})",
+                   "likelihood": f"{C3_MFMA_DESC[mname]}; fp32 accumulate, fp64 row sums, "
+                                 f"guarded accept test ({mguard})",
                    "partner": "walker"},
         "walker_steps_per_s": out[mname]["walker_steps_per_s"],
         "mfma_vs_fp32_walker_steps_per_s": round(ws_ratio, 2),
@@ -1385,13 +1403,14 @@ def main():
     ap.add_argument("--c5-burn", type=int, default=C5_BURN)
     ap.add_argument("--c5-run", type=int, default=C5_RUN)
     ap.add_argument("--c5-fp32", action="store_true", help="also the plain fp32 batched run")
-    ap.add_argument("--c5-mfma", choices=sorted(C3_MFMA), default=C3_MFMA_DEFAULT,
+    ap.add_argument("--c5-mfma", choices=sorted(C5_MFMA), default=C5_MFMA_DEFAULT,
                     help="matrix-core precision of the batched run")
     ap.add_argument("--c4-burn", type=int, default=C4_BURN)
     ap.add_argument("--c4-run", type=int, default=C4_RUN)
     ap.add_argument("--c3-burn", type=int, default=C3_BURN)
     ap.add_argument("--c3-run", type=int, default=C3_RUN)
-    ap.add_argument("--c3-guard", type=float, default=C3_GUARD)
+    ap.add_argument("--c3-guard", type=float, default=None,
+                    help="guard of the headline leg (default: its precision's, C3_GUARDS)")
     ap.add_argument("--c3-mfma", choices=sorted(C3_MFMA), default=C3_MFMA_DEFAULT,
                     help="matrix-core precision of the measured leg")
     ap.add_argument("--c3-ess-calibration", action="store_true",

@@ -59,6 +59,12 @@ extern "C" {
 #define BH_GLM_FP16X2 4     /* MFMA on an fp16 copy of X, proposals split in two fp16 terms
                                (fp32-accurate; refused when |x| > 65504 somewhere; values
                                below 2^-17 in magnitude keep an absolute error <= 2^-25) */
+#define BH_GLM_FP16 5       /* MFMA on the fp16 copy of X, proposals as one fp16 term, the
+                               log posterior corrected to second order for that rounding
+                               (the softplus sum's gradient and Hessian at a reference point
+                               of the ensemble); single samplers only */
+/* The split precisions (BF16, BF16X3, FP16X2) carry the first-order part of
+ * that correction (the gradient term). */
 
 typedef struct bh_ctx bh_ctx;
 typedef struct bh_dist_engine bh_dist_engine;
