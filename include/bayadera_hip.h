@@ -255,7 +255,10 @@ int bh_stretch_set_glm_precision(bh_stretch* s, int precision);
  * below the guard, every decision equals the fp64 sampler's.  guard = 0
  * (the default) turns it off.  Every walker inside the guard is
  * re-evaluated (the list holds a whole half-ensemble), so no decision inside
- * the guard falls back to the reduced precision. */
+ * the guard falls back to the reduced precision.  At BH_GLM_FP16 the
+ * guard is widened per walker by max(1, q / (2 DIM)), q the larger squared
+ * H0-norm of Y's and X's distances from the correction's reference point
+ * (its remainder grows with that distance). */
 int bh_stretch_set_glm_guard(bh_stretch* s, double guard);
 /* Walkers re-evaluated in fp64, summed over all guarded half-steps of the
  * sampler; overflow counts listed walkers left to the reduced precision,
@@ -386,7 +389,8 @@ int bh_batch_last_gpu_ms(const bh_batch* b, double* ms);
  * Each problem runs the single GLM sampler's moves (move-bare! arithmetic,
  * amd-gcn-mcmc-stretch.cl:31-81) with whole-walker partners and seeds
  * seed0 + 2p / seed0 + 2p + 1 for its two halves.  walkers: a multiple of
- * 512.  Precision as bh_stretch_set_glm_precision (BH_GLM_*). */
+ * 512.  Precision as bh_stretch_set_glm_precision (BH_GLM_* up to
+ * BH_GLM_FP16X2; the batch does not take BH_GLM_FP16). */
 typedef struct bh_glm_batch bh_glm_batch;
 int bh_glm_batch_create(bh_ctx* ctx, int kind, uint32_t problems, const uint64_t* rows,
                         const uint32_t* dims, const void* X_fp8, const float* col_scales,
