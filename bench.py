@@ -27,7 +27,7 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
   ESS_d = Var_d(final ensemble) / sigma_d^2 with sigma_d from the engine's
   acor of the run-sampler means (Sokal's windowed estimate where acor gives
   no value), ESS per walker-step = min_d ESS_d / (W x run steps), measured
-  on guarded runs of 70000 steps (about 40 tau_max) from three seeds
+  on guarded runs of 70000 steps (about 40 tau_max) from six seeds
   (--c3-ess-calibration, kept under profiles/); one job of the headline leg
   (500 burn-in + 12000 run-sampler steps) in every run reports its own
   (shorter, noisier) estimate beside it.  value = ESS per walker-step x the
@@ -102,7 +102,7 @@ C3_PRIOR_SD = 5.0
 C3_BOX_SD = 3.0 ** 0.5
 # ESS calibration (--c3-ess-calibration): seeds and run length (about 40
 # tau_max)
-C3_CAL_SEEDS = (123, 124, 125)
+C3_CAL_SEEDS = (123, 124, 125, 126, 127, 128)
 C3_CAL_RUN = 70000
 C3_CAL_FILE = "glm_logistic_1Mx64_ess_calibration.json"
 # guard of the reduced-precision accept test per precision, in log units:
@@ -1141,27 +1141,39 @@ def c3_long_run(sf, prec, limits, burn, run, guard, seed, acor, chunk=10000):
             "xs": xs, "acc": float(np.sum(accs) / run), "guard": gs, "burn": burn, "run": run}
 
 
-def c3_calibration():
+def c3_calibration(seeds, mname):
     """ESS per walker-step of the logistic workload, measured on long runs:
-    for each of three seeds, a guarded fp16x2 job of 500 burn-in +
-    C3_CAL_RUN run-sampler steps (about 40 tau_max), and, from the first
-    C3_RUN steps of the same series, the estimate a job of the bench's length
-    gives.  (The guarded chain is the fp64 chain step for step; the plain
-    fp32 chain is the same Markov kernel up to rounding, so its ESS per
-    walker-step is the same quantity.)"""
+    for each seed, a guarded job of 500 burn-in + C3_CAL_RUN run-sampler
+    steps (about 40 tau_max) on the matrix-core precision `mname` at its
+    guard, and, from the first C3_RUN steps of the same series, the estimate
+    a job of the bench's length gives.  The guarded chain is the fp64 chain
+    step for step at every precision, so runs of different precisions
+    measure one quantity; runs of other seeds already in the calibration
+    file (same run length) are kept, and the summary covers all of them.
+    (The plain fp32 chain is the same Markov kernel up to rounding, so its
+    ESS per walker-step is the same quantity.)"""
     from bayadera_amd import _lib, engine, models
     Xb, Xd, y, _, _, limits = c3_problem()
     fac = engine.HipBayaderaFactory(0, 256)
     sf = fac.mcmc_factory(models.glm_logistic_model(Xb, y, prior_sd=C3_PRIOR_SD))
     acor = fac.acor_engine()
+    prec, guard = C3_MFMA[mname][0], C3_GUARDS[mname]
+    cmd = ("python bench.py --c3-ess-calibration --c3-cal-seeds "
+           + ",".join(str(x) for x in seeds) + f" --c3-mfma {mname}")
+    kept = []
+    path = os.path.join(ROOT, "profiles", C3_CAL_FILE)
+    if os.path.exists(path):
+        with open(path) as f:
+            old = json.load(f)
+        if old.get("ess_per_walker_step", {}).get("run_steps") == C3_CAL_RUN:
+            kept = [r for r in old["runs"] if r["seed"] not in seeds]
     out = {"workload": "logistic regression 1000000x64, 16384 walkers, a=2, prior N(0,5^2), "
-                       "guarded fp16x2 likelihood (the fp64 chain)",
-           "command": "python bench.py --c3-ess-calibration", "runs": []}
-    for seed in C3_CAL_SEEDS:
-        print(f"calibration: seed {seed}, {C3_BURN} + {C3_CAL_RUN} steps", file=sys.stderr,
-              flush=True)
-        r = c3_long_run(sf, _lib.BH_GLM_FP16X2, limits, C3_BURN, C3_CAL_RUN, C3_GUARD, seed,
-                        acor)
+                       "guarded matrix-core likelihood (the fp64 chain)",
+           "runs": kept}
+    for seed in seeds:
+        print(f"calibration: seed {seed}, {mname}, {C3_BURN} + {C3_CAL_RUN} steps",
+              file=sys.stderr, flush=True)
+        r = c3_long_run(sf, prec, limits, C3_BURN, C3_CAL_RUN, guard, seed, acor)
         full = c3_ess_summary(r)
         full.pop("ess_per_s_job")
         full.pop("job_ms_per_step_run")
@@ -1171,12 +1183,15 @@ def c3_calibration():
         prefix = c3_ess_summary(r_pre)
         for k in ("ess_per_s_job", "job_ms_per_step_run", "job_wall_s", "job_steps"):
             prefix.pop(k)
-        out["runs"].append({"seed": seed, "full": full, f"first_{C3_RUN}_steps": prefix,
+        out["runs"].append({"seed": seed, "precision": mname, "guard": guard, "command": cmd,
+                            "full": full, f"first_{C3_RUN}_steps": prefix,
                             "guard_overflow": r["guard"]["overflow"]})
         print(json.dumps(out["runs"][-1]), file=sys.stderr, flush=True)
+    out["runs"].sort(key=lambda x: x["seed"])
     e = np.array([x["full"]["ess_per_walker_step"] for x in out["runs"]])
     ep = np.array([x[f"first_{C3_RUN}_steps"]["ess_per_walker_step"] for x in out["runs"]])
     out["ess_per_walker_step"] = {"mean": float(e.mean()), "sd": float(e.std(ddof=1)),
+                                  "sem": float(e.std(ddof=1) / np.sqrt(len(e))),
                                   "run_steps": C3_CAL_RUN, "seeds": len(e)}
     out[f"ess_per_walker_step_first_{C3_RUN}"] = {"mean": float(ep.mean()),
                                                   "sd": float(ep.std(ddof=1))}
@@ -1415,7 +1430,10 @@ def main():
                     help="matrix-core precision of the measured leg")
     ap.add_argument("--c3-ess-calibration", action="store_true",
                     help="only the ESS-per-walker-step calibration of the logistic workload "
-                         "(three seeds; about 20 minutes)")
+                         "(about 4 minutes per seed on fp16); prints the calibration file's "
+                         "runs of other seeds merged with these")
+    ap.add_argument("--c3-cal-seeds", default=",".join(str(x) for x in C3_CAL_SEEDS),
+                    help="comma-separated seeds of the calibration runs")
     ap.add_argument("--c3-bf16", action="store_true", help="also the single-bf16 likelihood")
     ap.add_argument("--no-fp64", action="store_true", help="skip the coupled fp64 windows")
     ap.add_argument("--c3-fp64-full", action="store_true",
@@ -1458,7 +1476,8 @@ def main():
                           "calibration_copy_bytes": src.numel() * 4}), flush=True)
         return
     if args.c3_ess_calibration:
-        return c3_calibration()
+        seeds = tuple(int(x) for x in args.c3_cal_seeds.split(","))
+        return c3_calibration(seeds, args.c3_mfma)
     if args.workload == "c3":
         return main_c3(args)
     if args.workload == "c2":
