@@ -1053,6 +1053,9 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
     constexpr int CPT = (DIM + BH_GLM_TPW - 1) / BH_GLM_TPW;  // coordinates per thread
     __shared__ float wsh[WGS / BH_GLM_TPW][DIM];
     __shared__ float wxs[WGS / BH_GLM_TPW][DIM];
+    // H0 rows padded to DIM + 1 floats: the 16 rows a wave reads at once
+    // fall in different banks
+    __shared__ float hsh[DIM * (DIM + 1)];
     const unsigned gt = blockIdx.x * WGS + threadIdx.x;
     const unsigned k = gt / BH_GLM_TPW, sub = gt % BH_GLM_TPW;
     if (guard_count && gt == 0) *guard_count = 0u;
@@ -1103,9 +1106,12 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
     float qy = 0.0f, qx = 0.0f;
     if (nsplit > 0 && g0 && H0) {
         // the second-order term: the walker's y' - b + dl / 2 through LDS,
-        // each thread the rows of H0 of its coordinates; with it the
-        // H0-norms of the proposal's and the position's distances from b
+        // each thread the rows of H0 of its coordinates (H0 staged in LDS
+        // once per work-group); with it the H0-norms of the proposal's and
+        // the position's distances from b
         const unsigned wl = threadIdx.x / BH_GLM_TPW;
+        for (unsigned q = threadIdx.x; q < DIM * DIM; q += WGS)
+            hsh[(q / DIM) * (DIM + 1) + q % DIM] = H0[q];
         m = 0;
         for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
             wsh[wl][i] = wv[m];
@@ -1117,7 +1123,7 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
             float v = 0.0f, vx = 0.0f;
 #pragma unroll 8
             for (int j = 0; j < DIM; ++j) {
-                const float h = H0[i * DIM + j];
+                const float h = hsh[i * (DIM + 1) + j];
                 v = __builtin_fmaf(h, wsh[wl][j], v);
                 vx = __builtin_fmaf(h, wxs[wl][j], vx);
             }
@@ -1293,9 +1299,18 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
     }
 #else
     // the walker's BH_GLM_STPW threads (consecutive lanes) share its sums:
-    // row ranges sub, sub + STPW, ... and coordinates likewise
+    // row ranges sub, sub + STPW, ... (in that order, eight loads in
+    // flight) and coordinates likewise
     double sm = 0.0, l = 0.0, q = 0.0;
-    for (unsigned r = sub; r < nranges; r += BH_GLM_STPW) sm += partial[(size_t)r * K + k];
+    unsigned r = sub;
+    for (; r + 7 * BH_GLM_STPW < nranges; r += 8 * BH_GLM_STPW) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = partial[(size_t)(r + e * BH_GLM_STPW) * K + k];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sm += v[e];
+    }
+    for (; r < nranges; r += BH_GLM_STPW) sm += partial[(size_t)r * K + k];
     for (unsigned i = sub; i < DIM; i += BH_GLM_STPW) {
         const double yi = (double)Y[(size_t)k * DIM + i];
         l = __builtin_fma(lin[i], yi, l);
