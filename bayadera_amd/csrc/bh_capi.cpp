@@ -500,10 +500,11 @@ constexpr float kLog2e = 1.44269504088896340736f;
 // BH_GLM_TPW, BH_GLM_STPW)
 constexpr unsigned kGlmProposeThreads = 16;
 constexpr unsigned kGlmScreenThreads = 8;
-// steps between refreshes of the split precisions' correction gradient
-// (the ensemble's reference point moves little in a few steps; the
-// correction needs g0 to ~1e-3 relative)
-constexpr int kGlmGradRefresh = 4;
+// steps between refreshes of the split precisions' correction gradient and
+// Hessian (the ensemble's reference point moves little in 16 steps, and
+// both are always taken at the same point; the correction needs g0 to
+// ~1e-3 relative)
+constexpr int kGlmGradRefresh = 16;
 
 int glm_nsplit(int prec) {
     return prec == BH_GLM_BF16X3 ? 3
