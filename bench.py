@@ -1309,6 +1309,7 @@ def main_c3(args):
             o["guard_listed_per_half_step"] = round(t["guard"]["listed"] / (2 * K), 3)
             o["guard_overflow"] = t["guard"]["overflow"]
         out[name] = o
+    full = None
     if ref is not None:
         # the same job (seeds, steps) on the fp64 kernel: posterior means and
         # SDs of the guarded and plain legs against it, in MC standard errors
@@ -1322,7 +1323,6 @@ def main_c3(args):
                                                   6),
                           "trajectory_equals_fp64": bool(np.array_equal(r["xs"], ref["xs"])),
                           "acceptance_rate": round(r["acc"], 4)}
-        out["fp64_full_job"] = full
     mt, ft = timed[mname], timed["fp32"]
     ws_ratio = mt["walker_steps_per_s"] / ft["walker_steps_per_s"]
     kt = mt["kernel"]
@@ -1372,7 +1372,10 @@ def main_c3(args):
         "setup_s": round(t_setup, 1),
     }
     fj = os.path.join(ROOT, "profiles", "glm_logistic_1Mx64_fp64_full_job.json")
-    if os.path.exists(fj):
+    if full is not None:
+        # this run's whole-job comparison (--c3-fp64-full)
+        line["fp64_full_job"] = dict(full, steps=f"{burn} burn-in + {run} run-sampler")
+    elif os.path.exists(fj):
         with open(fj) as f:
             fjd = json.load(f)
         g = fjd["fp64_full_job"].get(mname, {})
