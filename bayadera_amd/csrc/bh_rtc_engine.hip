@@ -1051,11 +1051,12 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                double* __restrict__ corr, const float* __restrict__ H0,
                const float* __restrict__ bref, float* __restrict__ gscale) {
     constexpr int CPT = (DIM + BH_GLM_TPW - 1) / BH_GLM_TPW;  // coordinates per thread
-    __shared__ float wsh[WGS / BH_GLM_TPW][DIM];
-    __shared__ float wxs[WGS / BH_GLM_TPW][DIM];
-    // H0 rows padded to DIM + 1 floats: the 16 rows a wave reads at once
-    // fall in different banks
-    __shared__ float hsh[DIM * (DIM + 1)];
+    // H0 rows padded to a multiple of 4 floats plus 4 (16-byte reads; the
+    // 16 rows a wave reads at once start in different banks)
+    constexpr int HP = (DIM + 3) / 4 * 4 + 4;
+    __shared__ __attribute__((aligned(16))) float wsh[WGS / BH_GLM_TPW][HP];
+    __shared__ __attribute__((aligned(16))) float wxs[WGS / BH_GLM_TPW][HP];
+    __shared__ __attribute__((aligned(16))) float hsh[DIM * HP];
     const unsigned gt = blockIdx.x * WGS + threadIdx.x;
     const unsigned k = gt / BH_GLM_TPW, sub = gt % BH_GLM_TPW;
     if (guard_count && gt == 0) *guard_count = 0u;
@@ -1110,8 +1111,15 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
         // once per work-group); with it the H0-norms of the proposal's and
         // the position's distances from b
         const unsigned wl = threadIdx.x / BH_GLM_TPW;
-        for (unsigned q = threadIdx.x; q < DIM * DIM; q += WGS)
-            hsh[(q / DIM) * (DIM + 1) + q % DIM] = H0[q];
+        // rows padded with zeros past DIM
+        for (unsigned q = threadIdx.x; q < DIM * HP; q += WGS) {
+            const unsigned r = q / HP, j = q % HP;
+            hsh[q] = j < DIM ? H0[r * DIM + j] : 0.0f;
+        }
+        for (unsigned j = DIM + sub; j < HP; j += BH_GLM_TPW) {
+            wsh[wl][j] = 0.0f;
+            wxs[wl][j] = 0.0f;
+        }
         m = 0;
         for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
             wsh[wl][i] = wv[m];
@@ -1121,11 +1129,20 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
         m = 0;
         for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
             float v = 0.0f, vx = 0.0f;
-#pragma unroll 8
-            for (int j = 0; j < DIM; ++j) {
-                const float h = hsh[i * (DIM + 1) + j];
-                v = __builtin_fmaf(h, wsh[wl][j], v);
-                vx = __builtin_fmaf(h, wxs[wl][j], vx);
+            const float4* hr = reinterpret_cast<const float4*>(hsh + i * HP);
+            const float4* ws = reinterpret_cast<const float4*>(wsh[wl]);
+            const float4* wxr = reinterpret_cast<const float4*>(wxs[wl]);
+#pragma unroll 4
+            for (int j4 = 0; j4 < (DIM + 3) / 4; ++j4) {
+                const float4 h = hr[j4], a = ws[j4], b = wxr[j4];
+                v = __builtin_fmaf(h.x, a.x, v);
+                v = __builtin_fmaf(h.y, a.y, v);
+                v = __builtin_fmaf(h.z, a.z, v);
+                v = __builtin_fmaf(h.w, a.w, v);
+                vx = __builtin_fmaf(h.x, b.x, vx);
+                vx = __builtin_fmaf(h.y, b.y, vx);
+                vx = __builtin_fmaf(h.z, b.z, vx);
+                vx = __builtin_fmaf(h.w, b.w, vx);
             }
             c = __builtin_fma(-dl[m], (double)v, c);
             qy = __builtin_fmaf(wv[m], v, qy);
@@ -1381,6 +1398,57 @@ bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
     }
     const unsigned waves = WGS / 64;
     for (unsigned i = blockIdx.x * waves + threadIdx.x / 64; i < c; i += gridDim.x * waves) {
+#if BH_GLM_KIND != 2
+        // both columns of a range in one 16-byte load (the row stride is
+        // even), eight loads in flight per lane; the coordinates' terms
+        // across the lanes; then one butterfly for all six sums
+        const unsigned lane = threadIdx.x & 63u;
+        double sy = 0.0, sx = 0.0;
+        unsigned r = lane;
+        for (; r + 7 * 64 < nranges64; r += 8 * 64) {
+            double2 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[e] = *reinterpret_cast<const double2*>(partial64 + (size_t)(r + e * 64) * stride +
+                                                         2 * i);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                sy += v[e].x;
+                sx += v[e].y;
+            }
+        }
+        for (; r < nranges64; r += 64) {
+            const double2 v =
+                *reinterpret_cast<const double2*>(partial64 + (size_t)r * stride + 2 * i);
+            sy += v.x;
+            sx += v.y;
+        }
+        const unsigned k = list[i];
+        double ly = 0.0, lx = 0.0, qy = 0.0, qx = 0.0;
+        for (unsigned j = lane; j < DIM; j += 64) {
+            const double yj = (double)Y[(size_t)k * DIM + j];
+            const double xj = (double)X[(size_t)k * DIM + j];
+            ly = __builtin_fma(lin64[j], yj, ly);
+            lx = __builtin_fma(lin64[j], xj, lx);
+            qy = __builtin_fma(yj, yj, qy);
+            qx = __builtin_fma(xj, xj, qx);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            sy += __shfl_xor(sy, off, 64);
+            sx += __shfl_xor(sx, off, 64);
+            ly += __shfl_xor(ly, off, 64);
+            lx += __shfl_xor(lx, off, 64);
+            qy += __shfl_xor(qy, off, 64);
+            qx += __shfl_xor(qx, off, 64);
+        }
+        if (lane == 0) {
+            // the logistic log posterior (bh_glm_logpost_sum, scale 1)
+            const double lpy = ly - sy - 0.5 * prior_prec * qy;
+            const double lpx = lx - sx - 0.5 * prior_prec * qx;
+            flag[k] = bh_glm_decide(lpy, lpx, zbuf[k], u2buf[k], beta) ? 2u : 1u;
+        }
+#else
         const double sy = bh_wave_range_sum(partial64, nranges64, stride, 2 * i);
         const double sx = bh_wave_range_sum(partial64, nranges64, stride, 2 * i + 1);
         if ((threadIdx.x & 63u) == 0) {
@@ -1395,6 +1463,7 @@ bh_glm_resolve(const float* __restrict__ Y, const float* __restrict__ X,
             const double lx = bh_glm_logpost_sum(xk, sx, lin64, 1.0, prior_prec);
             flag[k] = bh_glm_decide(ly, lx, zbuf[k], u2buf[k], beta) ? 2u : 1u;
         }
+#endif
     }
 }
 
