@@ -580,7 +580,11 @@ int glm_prepare(bh_stretch_factory* f, int prec) {
 // last round leaves compute units idle; a few rounds of smaller ranges even
 // out the ranges' finishing times.
 unsigned glm_ranges(const bh_ctx* c, const GlmData& g, int prec, unsigned n) {
-    const unsigned groups = std::max(n / bh::glm_point_quantum(prec), 1u);
+    unsigned groups = std::max(n / bh::glm_point_quantum(prec), 1u);
+    // the linear-regression kernel's work-groups take several times 256
+    // points: more ranges keep the work-group count
+    if (g.kind == BH_GLM_GAUSSIAN)
+        groups = std::max(groups / bh::glm_gauss_points_per_thread(prec, g.D, n), 1u);
     unsigned blocks = 4u * (unsigned)c->cu;
     if (g.kind == BH_GLM_LOGISTIC) {
         if (!g.bpc[prec]) g.bpc[prec] = bh::glm_blocks_per_cu(prec, g.d);

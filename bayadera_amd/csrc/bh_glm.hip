@@ -266,10 +266,11 @@ glm_logistic_fp64_kernel(const __bf16* __restrict__ X, unsigned long long N,
 // range r and point k (coefficients b_k = the first d of its DIM = d + 1
 // coordinates, row stride pstride),
 //   partial[r][k] = sum_{i in r} (y_i - x_i . b_k)^2,
-// X fp32 N x D (columns past d zero), y fp32.  The plain fp32 kernel: one
-// point per thread, tiles of 32 rows of X and y in LDS read as broadcasts,
+// X fp32 N x D (columns past d zero), y fp32.  The plain fp32 kernel: PPT
+// points per thread (their coefficients in registers), tiles of 32 rows of
+// X and y in LDS read as broadcasts, each read feeding every point's fma;
 // fp32 residuals, squares summed four rows at a time into a double.
-template <int D>
+template <int D, int PPT>
 __global__ void __launch_bounds__(256)
 glm_gauss_fp32_kernel(const float* __restrict__ X, const float* __restrict__ yv,
                       unsigned long long N, const float* __restrict__ P, unsigned pstride,
@@ -281,14 +282,21 @@ glm_gauss_fp32_kernel(const float* __restrict__ X, const float* __restrict__ yv,
     unsigned group, range;
     glm_tile(groups, group, range);
     if (range >= nranges) return;
-    const unsigned k = group * 256u + threadIdx.x;
-    float b[D];
+    // PPT points per thread, 256 apart: each broadcast read of a row feeds
+    // every point's fma
+    const unsigned k = group * (256u * PPT) + threadIdx.x;
+    float b[PPT][D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) b[j] = (unsigned)j < d ? P[(size_t)k * pstride + j] : 0.0f;
+    for (int p = 0; p < PPT; ++p)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            b[p][j] = (unsigned)j < d ? P[(size_t)(k + 256u * p) * pstride + j] : 0.0f;
     const unsigned long long r0 = (unsigned long long)range * rows_per_range;
     unsigned long long r1 = r0 + rows_per_range;
     if (r1 > N) r1 = N;
-    double acc = 0.0;
+    double acc[PPT];
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) acc[p] = 0.0;
     for (unsigned long long rb = r0; rb < r1; rb += TR) {
         __syncthreads();
         for (unsigned e = threadIdx.x; e < TR * D / 4; e += 256) {
@@ -300,25 +308,37 @@ glm_gauss_fp32_kernel(const float* __restrict__ X, const float* __restrict__ yv,
         __syncthreads();
         // rows past r1 are zero with y = 0: residual 0
         for (int r = 0; r < TR; r += 4) {
-            float s4 = 0.0f;
+            float s4[PPT];
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) s4[p] = 0.0f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float t = 0.0f;
+                float t[PPT];
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) t[p] = 0.0f;
 #pragma unroll
                 for (int j4 = 0; j4 < D / 4; ++j4) {
                     const float4 x = xs[(r + i) * (D / 4) + j4];
-                    t = __builtin_fmaf(x.x, b[4 * j4 + 0], t);
-                    t = __builtin_fmaf(x.y, b[4 * j4 + 1], t);
-                    t = __builtin_fmaf(x.z, b[4 * j4 + 2], t);
-                    t = __builtin_fmaf(x.w, b[4 * j4 + 3], t);
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p) {
+                        t[p] = __builtin_fmaf(x.x, b[p][4 * j4 + 0], t[p]);
+                        t[p] = __builtin_fmaf(x.y, b[p][4 * j4 + 1], t[p]);
+                        t[p] = __builtin_fmaf(x.z, b[p][4 * j4 + 2], t[p]);
+                        t[p] = __builtin_fmaf(x.w, b[p][4 * j4 + 3], t[p]);
+                    }
                 }
-                const float res = ys[r + i] - t;
-                s4 = __builtin_fmaf(res, res, s4);
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) {
+                    const float res = ys[r + i] - t[p];
+                    s4[p] = __builtin_fmaf(res, res, s4[p]);
+                }
             }
-            acc += (double)s4;
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) acc[p] += (double)s4[p];
         }
     }
-    partial[(size_t)range * n + k] = acc;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) partial[(size_t)range * n + k + 256u * p] = acc[p];
 }
 
 // fp64 reference of the same sums: exact products of fp32 data and
@@ -1027,32 +1047,62 @@ hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long 
     return hipGetLastError();
 }
 
+// Points per thread of the fp32 linear-regression kernel: 2 for up to 32
+// features (4 points at 16 features measured no faster, 2 at 64 spill) when
+// n divides into whole work-groups of 512 points.
+unsigned glm_gauss_points_per_thread(int precision, unsigned D, unsigned n) {
+    if (precision != kGlmFp32) return 1;
+    const unsigned pmax = D <= 32 ? 2 : 1;
+    if (pmax >= 4 && n % 1024 == 0) return 4;
+    if (pmax >= 2 && n % 512 == 0) return 2;
+    return 1;
+}
+
+template <int D>
+hipError_t launch_gauss_d(int precision, hipStream_t s, const float* X, const float* yv,
+                          unsigned long long N, const float* P, unsigned pstride, unsigned d,
+                          unsigned n, unsigned nranges, unsigned long long rows_per_range,
+                          double* partial) {
+    const unsigned groups = n / 256;
+    const unsigned blocks = groups * nranges;
+    if (precision == kGlmFp32) {
+        const unsigned ppt = glm_gauss_points_per_thread(precision, D, n);
+        if (ppt == 4) {
+            if constexpr (D <= 16)
+                glm_gauss_fp32_kernel<D, 4><<<blocks / 4, 256, 0, s>>>(
+                    X, yv, N, P, pstride, d, n, groups / 4, nranges, rows_per_range, partial);
+        } else if (ppt == 2) {
+            if constexpr (D <= 32)
+                glm_gauss_fp32_kernel<D, 2><<<blocks / 2, 256, 0, s>>>(
+                    X, yv, N, P, pstride, d, n, groups / 2, nranges, rows_per_range, partial);
+        } else {
+            glm_gauss_fp32_kernel<D, 1><<<blocks, 256, 0, s>>>(X, yv, N, P, pstride, d, n,
+                                                               groups, nranges, rows_per_range,
+                                                               partial);
+        }
+    } else if (precision == kGlmFp64) {
+        glm_gauss_fp64_kernel<D><<<blocks, 256, 0, s>>>(X, yv, N, P, pstride, d, n, groups,
+                                                        nranges, rows_per_range, partial);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_glm_gauss(int precision, hipStream_t s, const float* X, const float* yv,
                             unsigned long long N, unsigned D, const float* P, unsigned pstride,
                             unsigned d, unsigned n, unsigned nranges,
                             unsigned long long rows_per_range, double* partial) {
-    const unsigned groups = n / 256;
-    const unsigned blocks = groups * nranges;
-#define BH_GAUSS(DD)                                                                           \
-    if (precision == kGlmFp32)                                                                 \
-        glm_gauss_fp32_kernel<DD><<<blocks, 256, 0, s>>>(X, yv, N, P, pstride, d, n, groups,    \
-                                                         nranges, rows_per_range, partial);    \
-    else if (precision == kGlmFp64)                                                            \
-        glm_gauss_fp64_kernel<DD><<<blocks, 256, 0, s>>>(X, yv, N, P, pstride, d, n, groups,    \
-                                                         nranges, rows_per_range, partial);    \
-    else                                                                                       \
-        return hipErrorInvalidValue;
-    if (D == 16) {
-        BH_GAUSS(16)
-    } else if (D == 32) {
-        BH_GAUSS(32)
-    } else if (D == 64) {
-        BH_GAUSS(64)
-    } else {
-        return hipErrorInvalidValue;
-    }
-#undef BH_GAUSS
-    return hipGetLastError();
+    if (D == 16)
+        return launch_gauss_d<16>(precision, s, X, yv, N, P, pstride, d, n, nranges,
+                                  rows_per_range, partial);
+    if (D == 32)
+        return launch_gauss_d<32>(precision, s, X, yv, N, P, pstride, d, n, nranges,
+                                  rows_per_range, partial);
+    if (D == 64)
+        return launch_gauss_d<64>(precision, s, X, yv, N, P, pstride, d, n, nranges,
+                                  rows_per_range, partial);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_glm_lin(hipStream_t s, const void* X, const float* yv, unsigned long long N,
