@@ -1134,15 +1134,15 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
             const float4* wxr = reinterpret_cast<const float4*>(wxs[wl]);
 #pragma unroll 4
             for (int j4 = 0; j4 < (DIM + 3) / 4; ++j4) {
-                const float4 h = hr[j4], a = ws[j4], b = wxr[j4];
-                v = __builtin_fmaf(h.x, a.x, v);
-                v = __builtin_fmaf(h.y, a.y, v);
-                v = __builtin_fmaf(h.z, a.z, v);
-                v = __builtin_fmaf(h.w, a.w, v);
-                vx = __builtin_fmaf(h.x, b.x, vx);
-                vx = __builtin_fmaf(h.y, b.y, vx);
-                vx = __builtin_fmaf(h.z, b.z, vx);
-                vx = __builtin_fmaf(h.w, b.w, vx);
+                const float4 h = hr[j4], wy = ws[j4], wxx = wxr[j4];
+                v = __builtin_fmaf(h.x, wy.x, v);
+                v = __builtin_fmaf(h.y, wy.y, v);
+                v = __builtin_fmaf(h.z, wy.z, v);
+                v = __builtin_fmaf(h.w, wy.w, v);
+                vx = __builtin_fmaf(h.x, wxx.x, vx);
+                vx = __builtin_fmaf(h.y, wxx.y, vx);
+                vx = __builtin_fmaf(h.z, wxx.z, vx);
+                vx = __builtin_fmaf(h.w, wxx.w, vx);
             }
             c = __builtin_fma(-dl[m], (double)v, c);
             qy = __builtin_fmaf(wv[m], v, qy);
