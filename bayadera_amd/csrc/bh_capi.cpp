@@ -581,8 +581,8 @@ int glm_prepare(bh_stretch_factory* f, int prec) {
 // out the ranges' finishing times.
 unsigned glm_ranges(const bh_ctx* c, const GlmData& g, int prec, unsigned n) {
     unsigned groups = std::max(n / bh::glm_point_quantum(prec), 1u);
-    // the linear-regression kernel's work-groups take several times 256
-    // points: more ranges keep the work-group count
+    // the linear-regression kernel's work-groups can take twice 256 points:
+    // twice the ranges keep the work-group count
     if (g.kind == BH_GLM_GAUSSIAN)
         groups = std::max(groups / bh::glm_gauss_points_per_thread(prec, g.D, n), 1u);
     unsigned blocks = 4u * (unsigned)c->cu;

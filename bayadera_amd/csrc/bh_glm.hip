@@ -1051,11 +1051,7 @@ hipError_t launch_glm_x_to_f16(hipStream_t s, const void* X, unsigned long long 
 // features (4 points at 16 features measured no faster, 2 at 64 spill) when
 // n divides into whole work-groups of 512 points.
 unsigned glm_gauss_points_per_thread(int precision, unsigned D, unsigned n) {
-    if (precision != kGlmFp32) return 1;
-    const unsigned pmax = D <= 32 ? 2 : 1;
-    if (pmax >= 4 && n % 1024 == 0) return 4;
-    if (pmax >= 2 && n % 512 == 0) return 2;
-    return 1;
+    return precision == kGlmFp32 && D <= 32 && n % 512 == 0 ? 2u : 1u;
 }
 
 template <int D>
@@ -1066,12 +1062,7 @@ hipError_t launch_gauss_d(int precision, hipStream_t s, const float* X, const fl
     const unsigned groups = n / 256;
     const unsigned blocks = groups * nranges;
     if (precision == kGlmFp32) {
-        const unsigned ppt = glm_gauss_points_per_thread(precision, D, n);
-        if (ppt == 4) {
-            if constexpr (D <= 16)
-                glm_gauss_fp32_kernel<D, 4><<<blocks / 4, 256, 0, s>>>(
-                    X, yv, N, P, pstride, d, n, groups / 4, nranges, rows_per_range, partial);
-        } else if (ppt == 2) {
+        if (glm_gauss_points_per_thread(precision, D, n) == 2) {
             if constexpr (D <= 32)
                 glm_gauss_fp32_kernel<D, 2><<<blocks / 2, 256, 0, s>>>(
                     X, yv, N, P, pstride, d, n, groups / 2, nranges, rows_per_range, partial);

@@ -16,7 +16,7 @@ bool glm_supported_dim(unsigned d);
 // the number of points n of a launch must be a multiple of this
 unsigned glm_point_quantum(int precision);
 // points per thread of the linear-regression likelihood kernel for n points
-// of D padded features (1, 2 or 4); its work-groups take 256 times as many
+// of D padded features (1 or 2); its work-groups take 256 times as many
 unsigned glm_gauss_points_per_thread(int precision, unsigned D, unsigned n);
 // work-groups of the logistic likelihood kernel of a precision that one
 // compute unit holds at once (the device's occupancy for its registers)
