@@ -242,6 +242,30 @@ def test_glm_gaussian_log_density_vs_fp64_and_user_model(fac, N, d):
     de.release()
 
 
+@pytest.mark.parametrize("d", [5, 16, 32])
+def test_glm_gaussian_two_point_kernel_equals_one_point(fac, d):
+    """The fp32 linear-regression kernel takes two points per thread when
+    the point count is a multiple of 512 (each LDS read of a row feeds both
+    points) and one otherwise, with the same arithmetic per point: at 3000
+    rows both calls below take the capped range count, so the log densities
+    of the same 768 points from a 1024-point call (two per thread) and a
+    768-point call (one per thread) are bit-identical, and both agree with
+    the float64 formula."""
+    N = 3000
+    X, y, _ = _linreg(N, d, 41 + d)
+    bhat = np.linalg.lstsq(X.astype(np.float64), y.astype(np.float64), rcond=None)[0]
+    rng = np.random.default_rng(5 + d)
+    pts = np.concatenate([bhat + 0.02 * rng.standard_normal((1024, d)),
+                          np.log(0.5) + 0.05 * rng.standard_normal((1024, 1))],
+                         1).astype(np.float32)
+    sf = fac.mcmc_factory(models.glm_gaussian_model(X, y, 10.0, 1.0))
+    two = sf.log_density(pts, _lib.BH_GLM_FP32)
+    one = sf.log_density(np.ascontiguousarray(pts[:768]), _lib.BH_GLM_FP32)
+    assert np.array_equal(two[:768], one)
+    ref = _linreg_lp64(X, y, pts)
+    assert np.max(np.abs(two - ref) / (np.abs(ref) + 1.0)) < 2e-6
+
+
 def test_glm_gaussian_sampler_moves_and_rejects_guard(fac):
     X, y, pts = _linreg(4000, 16, 41)
     sf = fac.mcmc_factory(models.glm_gaussian_model(X, y))
