@@ -1067,15 +1067,39 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
                          : bh_stretch_draw<BH_PARTNER_WALKER>(k, K, seed, odd_or_even, step, a, z,
                                                               e);
     const unsigned total = K * DIM;
+    // the loads of all the thread's coordinates (partner's and own position,
+    // g0 and the reference point) issued before any of them is used; where
+    // g0 or the reference point is absent the load reads the ensemble S
+    // instead (at least 4 K DIM >= 8 DIM bytes; the value is never used), so
+    // that no load sits in a branch of its own
+    const double* g0p = g0 ? g0 : reinterpret_cast<const double*>(S);
+    const float* brp = H0 ? bref : S;
+    float xjv[CPT], xiv[CPT], brv[CPT];
+    double g0v[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const unsigned i = sub + q * BH_GLM_TPW;
+        xjv[q] = 0.0f;
+        xiv[q] = 0.0f;
+        brv[q] = 0.0f;
+        g0v[q] = 0.0;
+        if (i < DIM) {
+            unsigned ei = e + i;
+            if (partner == BH_PARTNER_FLAT && ei >= total) ei -= total;
+            xjv[q] = S[ei];
+            xiv[q] = X[(size_t)k * DIM + i];
+            g0v[q] = g0p[i];
+            brv[q] = brp[i];
+        }
+    }
+    asm volatile("" ::: "memory");
     double c = 0.0;
     double dl[CPT];
     float wv[CPT], wx[CPT];
     int m = 0;
     for (unsigned i = sub; i < DIM; i += BH_GLM_TPW, ++m) {
-        unsigned ei = e + i;
-        if (partner == BH_PARTNER_FLAT && ei >= total) ei -= total;
-        const float xj = S[ei];
-        const float xi = X[(size_t)k * DIM + i];
+        const float xj = xjv[m];
+        const float xi = xiv[m];
         const float yi = __builtin_fmaf(z, xi - xj, xj);
         Y[(size_t)k * DIM + i] = yi;
         if (nsplit > 0) {
@@ -1097,10 +1121,10 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
             }
             const double yp = hs * lnscale;
             dl[m] = (double)yi - yp;
-            if (g0) c = __builtin_fma(g0[i], -dl[m], c);
+            if (g0) c = __builtin_fma(g0v[m], -dl[m], c);
             if (H0) {
-                wv[m] = (float)(yp - (double)bref[i] + 0.5 * dl[m]);
-                wx[m] = xi - bref[i];
+                wv[m] = (float)(yp - (double)brv[m] + 0.5 * dl[m]);
+                wx[m] = xi - brv[m];
             }
         }
     }
@@ -1111,10 +1135,21 @@ bh_glm_propose(unsigned seed, unsigned odd_or_even, const float* __restrict__ S,
         // once per work-group); with it the H0-norms of the proposal's and
         // the position's distances from b
         const unsigned wl = threadIdx.x / BH_GLM_TPW;
-        // rows padded with zeros past DIM
-        for (unsigned q = threadIdx.x; q < DIM * HP; q += WGS) {
-            const unsigned r = q / HP, j = q % HP;
-            hsh[q] = j < DIM ? H0[r * DIM + j] : 0.0f;
+        // rows padded with zeros past DIM; eight loads in flight per thread
+        constexpr unsigned NH = (DIM * HP + WGS - 1) / WGS;
+        for (unsigned t0 = 0; t0 < NH; t0 += 8) {
+            float hv[8];
+#pragma unroll
+            for (unsigned t = 0; t < 8; ++t) {
+                const unsigned q = threadIdx.x + (t0 + t) * WGS, r = q / HP, j = q % HP;
+                hv[t] = q < DIM * HP && j < DIM ? H0[r * DIM + j] : 0.0f;
+            }
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (unsigned t = 0; t < 8; ++t) {
+                const unsigned q = threadIdx.x + (t0 + t) * WGS;
+                if (q < DIM * HP) hsh[q] = hv[t];
+            }
         }
         for (unsigned j = DIM + sub; j < HP; j += BH_GLM_TPW) {
             wsh[wl][j] = 0.0f;
