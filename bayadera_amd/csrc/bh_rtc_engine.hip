@@ -1352,7 +1352,22 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
 #else
     // the walker's BH_GLM_STPW threads (consecutive lanes) share its sums:
     // row ranges sub, sub + STPW, ... (in that order, eight loads in
-    // flight) and coordinates likewise
+    // flight) and coordinates likewise.  The thread's coordinates and the
+    // walker's scalars are loaded first, all together (where corr or gscale
+    // is absent the load reads lp or zbuf instead; the value is unused).
+    constexpr int SC = (DIM + BH_GLM_STPW - 1) / BH_GLM_STPW;
+    float ycv[SC];
+    double lcv[SC];
+#pragma unroll
+    for (int c = 0; c < SC; ++c) {
+        const unsigned i = sub + c * BH_GLM_STPW;
+        ycv[c] = i < DIM ? Y[(size_t)k * DIM + i] : 0.0f;
+        lcv[c] = i < DIM ? lin[i] : 0.0;
+    }
+    const float u2k = u2buf[k], zk = zbuf[k];
+    const double lpk = lp[k];
+    const double ck = (corr ? corr : lp)[k];
+    const float gsk = (gscale ? gscale : zbuf)[k];
     double sm = 0.0, l = 0.0, q = 0.0;
     unsigned r = sub;
     for (; r + 7 * BH_GLM_STPW < nranges; r += 8 * BH_GLM_STPW) {
@@ -1363,10 +1378,13 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
         for (int e = 0; e < 8; ++e) sm += v[e];
     }
     for (; r < nranges; r += BH_GLM_STPW) sm += partial[(size_t)r * K + k];
-    for (unsigned i = sub; i < DIM; i += BH_GLM_STPW) {
-        const double yi = (double)Y[(size_t)k * DIM + i];
-        l = __builtin_fma(lin[i], yi, l);
-        q = __builtin_fma(yi, yi, q);
+#pragma unroll
+    for (int c = 0; c < SC; ++c) {
+        if (sub + c * BH_GLM_STPW < DIM) {
+            const double yi = (double)ycv[c];
+            l = __builtin_fma(lcv[c], yi, l);
+            q = __builtin_fma(yi, yi, q);
+        }
     }
 #pragma unroll
     for (int off = BH_GLM_STPW / 2; off >= 1; off >>= 1) {
@@ -1375,18 +1393,23 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
         q += __shfl_xor(q, off, 64);
     }
     double lpy = l - scale * sm - 0.5 * prior_prec * q;
-    if (corr) lpy += corr[k];
+    if (corr) lpy += ck;
 #endif
     // the walker's first thread decides or lists it; all its threads stage
     // the listed points
+#if BH_GLM_KIND == 2
+    const float u2k = u2buf[k], zk = zbuf[k];
+    const double lpk = lp[k];
+    const float gsk = gscale ? gscale[k] : 1.0f;
+#endif
     unsigned slot = 0xffffffffu;
     if (sub == 0) {
         lpy_out[k] = lpy;
         unsigned f = 0;
         if (__builtin_isfinite(lpy)) {
-            const double m = log((double)u2buf[k]) - (double)(DIM - 1) * log((double)zbuf[k]) -
-                             (double)beta * (lpy - lp[k]);
-            const double gk = gscale ? guard * (double)gscale[k] : guard;
+            const double m = log((double)u2k) - (double)(DIM - 1) * log((double)zk) -
+                             (double)beta * (lpy - lpk);
+            const double gk = gscale ? guard * (double)gsk : guard;
             if (__builtin_fabs(m) <= gk) {
                 const unsigned i = atomicAdd(count, 1u);
                 if (i < lmax) {
@@ -1398,7 +1421,7 @@ bh_glm_screen(const float* __restrict__ Y, const float* __restrict__ zbuf,
         }
         // walkers not listed are decided here, on this precision's values
         // (1 reject, 2 accept; bh_glm_resolve decides the listed ones)
-        if (f == 0) f = bh_glm_decide(lpy, lp[k], zbuf[k], u2buf[k], beta) ? 2u : 1u;
+        if (f == 0) f = bh_glm_decide(lpy, lpk, zk, u2k, beta) ? 2u : 1u;
         flag[k] = f;
     }
     slot = __shfl(slot, base, 64);
