@@ -27,7 +27,7 @@ Default workload (the metric's comparison, matrix cores vs plain fp32):
   ESS_d = Var_d(final ensemble) / sigma_d^2 with sigma_d from the engine's
   acor of the run-sampler means (Sokal's windowed estimate where acor gives
   no value), ESS per walker-step = min_d ESS_d / (W x run steps), measured
-  on guarded runs of 70000 steps (about 40 tau_max) from nine seeds
+  on guarded runs of 70000 steps (about 40 tau_max) from twelve seeds
   (--c3-ess-calibration, kept under profiles/); one job of the headline leg
   (500 burn-in + 12000 run-sampler steps) in every run reports its own
   (shorter, noisier) estimate beside it.  value = ESS per walker-step x the
@@ -102,7 +102,7 @@ C3_PRIOR_SD = 5.0
 C3_BOX_SD = 3.0 ** 0.5
 # ESS calibration (--c3-ess-calibration): seeds and run length (about 40
 # tau_max)
-C3_CAL_SEEDS = (123, 124, 125, 126, 127, 128, 129, 130, 131)
+C3_CAL_SEEDS = tuple(range(123, 135))
 C3_CAL_RUN = 70000
 C3_CAL_FILE = "glm_logistic_1Mx64_ess_calibration.json"
 # guard of the reduced-precision accept test per precision, in log units:
